@@ -1,0 +1,183 @@
+"""Optical-stage throughput on one MI355X: sites/s for the 4,225-site plain stage.
+
+One step = the reference's optical-stage timing boundary (maomi/fixed.py:337-378):
+phase preparation + beam preparation + propagation of every site + mask assembly,
+with the refractive-index volume already resident in HBM.  Workload: 512 x 512 x 60
+index volume (n0 = 1.35 + Gaussian random field, std 0.02, correlation 3 voxels),
+65 x 65 sites, 256 x 256 complex64 field per site, 60 slices, Zernike pupil, FP32.
+
+Prints one JSON line with the throughput, the roofline of the persistent sites kernel
+(timed with HIP events on its stream) and a same-run CPU baseline (the NumPy/SciPy
+restatement of the reference CPU path, on a bounded sample of sites, extrapolated).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "sites/s on 4225-site optical stage; surrogate vs propagator at ≤1% rel L2"
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(cfg, vol, offsets, zw, n_sites, q_sample, cores):
+    """Time the oracle (complex64 restatement of the reference CPU path) on a sample."""
+    import oracle as O
+    O.set_workers(cores)
+    t0 = time.perf_counter()
+    base, _ = O.plain_beam(cfg, zw)
+    k0dz = 2 * np.pi / cfg.wavelength_um * cfg.dz_um / cfg.voxels_per_slice
+    phase = O.plain_phase(vol, cfg.n0, k0dz, cfg.voxels_per_slice)
+    t_plan = time.perf_counter() - t0
+    rng = np.random.default_rng(cfg.seed + 3)
+    pick = np.sort(rng.choice(len(offsets), min(n_sites, len(offsets)), replace=False))
+    t0 = time.perf_counter()
+    masks = O.plain_stage_masks(base, phase, offsets[pick], wvl=np.float32(cfg.wavelength_um * 1e-6),
+                                nr=np.float32(cfg.n0), dx=np.float32(cfg.dx_um * 1e-6),
+                                dz=cfg.dz_um * 1e-6)
+    t_sites = time.perf_counter() - t0
+    energies = np.full(len(offsets), float(np.sum(masks[0], dtype=np.float64)))
+    # v4: full FP64 solve, query evaluation timed on a sample and scaled to the FOV
+    from oracle.optics_np import _green
+    from scipy.spatial.distance import cdist
+    p = offsets.astype(np.float64)
+    t0 = time.perf_counter()
+    coef = np.linalg.solve(_green(p, p), energies)
+    t_solve = time.perf_counter() - t0
+    q = np.stack(np.meshgrid(np.arange(cfg.fov) + 0.5, np.arange(cfg.fov) + 0.5, indexing="ij"),
+                 -1).reshape(-1, 2)
+    qs = q[:q_sample]
+    t0 = time.perf_counter()
+    chunk = max(1, 8_388_608 // len(p))
+    for s in range(0, len(qs), chunk):
+        d2 = cdist(qs[s:s + chunk], p, "sqeuclidean")
+        with np.errstate(divide="ignore", invalid="ignore"):
+            b = np.log(d2)
+            b *= 0.5
+            b -= 1
+            b *= d2
+        b[d2 == 0] = 0
+        _ = b @ coef
+    t_eval = (time.perf_counter() - t0) * len(q) / len(qs)
+    total = t_plan + t_sites / len(pick) * len(offsets) + t_solve + t_eval
+    return dict(value=len(offsets) / total, unit="sites/s", cores=cores, kind="port",
+                sample=(f"{len(pick)} of {len(offsets)} sites propagated (scipy.fft, {cores} "
+                        f"threads, serial site loop, shifted FFT sequence), full plan "
+                        f"({t_plan:.2f} s) and v4 solve ({t_solve:.2f} s), v4 evaluation on "
+                        f"{len(qs)} of {len(q)} pixels; per-site time "
+                        f"{t_sites / len(pick):.3f} s; extrapolated total {total:.1f} s"))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2))
+    ap.add_argument("--cpu-sites", type=int, default=12)
+    ap.add_argument("--cpu-queries", type=int, default=16384)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "mixed"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the stage shards by sites; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from maomi_amd.plain import PlainStage
+    from maomi_amd.workloads import CONFIG1, CONFIG2, index_volume, site_offsets, zernike_weights
+
+    cfg = CONFIG2 if args.config == 2 else CONFIG1
+    vol_np = index_volume(cfg)
+    offsets = site_offsets(cfg)
+    dev = torch.device("cuda:0")
+    vol = torch.as_tensor(vol_np, device=dev)
+    st = PlainStage(cfg, vol, device=dev, precision=args.precision)
+    print(f"[bench] config {args.config}: {len(offsets)} sites, N={cfg.n}, S={cfg.slices}, "
+          f"{args.precision}", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        st.run()
+        print(f"[bench] warmup {i}: {st.timing}", file=sys.stderr, flush=True)
+
+    # kernel time of the persistent launch, HIP events on the launch stream
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st.plan()
+        ev0.record()
+        st.propagate()
+        ev1.record()
+        st.aggregate()
+        torch.cuda.synchronize()
+        kernel_ms.append(ev0.elapsed_time(ev1))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    P = len(offsets)
+    value = P * args.steps / elapsed
+    kms = float(np.mean(kernel_ms))
+    half = args.precision == "mixed"
+    bytes_launch = cfg.bytes_per_site(half) * P
+    achieved = bytes_launch / (kms * 1e-3) / 1e9
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s; sites kernel {kms:.2f} ms", file=sys.stderr,
+          flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cores = cpu_cores()
+        cpu = cpu_baseline(cfg, vol_np, offsets, zernike_weights(cfg), args.cpu_sites,
+                           args.cpu_queries, cores)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "sites/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 screens, f32 fields" if half else "f32",
+        "data": "synthetic (seeded Gaussian random index field; no downloads)",
+        "config": {
+            "workload": (f"plain optical stage: {cfg.padded - cfg.n}x{cfg.padded - cfg.n}x{cfg.slices} "
+                         f"index volume, {P} sites, {cfg.n}x{cfg.n} complex64 field, "
+                         f"{cfg.slices} slices, Zernike pupil" if cfg.zernike_std_rad > 0 else ""),
+            "sites": P, "field": cfg.n, "slices": cfg.slices, "fov": cfg.fov,
+            "precision": args.precision,
+            "timing_boundary": "plan + propagation + aggregation (fixed.py:337-378)",
+        },
+        "stage_seconds": {k: round(v, 6) for k, v in st.timing.items()},
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+            "kernel": "sites_kernel<%d>" % cfg.n, "kernel_ms": kms,
+            "algorithmic_bytes_per_launch": bytes_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+/*
+ * maomi_amd.h -- C ABI of the MI355X-native MaoMi optical stage.
+ *
+ * All pointers are device pointers (owned by the caller, e.g. torch tensors)
+ * unless noted.  Every function enqueues work on `stream` and returns 0 on
+ * success, a negative MAOMI_E* code when an argument is rejected, or a
+ * positive hipError_t from the runtime.  maomi_last_error() gives the text of
+ * the last failure on the calling thread.  No function allocates device
+ * memory; workspaces are passed in and their sizes are given by the
+ * *_workspace_bytes queries.
+ *
+ * The reference (TomaKakuei/MaoMi 0.8.0b1) has no C ABI: its optical stage is
+ * Python driving CuPy RawKernels and torch.fft.  Each entry point below names
+ * the reference code it replaces.
+ */
+#ifndef MAOMI_AMD_H
+#define MAOMI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* maomi_stream_t; /* a hipStream_t (torch.cuda.current_stream().cuda_stream) */
+
+#define MAOMI_OK 0
+#define MAOMI_EINVAL (-1)
+#define MAOMI_EUNSUPPORTED (-2)
+
+/* Text of the last error on this thread (never NULL). */
+const char* maomi_last_error(void);
+
+/* Library version string. */
+const char* maomi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Plain split-step stage: every site of a run in one persistent launch.
+ *
+ * Replaces the per-batch Python loop of OpticalPlan.evaluate / _stage /
+ * _screens (maomi/backends/cuda/batched.py:148-201) together with the
+ * gather_screens kernel (maomi/backends/cuda/optical_kernels.cu:3-12) and the
+ * cuFFT fft2/ifft2 calls it drives, for a single constant-pitch segment
+ * (numerics.py:156-210 with sc = 1).  Per site:
+ *   u_0 = base;  for s < S: u_s <- u_s * exp(i phi_s(window)) * edge;
+ *   M += trap_s |u_s|^4 ;  u_{s+1} = IFFT2(Q * FFT2(u_s));  M += 1/2 |u_S|^4
+ * with trap_0 = 1/2.  Field sides N in {64,128,256,512}; sites may mix N and S.
+ * ------------------------------------------------------------------------- */
+typedef struct maomi_sites_desc {
+  const void* bases;          /* complex64 base fields, concatenated                     */
+  const int64_t* base_off;    /* [P] complex-element offset of each site's base (device) */
+  const float* phase;         /* [Z][H][W] float32 phase (radians), or NULL              */
+  const void* phase16;        /* [Z][H][W] float16 phase / slice_scale[z], or NULL       */
+  const float* slice_scale;   /* [Z] per-slice scale of phase16                          */
+  const float* edge;          /* concatenated 1-D edge windows (N floats per class)      */
+  const void* q1d;            /* concatenated 1-D Fresnel factors, complex64, FFT order  */
+  const void* twiddle;        /* concatenated W_N^j tables, complex64                    */
+  const int32_t* site_x;      /* [P] window top-left row in the phase volume             */
+  const int32_t* site_y;      /* [P] window top-left column                              */
+  const int32_t* site_n;      /* [P] field side N                                        */
+  const int32_t* site_s;      /* [P] number of screens S (<= Z)                          */
+  const int32_t* order;       /* [P] processing order (a permutation of 0..P-1)          */
+  float* mask;                /* concatenated per-site N*N float32 masks (output)        */
+  const int64_t* mask_off;    /* [P] float offset of each site's mask                    */
+  double* energy;             /* [P] FP64 sum of each mask (output)                      */
+  void* workspace;            /* grid * scratch_stride complex64 elements                */
+  int64_t scratch_stride;     /* complex elements per block (>= max N * N)               */
+  int64_t P, H, W, Z;
+  int64_t grid;               /* persistent blocks (0 = choose)                          */
+  int64_t uniform_n;          /* N shared by every site, or 0 for a mixed-N launch       */
+  int64_t class_edge[5];      /* offsets for N = 64,128,256,512,(unused)                 */
+  int64_t class_q[5];
+  int64_t class_tw[5];
+} maomi_sites_desc;
+
+int maomi_persistent_sites(const maomi_sites_desc* desc, maomi_stream_t stream);
+/* Workspace bytes for `grid` blocks at largest side nmax. */
+int64_t maomi_sites_workspace_bytes(int64_t grid, int64_t nmax);
+/* Default persistent grid for this device (blocks resident at once). */
+int64_t maomi_sites_default_grid(void);
+
+/* ---------------------------------------------------------------------------
+ * Phase preparation from a refractive-index volume.
+ *
+ * Replaces the per-run phase preparation of OpticalPlan.__init__
+ * (batched.py:52-60: exp(1j*resize_xy(raw)*scale)) for index volumes given
+ * per voxel: phase[z][x][y] = k0 * dz * sum_{v<vps} (n[x][y][z*vps+v] - n0),
+ * accumulated in FP64.  `index` is [X][Y][Zv] float32.  Writes float32 phase
+ * (phase32 != NULL) and/or float16 phase with per-slice max-abs scale
+ * (phase16 != NULL; slice_scale receives max|phase_z| / 1, so stored values
+ * are in [-1, 1]).  slice_max must hold Z floats of workspace.
+ * ------------------------------------------------------------------------- */
+int maomi_phase_prepare(const float* index, int64_t X, int64_t Y, int64_t Zv, int64_t vps,
+                        double n0, double k0dz, float* phase32, void* phase16,
+                        float* slice_scale, float* slice_max, maomi_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Beam preparation (once per run, inside the timed "plan" stage).
+ * ------------------------------------------------------------------------- */
+
+/* W_N^j = exp(-2 pi i j / N), j < N, complex64, computed in FP64.  Twiddles of
+ * the FFTs that replace torch.fft.fft2/ifft2 (batched.py:163; numerics.py:200-203). */
+int maomi_twiddles(void* tw, int64_t N, maomi_stream_t stream);
+
+/* Gaussian pupil of generate_ba (maomi/_optical_math.py:121-134, 232-266) on an
+ * N x N grid of pitch D1: exp(-rho2/rad^2) [rho2 < objrad^2] exp(-i k rho2/(2 fl)),
+ * with the Zernike screen of apply_zernike (_optical_math.py:137-197) when nz > 0:
+ * zern_abb[j-1] = weight_j * lambda (metres), Noll j = 1..nz.  zmap (N*N floats)
+ * and dwork (1 double) are workspace when nz > 0. */
+int maomi_pupil_gaussian(void* out, int64_t N, double D1, double rad, double objrad, double k,
+                         double fl, const float* zern_abb, int64_t nz, float* zmap,
+                         double* dwork, maomi_stream_t stream);
+
+/* One Fresnel step of numerics.fresnel (maomi/numerics.py:156-210) for B fields of
+ * N x N (N even), in place: entry chirp and screen0, FFT, Q (scaled when dx1 != dx0),
+ * inverse FFT, screen1, exit chirp.  This is the base-beam step of
+ * OpticalPlan.__init__ (batched.py:43-50).  screen0/screen1 are N x N float32
+ * (or NULL for 1); tw is maomi_twiddles(N). */
+int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, double dx1, double dz,
+                       double wavelength, double nr, const float* screen0, const float* screen1,
+                       const void* tw, maomi_stream_t stream);
+
+/* u <- u / ||u||_2 with the norm accumulated in FP64 (batched.py:51).  dwork: 1 double. */
+int maomi_normalize_l2(void* u, int64_t n, double* dwork, maomi_stream_t stream);
+
+/* Super-Gaussian edge window of edge_apodization (maomi/_propagation.py:59-61):
+ * e[i] = exp(-a_i^16), a_i = (i - N/2)/(0.47 N); e1d (N) and/or e2d = e e^T (N*N). */
+int maomi_edge_window(float* e1d, float* e2d, int64_t N, maomi_stream_t stream);
+
+/* 1-D factor of the constant-pitch Fresnel transfer function in FFT order
+ * (batched.py:68-82): q[k] = exp(-i pi lam_eff dz f_k^2) / N, f_k = m_k/(N dx). */
+int maomi_fresnel_q1d(void* q, int64_t N, double dx, double dz, double lam_eff,
+                      maomi_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Mask assembly (the timed "aggregation" stage), FP64.
+ * ------------------------------------------------------------------------- */
+
+/* Green matrix of v4_interpolate (maomi/numerics.py:122-129): G[i][j] = d^2 (ln d - 1),
+ * d = |p_i - p_j|, G = 0 at d = 0.  points: P x 2 doubles; G: P x P doubles. */
+int maomi_v4_green_matrix_f64(const double* points, int64_t P, double* G, maomi_stream_t stream);
+
+/* Replaces green_apply (maomi/backends/cuda/interpolation.cu:3-19):
+ * out[t][c] = sum_i g(|q_t - p_i|^2) coef[i][c], g(r2) = r2 (ln(r2)/2 - 1), 1 <= C <= 8. */
+int maomi_v4_apply_f64(const double* points, const double* coef, const double* queries,
+                       double* out, int64_t P, int64_t Q, int64_t C, maomi_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAOMI_AMD_H */

@@ -1,0 +1,88 @@
+"""ctypes binding of libmaomi_amd.so (the C ABI declared in include/maomi_amd.h).
+
+The library is loaded lazily on first use; there is no fallback: every GPU
+operation of the package raises if the HIP library is missing or a call fails.
+"""
+
+import ctypes
+from ctypes import c_char_p, c_double, c_int, c_int64, c_void_p
+from pathlib import Path
+
+_PATH = Path(__file__).resolve().with_name("libmaomi_amd.so")
+_LIB = None
+
+
+class SitesDesc(ctypes.Structure):
+    """Mirror of maomi_sites_desc."""
+    _fields_ = [
+        ("bases", c_void_p), ("base_off", c_void_p), ("phase", c_void_p), ("phase16", c_void_p),
+        ("slice_scale", c_void_p), ("edge", c_void_p), ("q1d", c_void_p), ("twiddle", c_void_p),
+        ("site_x", c_void_p), ("site_y", c_void_p), ("site_n", c_void_p), ("site_s", c_void_p),
+        ("order", c_void_p), ("mask", c_void_p), ("mask_off", c_void_p), ("energy", c_void_p),
+        ("workspace", c_void_p), ("scratch_stride", c_int64),
+        ("P", c_int64), ("H", c_int64), ("W", c_int64), ("Z", c_int64),
+        ("grid", c_int64), ("uniform_n", c_int64),
+        ("class_edge", c_int64 * 5), ("class_q", c_int64 * 5), ("class_tw", c_int64 * 5),
+    ]
+
+
+_SIGS = {
+    "maomi_last_error": (c_char_p, []),
+    "maomi_version": (c_char_p, []),
+    "maomi_persistent_sites": (c_int, [ctypes.POINTER(SitesDesc), c_void_p]),
+    "maomi_sites_workspace_bytes": (c_int64, [c_int64, c_int64]),
+    "maomi_sites_default_grid": (c_int64, []),
+    "maomi_phase_prepare": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_double,
+                                    c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "maomi_twiddles": (c_int, [c_void_p, c_int64, c_void_p]),
+    "maomi_pupil_gaussian": (c_int, [c_void_p, c_int64, c_double, c_double, c_double, c_double,
+                                     c_double, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "maomi_fresnel_step": (c_int, [c_void_p, c_int64, c_int64, c_double, c_double, c_double,
+                                   c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "maomi_normalize_l2": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "maomi_edge_window": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "maomi_fresnel_q1d": (c_int, [c_void_p, c_int64, c_double, c_double, c_double, c_void_p]),
+    "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                   c_int64, c_void_p]),
+}
+
+
+def library_path():
+    return _PATH
+
+
+def lib():
+    """Load libmaomi_amd.so and declare every C-ABI signature."""
+    global _LIB
+    if _LIB is None:
+        if not _PATH.exists():
+            raise RuntimeError(f"HIP library not built: {_PATH} is missing "
+                               "(run `python -c 'import __graft_entry__ as g; g.build()'`)")
+        handle = ctypes.CDLL(str(_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().maomi_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed with code {rc}: {msg}")
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,386 @@
+// Persistent per-site split-step propagation for the plain optical stage.
+//
+// One block owns one site at a time and runs all of its slices: the field
+// lives in a block-private workspace (L2/MALL resident), each slice is one
+// row pass and one column pass over LDS tiles, and every pass reads and
+// writes the field exactly once.  The row pass fuses: inverse FFT along y of
+// the previous slice, the screen exp(i phi) * edge gathered straight from the
+// phase volume, |u|^4 trapezoid accumulation into the site's mask, and the
+// forward FFT along y of the next slice.  The column pass fuses the forward
+// FFT along x, the Fresnel factor Q (separable, 1/N^2 folded in) and the
+// inverse FFT along x.  Reference behaviour: maomi/numerics.py:156-210
+// (fresnel, constant pitch), maomi/backends/cuda/batched.py:157-168 (_stage),
+// maomi/optics.py:327-340 (trapezoid over saved planes).
+#include "fft_reg.hpp"
+#include "maomi_amd.h"
+
+namespace {
+
+constexpr int NT = 512;  // threads per block
+
+template <int N> struct ClassIdx;
+template <> struct ClassIdx<64> { static constexpr int v = 0; };
+template <> struct ClassIdx<128> { static constexpr int v = 1; };
+template <> struct ClassIdx<256> { static constexpr int v = 2; };
+template <> struct ClassIdx<512> { static constexpr int v = 3; };
+
+constexpr int kLdsSlots = fftk::cmax(fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2),
+                                     fftk::cmax(fftk::Tile<256, NT>::LDS2, fftk::Tile<512, NT>::LDS2));
+constexpr int kNmax = 512;
+
+struct SitesArgs {
+  const float2* bases;
+  const int64_t* base_off;
+  const float* phase;
+  const __half* phase16;
+  const float* slice_scale;
+  const float* edge;
+  const float2* q1d;
+  const float2* twid;
+  const int32_t* site_x;
+  const int32_t* site_y;
+  const int32_t* site_n;
+  const int32_t* site_s;
+  const int32_t* order;
+  float* mask;
+  const int64_t* mask_off;
+  double* energy;
+  float2* scratch;
+  int64_t scratch_stride;
+  int64_t P, H, W, Z;
+  int64_t cedge[5], cq[5], ctw[5];
+};
+
+struct Smem {
+  float2 x[kLdsSlots];  // line exchange buffer
+  float2 tw[kNmax];     // W_N^j
+  float2 q[kNmax];      // 1-D Fresnel factor (FFT order, times 1/N)
+  float eg[kNmax];      // 1-D edge window
+  double red[NT / 64];
+};
+
+template <int N>
+__device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
+  using T = fftk::Tile<N, NT>;
+  constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
+  constexpr int C = ClassIdx<N>::v;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += NT) {
+    sm.tw[i] = a.twid[a.ctw[C] + i];
+    sm.q[i] = a.q1d[a.cq[C] + i];
+    sm.eg[i] = a.edge[a.cedge[C] + i];
+  }
+  __syncthreads();
+
+  const int S = a.site_s[site];
+  const int64_t ox = a.site_x[site], oy = a.site_y[site];
+  const float2* base = a.bases + a.base_off[site];
+  float* mask = a.mask + a.mask_off[site];
+  float2* fld = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+  const bool half = a.phase16 != nullptr;
+  double esum = 0.0;
+
+  for (int s = 0; s <= S; ++s) {
+    const bool first = (s == 0), last = (s == S);
+    const float wgt = (first || last) ? 0.5f : 1.0f;
+    const float sscale = half && !last ? a.slice_scale[s] : 1.0f;
+    const int64_t pslice = (int64_t)s * a.H;
+
+    // ---------------- row pass: [IFFT_y] -> screen, |u|^4 -> [FFT_y]
+    for (int t = 0; t < N / L; ++t) {
+      const int row0 = t * L;
+      if (!first) {
+        if (tid < L * N1) {  // stage 1 of the inverse transform along y
+          const int l = tid / N1, n1 = tid % N1;
+          const float2* src = fld + (int64_t)(row0 + l) * N + n1;
+          float2 v[N2];
+#pragma unroll
+          for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
+          fftk::fft<N2, true>(v);
+          float2* dst = sm.x + l * LS + n1 * (N2 + 1);
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmulc(v[k2], sm.tw[n1 * k2]);
+        }
+        __syncthreads();
+      }
+      float2 v[N1];
+      const bool p2 = tid < L * N2;
+      const int l = tid / N2, k2 = tid % N2;
+      if (p2) {
+        const int x = row0 + l;
+        if (first) {
+          const float2* src = base + (int64_t)x * N + k2;
+#pragma unroll
+          for (int k1 = 0; k1 < N1; ++k1) v[k1] = src[N2 * k1];
+        } else {
+          const float2* src = sm.x + l * LS + k2;
+#pragma unroll
+          for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[n1 * (N2 + 1)];
+          fftk::fft<N1, true>(v);
+        }
+        const float ex = sm.eg[x];
+        float* mrow = mask + (int64_t)x * N + k2;
+        const int64_t prow = (pslice + ox + x) * a.W + oy + k2;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+          const int y = k2 + N2 * k1;
+          if (!last) {
+            const float phi = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                                   : a.phase[prow + N2 * k1];
+            float sn, cs;
+            sincosf(phi, &sn, &cs);
+            const float sg = ex * sm.eg[y];
+            v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
+          }
+          const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
+          const float m = wgt * (pw * pw);
+          const float acc = first ? m : mrow[N2 * k1] + m;
+          mrow[N2 * k1] = acc;
+          if (last) esum += (double)acc;
+        }
+        if (!last) {  // stage 1 of the forward transform along y (factors swapped)
+          fftk::fft<N1, false>(v);
+#pragma unroll
+          for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
+        }
+      }
+      __syncthreads();
+      if (!last) {
+        if (p2) {
+          float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+#pragma unroll
+          for (int j = 0; j < N1; ++j) dst[j] = v[j];
+        }
+        __syncthreads();
+        if (tid < L * N1) {  // stage 2 of the forward transform along y
+          const int l3 = tid / N1, k2p = tid % N1;
+          float2 w[N2];
+          const float2* src = sm.x + l3 * LS + k2p;
+#pragma unroll
+          for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
+          fftk::fft<N2, false>(w);
+          float2* dst = fld + (int64_t)(row0 + l3) * N + k2p;
+#pragma unroll
+          for (int k1 = 0; k1 < N2; ++k1) dst[N1 * k1] = w[k1];
+        }
+        __syncthreads();
+      }
+    }
+    if (last) break;
+
+    // ---------------- column pass: FFT_x -> * Q -> IFFT_x
+    for (int t = 0; t < N / L; ++t) {
+      const int col0 = t * L;
+      if (tid < L * N1) {
+        const int l = tid % L, n1 = tid / L;
+        const float2* src = fld + (int64_t)n1 * N + col0 + l;
+        float2 v[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[(int64_t)N1 * N * n2];
+        fftk::fft<N2, false>(v);
+        float2* dst = sm.x + l * LS + n1 * (N2 + 1);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmul(v[k2], sm.tw[n1 * k2]);
+      }
+      __syncthreads();
+      float2 v[N1];
+      const bool p2 = tid < L * N2;
+      const int l = tid % L, k2 = tid / L;
+      if (p2) {
+        const float2* src = sm.x + l * LS + k2;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[n1 * (N2 + 1)];
+        fftk::fft<N1, false>(v);
+        const float2 qc = sm.q[col0 + l];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
+        fftk::fft<N1, true>(v);
+#pragma unroll
+        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+      }
+      __syncthreads();
+      if (p2) {
+        float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) dst[j] = v[j];
+      }
+      __syncthreads();
+      if (tid < L * N1) {
+        const int l3 = tid % L, k2p = tid / L;
+        float2 w[N2];
+        const float2* src = sm.x + l3 * LS + k2p;
+#pragma unroll
+        for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
+        fftk::fft<N2, true>(w);
+        float2* dst = fld + (int64_t)k2p * N + col0 + l3;
+#pragma unroll
+        for (int k1 = 0; k1 < N2; ++k1) dst[(int64_t)N1 * N * k1] = w[k1];
+      }
+      __syncthreads();
+    }
+  }
+  const double tot = block_sum_double(esum, sm.red);
+  if (tid == 0) a.energy[site] = tot;
+  __syncthreads();
+}
+
+// One field side for every site of the launch.
+template <int N>
+__global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
+  __shared__ Smem sm;
+  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N>(a, a.order[i], sm);
+}
+
+// Sites of mixed field sides in one launch (the order list puts large sites first).
+__global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
+  __shared__ Smem sm;
+  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
+    const int site = a.order[i];
+    switch (a.site_n[site]) {
+      case 64: run_site<64>(a, site, sm); break;
+      case 128: run_site<128>(a, site, sm); break;
+      case 256: run_site<256>(a, site, sm); break;
+      case 512: run_site<512>(a, site, sm); break;
+      default: break;  // rejected on the host
+    }
+  }
+}
+
+// ---------------------------------------------------------------- phase prep
+__global__ void phase_prepare_kernel(const float* __restrict__ index, int64_t X, int64_t Y,
+                                     int64_t Zv, int64_t vps, double n0, double k0dz,
+                                     float* __restrict__ phase32, float* __restrict__ slice_max) {
+  const int64_t Z = Zv / vps;
+  const int64_t tot = X * Y;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const float* col = index + p * Zv;
+    for (int64_t z = 0; z < Z; ++z) {
+      double acc = 0.0;
+      for (int64_t v = 0; v < vps; ++v) acc += (double)col[z * vps + v] - n0;
+      const float ph = (float)(k0dz * acc);
+      phase32[z * tot + p] = ph;
+      if (slice_max) atomicMax((int*)&slice_max[z], __float_as_int(fabsf(ph)));
+    }
+  }
+}
+
+__global__ void phase_to_half_kernel(const float* __restrict__ phase32, int64_t plane, int64_t Z,
+                                     const float* __restrict__ slice_max,
+                                     __half* __restrict__ out, float* __restrict__ scale) {
+  const int64_t tot = plane * Z;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = i / plane;
+    const float m = slice_max[z];
+    const float inv = m > 0.f ? 1.0f / m : 0.f;
+    out[i] = __float2half_rn(phase32[i] * inv);
+    if (i % plane == 0) scale[z] = m;
+  }
+}
+
+__global__ void fill_float_kernel(float* p, int64_t n, float v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+MAOMI_API int64_t maomi_sites_workspace_bytes(int64_t grid, int64_t nmax) {
+  return grid * nmax * nmax * (int64_t)sizeof(float2);
+}
+
+MAOMI_API int64_t maomi_sites_default_grid(void) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256>, NT, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (int64_t)cus * per_cu;
+}
+
+MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t stream) {
+  if (!d) { maomi_set_error("maomi_persistent_sites: null descriptor"); return MAOMI_EINVAL; }
+  if (d->P <= 0) return MAOMI_OK;
+  if (!d->bases || !d->base_off || !d->edge || !d->q1d || !d->twiddle || !d->site_x ||
+      !d->site_y || !d->site_n || !d->site_s || !d->order || !d->mask || !d->mask_off ||
+      !d->energy || !d->workspace) {
+    maomi_set_error("maomi_persistent_sites: null pointer argument");
+    return MAOMI_EINVAL;
+  }
+  if ((d->phase == nullptr) == (d->phase16 == nullptr)) {
+    maomi_set_error("maomi_persistent_sites: give exactly one of phase / phase16");
+    return MAOMI_EINVAL;
+  }
+  if (d->phase16 && !d->slice_scale) {
+    maomi_set_error("maomi_persistent_sites: phase16 needs slice_scale");
+    return MAOMI_EINVAL;
+  }
+  if (d->scratch_stride < 64 * 64 || d->H <= 0 || d->W <= 0 || d->Z <= 0) {
+    maomi_set_error("maomi_persistent_sites: bad sizes");
+    return MAOMI_EINVAL;
+  }
+  SitesArgs a;
+  a.bases = (const float2*)d->bases; a.base_off = d->base_off;
+  a.phase = d->phase; a.phase16 = (const __half*)d->phase16; a.slice_scale = d->slice_scale;
+  a.edge = d->edge; a.q1d = (const float2*)d->q1d; a.twid = (const float2*)d->twiddle;
+  a.site_x = d->site_x; a.site_y = d->site_y; a.site_n = d->site_n; a.site_s = d->site_s;
+  a.order = d->order; a.mask = d->mask; a.mask_off = d->mask_off; a.energy = d->energy;
+  a.scratch = (float2*)d->workspace; a.scratch_stride = d->scratch_stride;
+  a.P = d->P; a.H = d->H; a.W = d->W; a.Z = d->Z;
+  for (int i = 0; i < 5; ++i) { a.cedge[i] = d->class_edge[i]; a.cq[i] = d->class_q[i]; a.ctw[i] = d->class_tw[i]; }
+  int64_t grid = d->grid > 0 ? d->grid : maomi_sites_default_grid();
+  if (grid <= 0) grid = 256;
+  if (grid > d->P) grid = d->P;
+  const dim3 g((unsigned)grid), b(NT);
+  hipStream_t st = (hipStream_t)stream;
+  switch (d->uniform_n) {
+    case 64: hipLaunchKernelGGL(sites_kernel<64>, g, b, 0, st, a); break;
+    case 128: hipLaunchKernelGGL(sites_kernel<128>, g, b, 0, st, a); break;
+    case 256: hipLaunchKernelGGL(sites_kernel<256>, g, b, 0, st, a); break;
+    case 512: hipLaunchKernelGGL(sites_kernel<512>, g, b, 0, st, a); break;
+    case 0: hipLaunchKernelGGL(sites_mixed_kernel, g, b, 0, st, a); break;
+    default:
+      maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");
+      return MAOMI_EUNSUPPORTED;
+  }
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_phase_prepare(const float* index, int64_t X, int64_t Y, int64_t Zv,
+                                  int64_t vps, double n0, double k0dz, float* phase32,
+                                  void* phase16, float* slice_scale, float* slice_max,
+                                  maomi_stream_t stream) {
+  if (!index || !phase32 || X <= 0 || Y <= 0 || Zv <= 0 || vps <= 0 || Zv % vps) {
+    maomi_set_error("maomi_phase_prepare: bad arguments (phase32 is required as staging)");
+    return MAOMI_EINVAL;
+  }
+  if (phase16 && (!slice_scale || !slice_max)) {
+    maomi_set_error("maomi_phase_prepare: phase16 needs slice_scale and slice_max");
+    return MAOMI_EINVAL;
+  }
+  const int64_t Z = Zv / vps;
+  hipStream_t st = (hipStream_t)stream;
+  if (slice_max) {
+    hipLaunchKernelGGL(fill_float_kernel, dim3(1), dim3(64), 0, st, slice_max, Z, 0.0f);
+    MAOMI_CHECK_LAUNCH();
+  }
+  const int64_t tot = X * Y;
+  const unsigned blocks = (unsigned)((tot + 255) / 256 < 65535 ? (tot + 255) / 256 : 65535);
+  hipLaunchKernelGGL(phase_prepare_kernel, dim3(blocks), dim3(256), 0, st, index, X, Y, Zv, vps,
+                     n0, k0dz, phase32, slice_max);
+  MAOMI_CHECK_LAUNCH();
+  if (phase16) {
+    const int64_t n = tot * Z;
+    const unsigned b2 = (unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535);
+    hipLaunchKernelGGL(phase_to_half_kernel, dim3(b2), dim3(256), 0, st, phase32, tot, Z,
+                       slice_max, (__half*)phase16, slice_scale);
+    MAOMI_CHECK_LAUNCH();
+  }
+  return MAOMI_OK;
+}

@@ -1,0 +1,391 @@
+// Once-per-run beam preparation: pupil field, the scaled Fresnel step that
+// carries it to the top of the tissue, and its unit-norm scaling.  Built with
+// -ffp-contract=off: the chirp arguments reach ~1e4-1e5 rad, where float32
+// rounding of the argument is the dominant error, so every product and sum
+// is rounded in the same order as the reference's float32 expression.
+//
+// Reference: maomi/_optical_math.py:121-134 (gaussian profile), 137-197
+// (Zernike), 232-296 (generate_ba); maomi/numerics.py:156-210 (fresnel, one
+// scaled step); maomi/backends/cuda/batched.py:43-51 (base beam, unit norm).
+#include "common.hpp"
+#include "maomi_amd.h"
+#include <cmath>
+
+namespace {
+
+// ---------------------------------------------------------------- generic line FFT
+// Lines of arbitrary length N = N1 * N2 (each factor up to N) transformed by a
+// two-level direct DFT from LDS.  Used once per run for the base beam and by
+// the segmented stage for sizes that are not powers of two.
+__global__ void twiddle_table_kernel(float2* tw, int N) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+    const double a = 2.0 * M_PI * (double)j / (double)N;
+    tw[j] = make_float2((float)cos(a), (float)(-sin(a)));
+  }
+}
+
+// data: B fields of N x N; axis 1 = rows (contiguous), 0 = columns.
+__global__ void fft_lines_kernel(float2* data, int N, int N1, int N2, int axis, int inverse,
+                                 int L, const float2* __restrict__ tw) {
+  extern __shared__ float2 sh[];
+  float2* A = sh;
+  float2* Bf = sh + (size_t)L * N;
+  float2* T = Bf + (size_t)L * N;
+  const int line0 = blockIdx.x * L;
+  const int nl = min(L, N - line0);
+  float2* f = data + (size_t)blockIdx.y * N * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const float2 w = tw[i];
+    T[i] = inverse ? make_float2(w.x, -w.y) : w;
+  }
+  for (int it = threadIdx.x; it < nl * N; it += blockDim.x) {
+    int l, i;
+    if (axis == 1) { l = it / N; i = it % N; }
+    else { i = it / nl; l = it % nl; }
+    const size_t g = axis == 1 ? (size_t)(line0 + l) * N + i : (size_t)i * N + line0 + l;
+    A[l * N + i] = f[g];
+  }
+  __syncthreads();
+  const int s1 = N / N2, s2 = N / N1;  // W_N2 = W_N^{N1}, W_N1 = W_N^{N2}
+  for (int it = threadIdx.x; it < nl * N; it += blockDim.x) {
+    const int l = it / N, r = it % N, n1 = r / N2, k2 = r % N2;
+    const float2* a = A + l * N + n1;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int n2 = 0; n2 < N2; ++n2) acc = cadd(acc, cmul(a[N1 * n2], T[((n2 * k2) % N2) * s1]));
+    Bf[l * N + n1 * N2 + k2] = cmul(acc, T[(n1 * k2) % N]);
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < nl * N; it += blockDim.x) {
+    int l, k;
+    if (axis == 1) { l = it / N; k = it % N; }
+    else { k = it / nl; l = it % nl; }
+    const int k2 = k % N2, k1 = k / N2;
+    const float2* b = Bf + l * N + k2;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int n1 = 0; n1 < N1; ++n1) acc = cadd(acc, cmul(b[n1 * N2], T[((n1 * k1) % N1) * s2]));
+    const size_t g = axis == 1 ? (size_t)(line0 + l) * N + k : (size_t)k * N + line0 + l;
+    f[g] = acc;
+  }
+}
+
+// ---------------------------------------------------------------- pupil
+// generate_ba, gaussian profile: U = exp(-(X^2+Y^2)/rad^2) [rho2 < objrad^2]
+// exp(-i k rho2 / (2 fl)) with Y, X = meshgrid(y, x, 'ij'), x_j = (j - N/2) D1.
+__global__ void pupil_kernel(float2* out, float* zmap, int N, float D1, float rad2, float obj2,
+                             float ck, float objrad, const float* __restrict__ zw, int nz) {
+  const int tot = N * N;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tot; p += gridDim.x * blockDim.x) {
+    const int i = p / N, j = p % N;
+    const float X = (float)(j - N / 2) * D1;
+    const float Y = (float)(i - N / 2) * D1;
+    const float rho2 = X * X + Y * Y;
+    float U = expf(-(X * X + Y * Y) / rad2);
+    U = rho2 < obj2 ? U : 0.0f;
+    const float th = ck * rho2;
+    float s, c;
+    sincosf(th, &s, &c);
+    out[p] = make_float2(U * c, U * s);
+    if (nz > 0) {
+      const float x = X / objrad, y = Y / objrad;
+      const float theta = atan2f(y, x);
+      const float r = sqrtf(x * x + y * y);
+      float ph = 0.0f;
+      for (int q = 0; q < nz; ++q) {
+        const float w = zw[q];
+        if (w == 0.0f) continue;
+        const int jn = q + 1;  // Noll index
+        const int n = (int)ceil(sqrt(0.25 + 2.0 * jn) - 1.5);
+        const int rem = jn - (n + 1) * n / 2;
+        const int m = ((n + 1) % 2) * 2 * (rem / 2) + (n % 2) * (2 * ((rem + 1) / 2) - 1);
+        float R = 0.0f;
+        for (int sidx = 0; sidx <= (n - m) / 2; ++sidx) {
+          double num = ((sidx % 2) ? -1.0 : 1.0) * tgamma((double)(n - sidx + 1));
+          double den = tgamma((double)sidx + 1) * tgamma((double)((n + m) / 2 - sidx + 1)) *
+                       tgamma((double)((n - m) / 2 - sidx + 1));
+          const int e = n - 2 * sidx;
+          float pw = 1.0f;
+          for (int t = 0; t < e; ++t) pw *= r;
+          R += (float)(num / den) * pw;
+        }
+        float Z;
+        if (m == 0) Z = (float)sqrt((double)n + 1.0) * R;
+        else if (jn % 2 == 0) Z = (float)sqrt(2.0 * (n + 1)) * R * cosf((float)m * theta);
+        else Z = (float)sqrt(2.0 * (n + 1)) * R * sinf((float)m * theta);
+        ph += w * Z;
+      }
+      zmap[p] = ph;
+    }
+  }
+}
+
+__global__ void sum_kernel(const float* __restrict__ v, int64_t n, double* out) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)v[i];
+  const double t = block_sum_double(acc, red);
+  if (threadIdx.x == 0) atomicAdd(out, t);
+}
+
+__global__ void zernike_apply_kernel(float2* out, const float* __restrict__ zmap, int64_t n,
+                                     const double* sum, float kf) {
+  const float mean = (float)(*sum / (double)n);
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const float arg = kf * (zmap[p] - mean);
+    float s, c;
+    sincosf(arg, &s, &c);
+    out[p] = cmul(out[p], make_float2(c, s));
+  }
+}
+
+// ---------------------------------------------------------------- scaled step
+// Entry: u <- u * exp(i theta_in) * s0 with
+//   theta_in = ((k T) (1 - sc)) * (1 / (2 dz)),  T = (x dx0)^2 + (y dx0)^2,
+// the order Python evaluates 1j*k*T*(1-sc)/(2dz) on a complex64 tensor.
+__global__ void entry_kernel(float2* u, int B, int N, float dx0, float kf, float onemsc,
+                             float inv2dz, const float* __restrict__ s0, float pre) {
+  const int64_t tot = (int64_t)N * N;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot * B;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = p % tot;
+    const int i = (int)(q / N), j = (int)(q % N);
+    const float x = (float)(i - N / 2), y = (float)(j - N / 2);
+    float2 v = u[p];
+    if (onemsc != 0.0f) {
+      const float xa = x * dx0, ya = y * dx0;
+      const float T = xa * xa + ya * ya;
+      const float th = ((kf * T) * onemsc) * inv2dz;
+      float s, c;
+      sincosf(th, &s, &c);
+      v = cmul(v, make_float2(c, s));
+    }
+    if (s0) v = cscale(v, s0[q]);
+    if (pre != 1.0f) v = cscale(v, pre);  // u / sc as a multiply by the rounded reciprocal
+    u[p] = v;
+  }
+}
+
+// Multiply by the Fresnel factor in FFT order (ifftshift of the centred grid)
+// and 1/N^2: arg = (c * v) * (1/sc), v = (x/(N dx))^2 + (y/(N dx))^2.
+__global__ void fresnel_q_kernel(float2* u, int B, int N, float cq, float hd, float invsc,
+                                 int scaled, float norm) {
+  const int64_t tot = (int64_t)N * N;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot * B;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = p % tot;
+    const int ki = (int)(q / N), kj = (int)(q % N);
+    const int ci = (ki + N / 2) % N, cj = (kj + N / 2) % N;  // centred index
+    const float x = (float)(ci - N / 2), y = (float)(cj - N / 2);
+    const float xa = x / hd, ya = y / hd;
+    float arg = cq * (xa * xa + ya * ya);
+    if (scaled) arg = arg * invsc;
+    float s, c;
+    sincosf(arg, &s, &c);
+    const float2 v = cmul(u[p], make_float2(c, s));
+    u[p] = cscale(v, norm);
+  }
+}
+
+// Exit: u <- u * s1 * exp(i theta_out), theta_out = cexit * ((x dx1)^2 + (y dx1)^2).
+__global__ void exit_kernel(float2* u, int B, int N, float dx1, float cexit,
+                            const float* __restrict__ s1, int chirp) {
+  const int64_t tot = (int64_t)N * N;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot * B;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = p % tot;
+    const int i = (int)(q / N), j = (int)(q % N);
+    float2 v = u[p];
+    if (s1) v = cscale(v, s1[q]);
+    if (chirp) {
+      const float xa = (float)(i - N / 2) * dx1, ya = (float)(j - N / 2) * dx1;
+      const float th = cexit * (xa * xa + ya * ya);
+      float s, c;
+      sincosf(th, &s, &c);
+      v = cmul(v, make_float2(c, s));
+    }
+    u[p] = v;
+  }
+}
+
+__global__ void sumsq_kernel(const float2* __restrict__ u, int64_t n, double* out) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = u[i].x, b = u[i].y;
+    acc += a * a + b * b;
+  }
+  const double t = block_sum_double(acc, red);
+  if (threadIdx.x == 0) atomicAdd(out, t);
+}
+
+__global__ void scale_by_norm_kernel(float2* u, int64_t n, const double* sumsq) {
+  const float inv = 1.0f / (float)sqrt(*sumsq);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    u[i] = cscale(u[i], inv);
+}
+
+__global__ void zero_double_kernel(double* p, int n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0.0;
+}
+
+inline unsigned nblocks(int64_t n, int bs = 256) {
+  const int64_t b = (n + bs - 1) / bs;
+  return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+}  // namespace
+
+// Factor N = N1 * N2 with N1 <= N2 as balanced as possible.
+static void split_size(int N, int* N1, int* N2) {
+  int best = 1;
+  for (int d = 1; d * d <= N; ++d)
+    if (N % d == 0) best = d;
+  *N1 = best;
+  *N2 = N / best;
+}
+
+// In-place 2-D FFT of B fields of N x N (complex64); tw holds W_N^j.
+int maomi_fft2_generic(float2* u, int B, int N, int inverse, const float2* tw, hipStream_t st) {
+  int N1, N2;
+  split_size(N, &N1, &N2);
+  int L = 2048 / N;
+  if (L < 1) L = 1;
+  if (L > N) L = N;
+  const size_t shm = ((size_t)2 * L * N + N) * sizeof(float2);
+  const dim3 grid((unsigned)((N + L - 1) / L), (unsigned)B);
+  for (int axis = 1; axis >= 0; --axis) {
+    hipLaunchKernelGGL(fft_lines_kernel, grid, dim3(256), shm, st, u, N, N1, N2, axis, inverse, L,
+                       tw);
+    MAOMI_CHECK_LAUNCH();
+  }
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_twiddles(void* tw, int64_t N, maomi_stream_t stream) {
+  if (!tw || N < 2) { maomi_set_error("maomi_twiddles: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(twiddle_table_kernel, dim3(nblocks(N)), dim3(256), 0, (hipStream_t)stream,
+                     (float2*)tw, (int)N);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_pupil_gaussian(void* out, int64_t N, double D1, double rad, double objrad,
+                                   double k, double fl, const float* zern_abb, int64_t nz,
+                                   float* zmap, double* dwork, maomi_stream_t stream) {
+  if (!out || N < 2 || N > 8192 || (nz > 0 && (!zern_abb || !zmap || !dwork))) {
+    maomi_set_error("maomi_pupil_gaussian: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const float rad2 = (float)(rad * rad), obj2 = (float)(objrad * objrad);
+  const float ck = (float)(-k / (2.0 * fl));
+  const int64_t n = N * N;
+  hipLaunchKernelGGL(pupil_kernel, dim3(nblocks(n)), dim3(256), 0, st, (float2*)out, zmap, (int)N,
+                     (float)D1, rad2, obj2, ck, (float)objrad, zern_abb, (int)nz);
+  MAOMI_CHECK_LAUNCH();
+  if (nz > 0) {
+    hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(64), 0, st, dwork, 1);
+    hipLaunchKernelGGL(sum_kernel, dim3(nblocks(n)), dim3(256), 0, st, zmap, n, dwork);
+    hipLaunchKernelGGL(zernike_apply_kernel, dim3(nblocks(n)), dim3(256), 0, st, (float2*)out,
+                       zmap, n, dwork, (float)k);
+    MAOMI_CHECK_LAUNCH();
+  }
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, double dx1,
+                                 double dz, double wavelength, double nr, const float* screen0,
+                                 const float* screen1, const void* tw, maomi_stream_t stream) {
+  if (!u || !tw || B < 1 || N < 2 || N % 2 || !(dz > 0) || !(dx0 > 0) || !(dx1 > 0)) {
+    maomi_set_error("maomi_fresnel_step: bad arguments (even N, positive dz/dx required)");
+    return MAOMI_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  float2* f = (float2*)u;
+  const double lam = wavelength / nr;
+  const double k = 2.0 * M_PI / lam;
+  const double sc = dx1 / dx0;
+  const int64_t n = B * N * N;
+  const float onemsc = (float)(1.0 - sc);
+  const float inv2dz = 1.0f / (float)(2.0 * dz);
+  const float invsc0 = 1.0f / (float)sc;
+  hipLaunchKernelGGL(entry_kernel, dim3(nblocks(n)), dim3(256), 0, st, f, (int)B, (int)N,
+                     (float)dx0, (float)k, onemsc, inv2dz, screen0, sc != 1.0 ? invsc0 : 1.0f);
+  MAOMI_CHECK_LAUNCH();
+  int rc = maomi_fft2_generic(f, (int)B, (int)N, 0, (const float2*)tw, st);
+  if (rc) return rc;
+  const float cq = (float)(-M_PI * lam * dz);
+  const float hd = (float)(N * dx0);
+  const int scaled = sc != 1.0;
+  const float invsc = 1.0f / (float)sc;
+  hipLaunchKernelGGL(fresnel_q_kernel, dim3(nblocks(n)), dim3(256), 0, st, f, (int)B, (int)N, cq,
+                     hd, invsc, scaled, (float)(1.0 / ((double)N * (double)N)));
+  MAOMI_CHECK_LAUNCH();
+  rc = maomi_fft2_generic(f, (int)B, (int)N, 1, (const float2*)tw, st);
+  if (rc) return rc;
+  const double cexit = k / 2.0 * (sc - 1.0) / (sc * dz);
+  hipLaunchKernelGGL(exit_kernel, dim3(nblocks(n)), dim3(256), 0, st, f, (int)B, (int)N,
+                     (float)dx1, (float)cexit, screen1, scaled);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_normalize_l2(void* u, int64_t n, double* dwork, maomi_stream_t stream) {
+  if (!u || !dwork || n < 1) {
+    maomi_set_error("maomi_normalize_l2: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(64), 0, st, dwork, 1);
+  hipLaunchKernelGGL(sumsq_kernel, dim3(nblocks(n)), dim3(256), 0, st, (const float2*)u, n, dwork);
+  hipLaunchKernelGGL(scale_by_norm_kernel, dim3(nblocks(n)), dim3(256), 0, st, (float2*)u, n,
+                     dwork);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+// ---------------------------------------------------------------- per-size tables
+namespace {
+// e[i] = exp(-a^16), a = (i - N/2) / (0.47 N), float32 as edge_apodization; e2 = e e^T.
+__global__ void edge_kernel(float* e1, float* e2, int N, float denom) {
+  const int tot = N * N;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < tot; p += gridDim.x * blockDim.x) {
+    const int i = p / N, j = p % N;
+    const float ai = (float)(i - N / 2) / denom, aj = (float)(j - N / 2) / denom;
+    const float ei = expf(-powf(ai, 16.0f)), ej = expf(-powf(aj, 16.0f));
+    if (e2) e2[p] = ei * ej;
+    if (e1 && i == 0) e1[j] = ej;
+  }
+}
+
+// q[k] = exp(-i pi lambda' dz f_k^2) / N, f_k = m_k / (N dx), m_k the signed FFT index.
+__global__ void q1d_kernel(float2* q, int N, double dx, double dz, double lam) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    const int m = k < N / 2 ? k : k - N;
+    const double f = (double)m / ((double)N * dx);
+    const double a = -M_PI * lam * dz * f * f;
+    q[k] = make_float2((float)(cos(a) / N), (float)(sin(a) / N));
+  }
+}
+}  // namespace
+
+MAOMI_API int maomi_edge_window(float* e1d, float* e2d, int64_t N, maomi_stream_t stream) {
+  if ((!e1d && !e2d) || N < 2) { maomi_set_error("maomi_edge_window: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(edge_kernel, dim3(nblocks(N * N)), dim3(256), 0, (hipStream_t)stream, e1d, e2d,
+                     (int)N, (float)(0.47 * (double)N));
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_fresnel_q1d(void* q, int64_t N, double dx, double dz, double lam_eff,
+                                maomi_stream_t stream) {
+  if (!q || N < 2 || N % 2) { maomi_set_error("maomi_fresnel_q1d: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(q1d_kernel, dim3(nblocks(N)), dim3(256), 0, (hipStream_t)stream, (float2*)q,
+                     (int)N, dx, dz, lam_eff);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}

@@ -1,0 +1,226 @@
+"""Plain optical stage on one MI355X: phase prep, beam prep, all sites in one launch, v4 masks.
+
+Timing boundary as the reference's run_fixed (maomi/fixed.py:337-378):
+plan (phase + beam preparation) + propagation (all sites) + aggregation (mask
+assembly).  Every numerical step runs in libmaomi_amd.so; torch provides
+device memory, streams and the FP64 dense solve of the v4 system.
+"""
+
+import time
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .workloads import PlainConfig, beam_parameters, site_offsets, zernike_weights
+
+_CLASSES = {64: 0, 128: 1, 256: 2, 512: 3}
+_SLOT = 512  # table slots per class
+
+
+def _check_sizes(n):
+    if n not in _CLASSES:
+        raise ValueError(f"field side {n} unsupported; use one of {sorted(_CLASSES)}")
+
+
+class SiteTables:
+    """Per-class edge windows, 1-D Fresnel factors and twiddles on the device."""
+
+    def __init__(self, sizes, dx, dz, lam_eff, device):
+        lib = nat.lib()
+        st = nat.stream_handle(device)
+        self.edge = torch.zeros(5 * _SLOT, dtype=torch.float32, device=device)
+        self.q = torch.zeros(5 * _SLOT, dtype=torch.complex64, device=device)
+        self.tw = torch.zeros(5 * _SLOT, dtype=torch.complex64, device=device)
+        self.edge2d = {}
+        for n in sorted(set(int(s) for s in sizes)):
+            _check_sizes(n)
+            c = _CLASSES[n]
+            e2 = torch.empty((n, n), dtype=torch.float32, device=device)
+            e1 = self.edge[c * _SLOT:c * _SLOT + n]
+            nat.check(lib.maomi_edge_window(nat.ptr(e1), nat.ptr(e2), n, st), "edge window")
+            nat.check(lib.maomi_fresnel_q1d(nat.ptr(self.q[c * _SLOT:]), n, float(dx), float(dz),
+                                            float(lam_eff), st), "fresnel q1d")
+            nat.check(lib.maomi_twiddles(nat.ptr(self.tw[c * _SLOT:]), n, st), "twiddles")
+            self.edge2d[n] = e2
+        self.offsets = [c * _SLOT for c in range(5)]
+
+
+def make_pupil(n, bp, zern, device):
+    """generate_ba on an n x n grid (gaussian beam, objective aperture, Zernike screen)."""
+    lib = nat.lib()
+    st = nat.stream_handle(device)
+    u = torch.empty((n, n), dtype=torch.complex64, device=device)
+    abb = np.asarray(zern, float) * bp["wvl_um"] * 1e-6 if len(zern) else np.zeros(0)
+    nz = int(len(abb)) if np.any(abb != 0) else 0
+    zw = torch.as_tensor(abb.astype(np.float32), device=device) if nz else None
+    zmap = torch.empty((n, n), dtype=torch.float32, device=device) if nz else None
+    dwork = torch.zeros(1, dtype=torch.float64, device=device)
+    nat.check(lib.maomi_pupil_gaussian(nat.ptr(u), n, bp["d1"], bp["rad"], bp["objrad"], bp["k"],
+                                       bp["fl"], nat.ptr(zw), nz, nat.ptr(zmap), nat.ptr(dwork),
+                                       st), "pupil")
+    return u, (zw, zmap, dwork)
+
+
+def base_beam(u, bp, edge2d, tw, device, *, normalize=True):
+    """One scaled Fresnel step d1 -> dx to the entry plane, then unit L2 norm (batched.py:43-51)."""
+    lib = nat.lib()
+    st = nat.stream_handle(device)
+    n = u.shape[-1]
+    b = u.clone()
+    nat.check(lib.maomi_fresnel_step(nat.ptr(b), 1, n, float(bp["d1"]), float(bp["dx"]),
+                                     float(bp["dz_focus"]), float(bp["wvl"]), float(bp["nr"]),
+                                     nat.ptr(edge2d), nat.ptr(edge2d), nat.ptr(tw), st),
+              "fresnel step")
+    if normalize:
+        dwork = torch.zeros(1, dtype=torch.float64, device=device)
+        nat.check(lib.maomi_normalize_l2(nat.ptr(b), b.numel(), nat.ptr(dwork), st), "normalize")
+    return b
+
+
+def v4_masks(points, values, shape, device):
+    """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153)."""
+    lib = nat.lib()
+    st = nat.stream_handle(device)
+    p = torch.as_tensor(np.ascontiguousarray(points, np.float64), device=device)
+    v = torch.as_tensor(values, dtype=torch.float64, device=device)
+    if v.ndim == 1:
+        v = v[:, None]
+    P, C = v.shape
+    xs = torch.arange(shape[0], dtype=torch.float64, device=device) + 0.5
+    ys = torch.arange(shape[1], dtype=torch.float64, device=device) + 0.5
+    q = torch.stack(torch.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2).contiguous()
+    if P == 1:
+        return v[0].expand(shape[0], shape[1], C).clone()
+    G = torch.empty((P, P), dtype=torch.float64, device=device)
+    nat.check(lib.maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G), st), "green matrix")
+    coef = torch.linalg.solve(G, v).contiguous()
+    out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
+    nat.check(lib.maomi_v4_apply_f64(nat.ptr(p), nat.ptr(coef), nat.ptr(q), nat.ptr(out), P,
+                                     q.shape[0], C, st), "v4 apply")
+    return out.reshape(shape[0], shape[1], C)
+
+
+class PlainStage:
+    """One plain optical-stage run on `device` (uniform N and S over all sites)."""
+
+    def __init__(self, cfg: PlainConfig, index_volume, *, device="cuda", precision="fp32",
+                 grid=0, sites=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("PlainStage needs a ROCm GPU (torch.cuda.is_available() is False)")
+        _check_sizes(cfg.n)
+        if precision not in ("fp32", "mixed"):
+            raise ValueError("precision must be fp32 or mixed")
+        self.cfg, self.device, self.precision = cfg, torch.device(device), precision
+        vol = torch.as_tensor(index_volume, dtype=torch.float32, device=self.device)
+        exp = (cfg.padded, cfg.padded, cfg.slices * cfg.voxels_per_slice)
+        if tuple(vol.shape) != exp:
+            raise ValueError(f"index volume must have shape {exp}, got {tuple(vol.shape)}")
+        self.volume = vol.contiguous()
+        self.offsets = site_offsets(cfg) if sites is None else np.asarray(sites, np.int64)
+        if np.any(self.offsets < 0) or np.any(self.offsets > cfg.fov):
+            raise ValueError("site windows must lie inside the padded volume")
+        self.grid = int(grid)
+        self.bp = dict(beam_parameters(cfg), wvl_um=cfg.wavelength_um)
+        self.zern = zernike_weights(cfg)
+        self.timing = {}
+        P = len(self.offsets)
+        d = self.device
+        self.site_x = torch.as_tensor(self.offsets[:, 0].astype(np.int32), device=d)
+        self.site_y = torch.as_tensor(self.offsets[:, 1].astype(np.int32), device=d)
+        self.site_n = torch.full((P,), cfg.n, dtype=torch.int32, device=d)
+        self.site_s = torch.full((P,), cfg.slices, dtype=torch.int32, device=d)
+        self.order = torch.arange(P, dtype=torch.int32, device=d)
+        self.base_off = torch.zeros(P, dtype=torch.int64, device=d)
+        self.mask_off = torch.arange(P, dtype=torch.int64, device=d) * (cfg.n * cfg.n)
+
+    # -------------------------------------------------------------- stages
+    def plan(self):
+        cfg, d = self.cfg, self.device
+        lib = nat.lib()
+        st = nat.stream_handle(d)
+        Z = cfg.slices
+        H = W = cfg.padded
+        self.phase = torch.empty((Z, H, W), dtype=torch.float32, device=d)
+        self.phase16 = self.scale = None
+        smax = None
+        if self.precision == "mixed":
+            self.phase16 = torch.empty((Z, H, W), dtype=torch.float16, device=d)
+            self.scale = torch.empty(Z, dtype=torch.float32, device=d)
+            smax = torch.empty(Z, dtype=torch.float32, device=d)
+        nat.check(lib.maomi_phase_prepare(nat.ptr(self.volume), H, W, Z * cfg.voxels_per_slice,
+                                          cfg.voxels_per_slice, float(cfg.n0),
+                                          float(self.bp["k0dz"]), nat.ptr(self.phase),
+                                          nat.ptr(self.phase16), nat.ptr(self.scale),
+                                          nat.ptr(smax), st), "phase prepare")
+        if self.precision == "mixed":
+            self.phase = None  # only the float16 screens are read by the sites kernel
+        self.tables = SiteTables([cfg.n], self.bp["dx"], self.bp["dz"],
+                                 float(self.bp["wvl"]) / float(self.bp["nr"]), d)
+        c = _CLASSES[cfg.n]
+        tw = self.tables.tw[c * _SLOT:c * _SLOT + cfg.n]
+        self.pupil, _ = make_pupil(cfg.n, self.bp, self.zern, d)
+        self.base = base_beam(self.pupil, self.bp, self.tables.edge2d[cfg.n], tw, d)
+        return self
+
+    def propagate(self):
+        cfg, d = self.cfg, self.device
+        lib = nat.lib()
+        P = len(self.offsets)
+        grid = self.grid or int(lib.maomi_sites_default_grid()) or 256
+        grid = min(grid, P)
+        stride = cfg.n * cfg.n
+        if not hasattr(self, "workspace") or self.workspace.numel() < grid * stride:
+            self.workspace = torch.empty(grid * stride, dtype=torch.complex64, device=d)
+        self.masks = torch.empty((P, cfg.n, cfg.n), dtype=torch.float32, device=d)
+        self.energies = torch.empty(P, dtype=torch.float64, device=d)
+        desc = nat.SitesDesc()
+        desc.bases = self.base.data_ptr()
+        desc.base_off = self.base_off.data_ptr()
+        desc.phase = self.phase.data_ptr() if self.phase is not None else None
+        desc.phase16 = self.phase16.data_ptr() if self.phase16 is not None else None
+        desc.slice_scale = self.scale.data_ptr() if self.scale is not None else None
+        desc.edge = self.tables.edge.data_ptr()
+        desc.q1d = self.tables.q.data_ptr()
+        desc.twiddle = self.tables.tw.data_ptr()
+        desc.site_x, desc.site_y = self.site_x.data_ptr(), self.site_y.data_ptr()
+        desc.site_n, desc.site_s = self.site_n.data_ptr(), self.site_s.data_ptr()
+        desc.order = self.order.data_ptr()
+        desc.mask, desc.mask_off = self.masks.data_ptr(), self.mask_off.data_ptr()
+        desc.energy = self.energies.data_ptr()
+        desc.workspace = self.workspace.data_ptr()
+        desc.scratch_stride = stride
+        desc.P, desc.H, desc.W, desc.Z = P, cfg.padded, cfg.padded, cfg.slices
+        desc.grid, desc.uniform_n = grid, cfg.n
+        for i in range(5):
+            desc.class_edge[i] = desc.class_q[i] = desc.class_tw[i] = self.tables.offsets[i]
+        self._desc = desc
+        nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "persistent sites")
+        return self.masks, self.energies
+
+    def aggregate(self):
+        cfg = self.cfg
+        pts = self.offsets.astype(np.float64)
+        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device)
+        m = masks[..., 0]
+        mean = m.mean()
+        self.mask_raw, self.mask_mean = m, mean
+        self.mask = m / mean
+        return self.mask
+
+    def run(self):
+        """plan + propagation + aggregation, each bracketed by a device synchronisation."""
+        t = time.perf_counter()
+        self.plan()
+        torch.cuda.synchronize(self.device)
+        self.timing["plan"] = time.perf_counter() - t
+        t = time.perf_counter()
+        self.propagate()
+        torch.cuda.synchronize(self.device)
+        self.timing["propagation"] = time.perf_counter() - t
+        t = time.perf_counter()
+        self.aggregate()
+        torch.cuda.synchronize(self.device)
+        self.timing["aggregation"] = time.perf_counter() - t
+        self.timing["total"] = sum(self.timing[k] for k in ("plan", "propagation", "aggregation"))
+        return self

@@ -28,7 +28,7 @@ __all__ = [
     "zrf", "zidx", "zernike", "generate_ba", "pupil", "cubic", "resize_xy",
     "translate_xy", "fresnel", "propagate", "v4_interpolate", "assemble",
     "fixed_locations", "locations", "field_shapes", "base_field", "plain_stage_masks",
-    "resample_matrix", "set_workers",
+    "resample_matrix", "set_workers", "plain_phase", "plain_beam", "plain_stage",
 ]
 
 _WORKERS = 1
@@ -727,3 +727,59 @@ def plain_stage_masks(base, phase, offsets, *, wvl, nr, dx, dz, sizes=None, slic
         if return_planes:
             planes_out.append(saved)
     return (out, planes_out) if return_planes else out
+
+
+def plain_phase(index_volume, n0, k0dz, voxels_per_slice=1, dtype=np.complex64):
+    """Screens of the plain stage: phi[z] = k0 dz sum_v (n - n0), accumulated in FP64.
+
+    maomi/optics.py:232 (phase_scale * raw, raw = (n - n0)/n_diff summed over the slab).
+    index_volume (X, Y, Zv) -> phase (Z, X, Y).
+    """
+    v = np.asarray(index_volume)
+    X, Y, Zv = v.shape
+    acc = (v.astype(np.float64) - n0).reshape(X, Y, Zv // voxels_per_slice, voxels_per_slice)
+    acc = acc.sum(-1)
+    ph = (k0dz * acc).astype(_rdt(dtype))
+    return np.ascontiguousarray(np.moveaxis(ph, -1, 0))
+
+
+def plain_beam(cfg, zernike_weights=(), dtype=np.complex64):
+    """Pupil (generate_ba on an N-sample grid) carried to the entry plane, unit norm.
+
+    _optical_math.py:232-296 with N = cfg.n and D1 = beam_size(fl)/N; numerics.py:156-210
+    (one scaled step d1 -> dx over fl - focus); batched.py:51 (unit L2 norm, FP64 here).
+    """
+    N = int(cfg.n)
+    fl = float(cfg.focal_length_mm / 1000.0)
+    conf = OracleConfig(NA=cfg.NA, objNA=cfg.objNA, n=cfg.n0, wavelength_um=cfg.wavelength_um,
+                        focal_length_mm=cfg.focal_length_mm,
+                        zernike_weights=tuple(zernike_weights))
+    d1 = _grid_pitch(conf, np.array([N, N], np.float32))
+    u = generate_ba(conf, dtype, grid=N, pitch=d1)
+    sg = edge_apodization(N, dtype)
+    wvl, nr = np.float32(cfg.wavelength_um * 1e-6), np.float32(cfg.n0)
+    dx = np.float32(cfg.dx_um * 1e-6)
+    dzf = np.float32(fl - cfg.focus * 1e-6)
+    return base_field(u, wvl, d1, dx, dzf, nr, sg, norm="exact"), u
+
+
+def plain_stage(cfg, index_volume, offsets, zernike_weights=(), dtype=np.complex64,
+                sites=None, with_masks=True):
+    """The whole plain optical stage on the CPU: beam, screens, per-site masks, v4 FOV mask."""
+    base, u = plain_beam(cfg, zernike_weights, dtype)
+    k0dz = 2 * np.pi / cfg.wavelength_um * cfg.dz_um / cfg.voxels_per_slice
+    phase = plain_phase(index_volume, cfg.n0, k0dz, cfg.voxels_per_slice, dtype)
+    offs = np.asarray(offsets, int)
+    idx = np.arange(len(offs)) if sites is None else np.asarray(sites)
+    masks = plain_stage_masks(base, phase, offs[idx], wvl=np.float32(cfg.wavelength_um * 1e-6),
+                              nr=np.float32(cfg.n0), dx=np.float32(cfg.dx_um * 1e-6),
+                              dz=cfg.dz_um * 1e-6, dtype=dtype)
+    masks = np.stack(masks)
+    energies = masks.sum((1, 2), dtype=np.float64)
+    out = dict(masks=masks, energies=energies, base=base, pupil=u, phase=phase)
+    if with_masks and sites is None:
+        q = np.stack(np.meshgrid(np.arange(cfg.fov) + 0.5, np.arange(cfg.fov) + 0.5,
+                                 indexing="ij"), -1)
+        m = v4_interpolate(offs.astype(np.float64), energies, q)
+        out.update(mask_raw=m, mask_mean=float(m.mean()), mask=m / m.mean())
+    return out

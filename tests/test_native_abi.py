@@ -1,0 +1,48 @@
+"""The C-ABI library loads without a GPU and exports every symbol the header declares."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from maomi_amd import _native
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "maomi_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(maomi_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "maomi_persistent_sites" in names and len(names) >= 5
+
+
+def test_library_exports_every_declared_symbol():
+    if not _native.library_path().exists():
+        pytest.fail("libmaomi_amd.so is not built; run __graft_entry__.build()")
+    handle = ctypes.CDLL(str(_native.library_path()))
+    missing = [n for n in declared_functions() if not hasattr(handle, n)]
+    assert not missing, missing
+    # every binding in the Python wrapper is declared in the header
+    assert set(_native.exported_names()) <= set(declared_functions())
+
+
+def test_error_reporting_without_gpu_work():
+    lib = _native.lib()
+    assert lib.maomi_version().startswith(b"maomi_amd")
+    rc = lib.maomi_persistent_sites(None, None)
+    assert rc == -1
+    assert b"null descriptor" in lib.maomi_last_error()
+    with pytest.raises(RuntimeError, match="null descriptor"):
+        _native.check(rc, "maomi_persistent_sites")
+
+
+def test_sites_descriptor_layout_matches_header():
+    # 17 pointers + 9 int64 scalars + 3 x 5 int64 class tables, no padding
+    assert ctypes.sizeof(_native.SitesDesc) == 8 * (17 + 7 + 15)
