@@ -95,6 +95,58 @@ int maomi_phase_prepare(const float* index, int64_t X, int64_t Y, int64_t Zv, in
                         float* slice_scale, float* slice_max, maomi_stream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Reference-geometry stage: segments A / B / C per site in one persistent launch.
+ *
+ * Replaces OpticalPlan.evaluate (maomi/backends/cuda/batched.py:170-235) with
+ * _stage (157-168), _screens/gather_screens (148-155; optical_kernels.cu:3-12),
+ * ResamplePlan/resample_power (native.py:51-105; optical_kernels.cu:14-38) and
+ * the tail reductions (batched.py:202-226); same math as optics.propagate
+ * (maomi/optics.py:259-340).  Field sides n (A, C) and n2 (B) are any even
+ * sizes <= 1024 with n - n2 even.
+ * ------------------------------------------------------------------------- */
+typedef struct maomi_segment {
+  const float* phase;   /* [Z][H][W] phase (radians) of this segment's slabs       */
+  int64_t H, W, Z;      /* volume dims, Z = number of screens (>= steps)           */
+  int64_t n, steps;     /* field side, propagation steps                           */
+  int64_t n1, n2f;      /* line factorisation n = n1 * n2f                         */
+  const void* q;        /* complex64 1-D Fresnel tables, n entries each            */
+  const int32_t* qidx;  /* [steps] table used by each step                         */
+  const float* edge;    /* [n] edge window                                         */
+  const void* tw;       /* [n] W_n^j                                               */
+} maomi_segment;
+
+typedef struct maomi_segmented_desc {
+  maomi_segment seg[3];           /* A, B, C                                          */
+  const void* base;               /* n x n complex64 base beam                        */
+  const int32_t* site_x;          /* [P] window offsets (supersampled pixels)         */
+  const int32_t* site_y;
+  const int32_t* order_list;      /* [P] processing order                             */
+  int64_t P, cut, tails, order;   /* sites, first top-tail plane, tails on/off, photon order */
+  int64_t n, n2, fx, fy;          /* fields and focal window                          */
+  int64_t X, Y, TT;               /* focal output X x Y, tail output TT x TT          */
+  const float* mfx;               /* [X][fx] focal resample (translate + resize)      */
+  const float* mfy;               /* [Y][fy]                                          */
+  const float* mt;                /* [TT][n2] tail resample                           */
+  const float* ax_focus;          /* [b] axial weights                                */
+  const float* ax_top;            /* [a + 1 - cut]                                    */
+  const float* ax_bot;            /* [c + 1]                                          */
+  double ss2, focal_scale, tail_scale, prop_um;
+  float* psf;                     /* [P][X][Y][b]                                     */
+  float* top_conv;                /* [P][TT][TT]                                      */
+  float* top_prof;                /* [P][a + 1 - cut]                                 */
+  double* top_w;                  /* [P]                                              */
+  float* bot_conv;                /* [P][TT][TT]                                      */
+  float* bot_prof;                /* [P][c + 1]                                       */
+  double* bot_w;                  /* [P]                                              */
+  void* work;                     /* grid * work_stride complex64 elements            */
+  int64_t work_stride;            /* >= maomi_segmented_workspace_elems(...)          */
+  int64_t grid;                   /* persistent blocks (0 = one per site)             */
+} maomi_segmented_desc;
+
+int maomi_segmented_sites(const maomi_segmented_desc* desc, maomi_stream_t stream);
+int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy, int64_t X);
+
+/* ---------------------------------------------------------------------------
  * Beam preparation (once per run, inside the timed "plan" stage).
  * ------------------------------------------------------------------------- */
 

@@ -22,6 +22,7 @@ SOURCES = {
     "plain_stage.hip": [],
     "prep.hip": ["-ffp-contract=off"],
     "v4.hip": ["-ffp-contract=off"],
+    "segmented_stage.hip": [],
 }
 
 

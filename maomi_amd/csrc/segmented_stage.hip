@@ -1,0 +1,325 @@
+// Reference-geometry optical stage: per site, three split-step segments
+//   A (n x n, phase A, saves planes >= cut cropped to n2: top tail),
+//   B (n2 x n2, phase B, saves focal planes cropped to the focal window),
+//   C (n x n, phase C, the B field zero-padded back to n: bottom tail),
+// with the focal |u|^(2 order) resample, the trapezoid tail accumulations,
+// their profiles and weights, all inside one persistent launch (one block per
+// site at a time, field in a block-private workspace).
+//
+// Reference: OpticalPlan.evaluate (maomi/backends/cuda/batched.py:170-235),
+// _stage/_screens (148-168), gather_screens and resample_power
+// (optical_kernels.cu:3-38), the tail block of optics.propagate
+// (maomi/optics.py:318-340).  Field sides are arbitrary even sizes: every line
+// transform is a two-level direct DFT (N = N1 N2) run from LDS.
+#include "common.hpp"
+#include "maomi_amd.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+struct Seg {
+  const float* phase;    // [Z][H][W] phase in radians
+  int64_t H, W, Z;       // volume dims; Z = screens
+  int n, steps;          // field side, propagation steps
+  int n1, n2f;           // N = n1 * n2f for the line DFT
+  const float2* q;       // q tables, n entries each (FFT order, 1/n folded in)
+  const int32_t* qidx;   // [steps] table index per step
+  const float* edge;     // [n] edge window
+  const float2* tw;      // [n] W_n^j
+};
+
+struct SegArgs {
+  Seg seg[3];
+  const float2* base;
+  const int32_t* site_x;
+  const int32_t* site_y;
+  const int32_t* order_list;
+  int64_t P;
+  int cut, tails, order;
+  int n, n2, coff;           // coff = (n - n2) / 2
+  int fx, fy, foff_x, foff_y;
+  int X, Y, TT;              // focal output X x Y, tail output TT x TT
+  const float* mfx;          // [X][fx]
+  const float* mfy;          // [Y][fy]
+  const float* mt;           // [TT][n2]
+  const float* ax_focus;     // [b]
+  const float* ax_top;       // [a + 1 - cut]
+  const float* ax_bot;       // [c + 1]
+  float ss2;                 // ss^2 applied to tail intensities
+  float focal_scale;         // ss^4 r (1e6 lambda)^1.5 / (pi nr)
+  float tail_scale;          // ss^2 applied after the tail resample
+  double prop_um;
+  float* psf;                // [P][X][Y][b]
+  float* top_conv;           // [P][TT][TT]
+  float* top_prof;           // [P][a+1-cut]
+  double* top_w;             // [P]
+  float* bot_conv;
+  float* bot_prof;           // [P][c+1]
+  double* bot_w;
+  float2* work;              // per block: 2 n^2 complex + n2^2 + fx fy + X max(fy,n2) floats
+  int64_t work_stride;       // float2 elements per block
+  int L;                     // lines per LDS tile
+};
+
+__device__ __forceinline__ float2 twid(const float2* T, int j, bool inv) {
+  const float2 w = T[j];
+  return inv ? make_float2(w.x, -w.y) : w;
+}
+
+// Transform nl lines of length N held in A (natural order) into A, via B.
+__device__ void line_dft(float2* A, float2* B, int nl, int N, int N1, int N2, const float2* T,
+                         bool inv) {
+  const int s1 = N / N2, s2 = N / N1;
+  for (int it = threadIdx.x; it < nl * N; it += NT) {
+    const int l = it / N, r = it % N, n1 = r / N2, k2 = r % N2;
+    const float2* a = A + l * N + n1;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int m = 0; m < N2; ++m) acc = cadd(acc, cmul(a[N1 * m], twid(T, ((m * k2) % N2) * s1, inv)));
+    B[l * N + n1 * N2 + k2] = cmul(acc, twid(T, (n1 * k2) % N, inv));
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < nl * N; it += NT) {
+    const int l = it / N, k = it % N, k2 = k % N2, k1 = k / N2;
+    const float2* b = B + l * N + k2;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int m = 0; m < N1; ++m) acc = cadd(acc, cmul(b[m * N2], twid(T, ((m * k1) % N1) * s2, inv)));
+    A[l * N + k] = acc;
+  }
+  __syncthreads();
+}
+
+// out[x][y] = scale * sum_i mx[x][i] sum_j my[y][j] in[i][j]  (in: R x C, tmp: X x C)
+__device__ void resample2d(const float* in, int R, int C, const float* mx, int X, const float* my,
+                           int Y, float* tmp, float* out, int64_t ostride, float scale) {
+  for (int it = threadIdx.x; it < X * C; it += NT) {
+    const int x = it / C, j = it % C;
+    float acc = 0.f;
+    for (int i = 0; i < R; ++i) acc += mx[x * R + i] * in[i * C + j];
+    tmp[it] = acc;
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < X * Y; it += NT) {
+    const int x = it / Y, y = it % Y;
+    float acc = 0.f;
+    for (int j = 0; j < C; ++j) acc += my[y * C + j] * tmp[x * C + j];
+    out[(int64_t)it * ostride] = scale * acc;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float powk(float p, int order) {
+  return order == 1 ? p : order == 2 ? p * p : p * p * p;
+}
+
+// One segment of one site.  src: first-plane source (complex), with index shift
+// (sx, sy) and side src_n; dst (may be null): receives the last plane (n x n).
+__device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, const float2* src,
+                            int src_n, int sx, int sy, float2* fld, float2* dst, float* tail,
+                            float* stage, float* tmp, int site, float2* A, float2* B, float2* T,
+                            float2* Qt, float* E, double* red) {
+  const Seg& g = a.seg[k];
+  const int N = g.n, N1 = g.n1, N2 = g.n2f, S = g.steps, L = a.L;
+  for (int i = threadIdx.x; i < N; i += NT) { T[i] = g.tw[i]; E[i] = g.edge[i]; }
+  __syncthreads();
+  const bool top = k == 0, bot = k == 2, foc = k == 1;
+  const int first_saved = top ? a.cut : 0;
+  const int nsaved = top ? S + 1 - a.cut : S + 1;
+  double wsum = 0.0;
+  if ((top || bot) && a.tails)
+    for (int i = threadIdx.x; i < a.n2 * a.n2; i += NT) tail[i] = 0.f;
+  for (int s = 0; s <= S; ++s) {
+    const bool first = s == 0, last = s == S;
+    const bool save_tail = a.tails && (top || bot) && s >= first_saved;
+    const bool save_foc = foc && s < S;
+    const float trap = (s == first_saved || s == S) ? 0.5f : 1.0f;
+    const float ax = save_tail ? (top ? a.ax_top[s - a.cut] : a.ax_bot[s])
+                               : (save_foc ? a.ax_focus[s] : 1.f);
+    double prof = 0.0;
+    if (!last) {
+      for (int i = threadIdx.x; i < N; i += NT) Qt[i] = g.q[(int64_t)g.qidx[s] * N + i];
+    }
+    // ---------------- row pass
+    for (int r0 = 0; r0 < N; r0 += L) {
+      const int nl = min(L, N - r0);
+      if (first) {
+        for (int it = threadIdx.x; it < nl * N; it += NT) {
+          const int l = it / N, y = it % N;
+          const int xs = r0 + l + sx, ys = y + sy;
+          A[l * N + y] = (xs >= 0 && xs < src_n && ys >= 0 && ys < src_n)
+                             ? src[(int64_t)xs * src_n + ys] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+      } else {
+        for (int it = threadIdx.x; it < nl * N; it += NT) A[it] = fld[(int64_t)r0 * N + it];
+        __syncthreads();
+        line_dft(A, B, nl, N, N1, N2, T, true);
+      }
+      for (int it = threadIdx.x; it < nl * N; it += NT) {
+        const int l = it / N, y = it % N, x = r0 + l;
+        float2 v = A[it];
+        if (!last) {
+          const float phi = g.phase[((int64_t)s * g.H + ox + x) * g.W + oy + y];
+          float sn, cs;
+          sincosf(phi, &sn, &cs);
+          const float sg = E[x] * E[y];
+          v = cmul(v, make_float2(cs * sg, sn * sg));
+        }
+        if (save_tail) {
+          const int cx = x - a.coff, cy = y - a.coff;
+          if (cx >= 0 && cx < a.n2 && cy >= 0 && cy < a.n2) {
+            const float I = a.ss2 * powk(v.x * v.x + v.y * v.y, a.order) * ax;
+            prof += (double)I;
+            tail[cx * a.n2 + cy] += trap * I;
+          }
+        }
+        if (save_foc) {
+          const int cx = x - a.foff_x, cy = y - a.foff_y;
+          if (cx >= 0 && cx < a.fx && cy >= 0 && cy < a.fy)
+            stage[cx * a.fy + cy] = powk(v.x * v.x + v.y * v.y, a.order);
+        }
+        if (last && dst) dst[(int64_t)x * N + y] = v;
+        A[it] = v;
+      }
+      __syncthreads();
+      if (!last) {
+        line_dft(A, B, nl, N, N1, N2, T, false);
+        for (int it = threadIdx.x; it < nl * N; it += NT) fld[(int64_t)r0 * N + it] = A[it];
+        __syncthreads();
+      }
+    }
+    if (save_tail) {
+      const double p = block_sum_double(prof, red);
+      if (threadIdx.x == 0) (top ? a.top_prof : a.bot_prof)[(int64_t)site * nsaved + s - first_saved] = (float)p;
+      wsum += (double)trap * p;
+    }
+    if (save_foc) {
+      __syncthreads();
+      const int b = a.seg[1].steps;
+      resample2d(stage, a.fx, a.fy, a.mfx, a.X, a.mfy, a.Y, tmp,
+                 a.psf + (int64_t)site * a.X * a.Y * b + s, b, a.focal_scale * ax);
+    }
+    if (last) break;
+    // ---------------- column pass
+    for (int c0 = 0; c0 < N; c0 += L) {
+      const int nl = min(L, N - c0);
+      for (int it = threadIdx.x; it < nl * N; it += NT) {
+        const int i = it / nl, l = it % nl;
+        A[l * N + i] = fld[(int64_t)i * N + c0 + l];
+      }
+      __syncthreads();
+      line_dft(A, B, nl, N, N1, N2, T, false);
+      for (int it = threadIdx.x; it < nl * N; it += NT) {
+        const int l = it / N, kx = it % N;
+        A[it] = cmul(A[it], cmul(Qt[kx], Qt[c0 + l]));
+      }
+      __syncthreads();
+      line_dft(A, B, nl, N, N1, N2, T, true);
+      for (int it = threadIdx.x; it < nl * N; it += NT) {
+        const int i = it / nl, l = it % nl;
+        fld[(int64_t)i * N + c0 + l] = A[l * N + i];
+      }
+      __syncthreads();
+    }
+  }
+  if ((top || bot) && a.tails) {
+    __syncthreads();
+    float* conv = (top ? a.top_conv : a.bot_conv) + (int64_t)site * a.TT * a.TT;
+    resample2d(tail, a.n2, a.n2, a.mt, a.TT, a.mt, a.TT, tmp, conv, 1, a.tail_scale);
+    if (threadIdx.x == 0) (top ? a.top_w : a.bot_w)[site] = wsum * a.prop_um;
+  }
+}
+
+__global__ __launch_bounds__(NT) void segmented_sites_kernel(SegArgs a) {
+  extern __shared__ float2 sh[];
+  const int nmax = a.n;
+  float2* A = sh;
+  float2* B = A + a.L * nmax;
+  float2* T = B + a.L * nmax;
+  float2* Qt = T + nmax;
+  float* E = (float*)(Qt + nmax);
+  double* red = (double*)(E + nmax + (nmax & 1));
+  float2* w = a.work + (int64_t)blockIdx.x * a.work_stride;
+  float2* fld = w;
+  float2* xfer = w + (int64_t)a.n * a.n;          // A's last plane (n x n)
+  float2* xfer2 = xfer + (int64_t)a.n * a.n;      // B's last plane (n2 x n2)
+  float* tail = (float*)(xfer2 + (int64_t)a.n2 * a.n2);
+  float* stage = tail + a.n2 * a.n2;
+  float* tmp = stage + a.fx * a.fy;
+  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
+    const int site = a.order_list[i];
+    const int64_t ox = a.site_x[site], oy = a.site_y[site];
+    // A: base -> planes >= cut (top tail), last plane kept whole in xfer
+    run_segment(a, 0, ox, oy, a.base, a.n, 0, 0, fld, xfer, tail, stage, tmp, site, A, B, T, Qt, E,
+                red);
+    // B: crop of A's last plane -> focal planes; last plane kept in xfer (n2 x n2)
+    run_segment(a, 1, ox, oy, xfer, a.n, a.coff, a.coff, fld, a.tails ? xfer2 : nullptr, tail,
+                stage, tmp, site, A, B, T, Qt, E, red);
+    if (a.tails)  // C: B's last plane padded back to n -> bottom tail
+      run_segment(a, 2, ox, oy, xfer2, a.n2, -a.coff, -a.coff, fld, nullptr, tail, stage, tmp, site,
+                  A, B, T, Qt, E, red);
+  }
+}
+
+}  // namespace
+
+MAOMI_API int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy,
+                                                  int64_t X) {
+  const int64_t floats = n2 * n2 + fx * fy + X * (fy > n2 ? fy : n2) + 64;
+  return 2 * n * n + n2 * n2 + (floats + 1) / 2;
+}
+
+MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_t stream) {
+  if (!d) { maomi_set_error("maomi_segmented_sites: null descriptor"); return MAOMI_EINVAL; }
+  if (d->P <= 0) return MAOMI_OK;
+  const int64_t n = d->n, n2 = d->n2;
+  if (n < 2 || n2 < 2 || n2 > n || (n - n2) % 2 || n > 1024 || d->work == nullptr ||
+      !d->base || !d->site_x || !d->site_y || !d->order_list || !d->psf) {
+    maomi_set_error("maomi_segmented_sites: bad sizes or null pointers (n %lld, n2 %lld)",
+                    (long long)n, (long long)n2);
+    return MAOMI_EINVAL;
+  }
+  if (d->order < 1 || d->order > 3 || d->fx > n2 || d->fy > n2 || d->X < 1 || d->Y < 1) {
+    maomi_set_error("maomi_segmented_sites: bad order or focal window");
+    return MAOMI_EINVAL;
+  }
+  SegArgs a;
+  for (int k = 0; k < 3; ++k) {
+    const maomi_segment& s = d->seg[k];
+    a.seg[k].phase = s.phase; a.seg[k].H = s.H; a.seg[k].W = s.W; a.seg[k].Z = s.Z;
+    a.seg[k].n = (int)s.n; a.seg[k].steps = (int)s.steps; a.seg[k].n1 = (int)s.n1;
+    a.seg[k].n2f = (int)s.n2f; a.seg[k].q = (const float2*)s.q; a.seg[k].qidx = s.qidx;
+    a.seg[k].edge = s.edge; a.seg[k].tw = (const float2*)s.tw;
+    const bool used = k < 2 || d->tails;
+    if (used && (!s.phase || !s.q || !s.qidx || !s.edge || !s.tw || s.n1 * s.n2f != s.n ||
+                 s.steps < 1 || s.Z < s.steps || s.n != (k == 1 ? n2 : n))) {
+      maomi_set_error("maomi_segmented_sites: segment %d is inconsistent", k);
+      return MAOMI_EINVAL;
+    }
+  }
+  a.base = (const float2*)d->base; a.site_x = d->site_x; a.site_y = d->site_y;
+  a.order_list = d->order_list; a.P = d->P; a.cut = (int)d->cut; a.tails = (int)d->tails;
+  a.order = (int)d->order; a.n = (int)n; a.n2 = (int)n2; a.coff = (int)((n - n2) / 2);
+  a.fx = (int)d->fx; a.fy = (int)d->fy; a.foff_x = (int)((n2 - d->fx) / 2);
+  a.foff_y = (int)((n2 - d->fy) / 2); a.X = (int)d->X; a.Y = (int)d->Y; a.TT = (int)d->TT;
+  a.mfx = d->mfx; a.mfy = d->mfy; a.mt = d->mt; a.ax_focus = d->ax_focus; a.ax_top = d->ax_top;
+  a.ax_bot = d->ax_bot; a.ss2 = (float)d->ss2; a.focal_scale = (float)d->focal_scale;
+  a.tail_scale = (float)d->tail_scale; a.prop_um = d->prop_um;
+  a.psf = d->psf; a.top_conv = d->top_conv; a.top_prof = d->top_prof; a.top_w = d->top_w;
+  a.bot_conv = d->bot_conv; a.bot_prof = d->bot_prof; a.bot_w = d->bot_w;
+  a.work = (float2*)d->work; a.work_stride = d->work_stride;
+  if (a.work_stride < maomi_segmented_workspace_elems(n, n2, d->fx, d->fy, d->X)) {
+    maomi_set_error("maomi_segmented_sites: work_stride too small");
+    return MAOMI_EINVAL;
+  }
+  a.L = (int)(3584 / n);
+  if (a.L < 1) a.L = 1;
+  const size_t shm = ((size_t)2 * a.L * n + 2 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
+                     (NT / 64) * sizeof(double) + 16;
+  int64_t grid = d->grid > 0 ? d->grid : d->P;
+  if (grid > d->P) grid = d->P;
+  hipLaunchKernelGGL(segmented_sites_kernel, dim3((unsigned)grid), dim3(NT), shm,
+                     (hipStream_t)stream, a);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
