@@ -172,6 +172,12 @@ int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, double dx1, do
                        double wavelength, double nr, const float* screen0, const float* screen1,
                        const void* tw, maomi_stream_t stream);
 
+/* In-place unnormalised 2-D DFT of B square N x N complex64 fields (forward:
+ * exp(-2 pi i jk/N); inverse: exp(+...)), the torch.fft.fft2/ifft2 of
+ * numerics.fresnel (maomi/numerics.py:200-203) for any N factoring as N1*N2. */
+int maomi_fft2_c64(void* u, int64_t B, int64_t N, int64_t inverse, const void* tw,
+                   maomi_stream_t stream);
+
 /* u <- u / ||u||_2 with the norm accumulated in FP64 (batched.py:51).  dwork: 1 double. */
 int maomi_normalize_l2(void* u, int64_t n, double* dwork, maomi_stream_t stream);
 
@@ -183,6 +189,15 @@ int maomi_edge_window(float* e1d, float* e2d, int64_t N, maomi_stream_t stream);
  * (batched.py:68-82): q[k] = exp(-i pi lam_eff dz f_k^2) / N, f_k = m_k/(N dx). */
 int maomi_fresnel_q1d(void* q, int64_t N, double dx, double dz, double lam_eff,
                       maomi_stream_t stream);
+
+/* Phase screens of the reference geometry (batched.py:52-60, numerics.py:15-39):
+ * dst[z][xo][yo] = scale * sum_k w1[yo][k] sum_j w0[xo][j] src[idx0[xo][j]][idx1[yo][k]][z]
+ * for a raw (X, Y, Z) float32 field; idx/w are the imresize tap tables (m x K);
+ * tmp holds m0 * Y * Z floats. */
+int maomi_resize_phase(const float* src, int64_t X, int64_t Y, int64_t Z, const int32_t* idx0,
+                       const float* w0, int64_t m0, int64_t K0, const int32_t* idx1,
+                       const float* w1, int64_t m1, int64_t K1, double scale, float* tmp,
+                       float* dst, maomi_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Mask assembly (the timed "aggregation" stage), FP64.

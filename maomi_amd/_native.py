@@ -26,6 +26,31 @@ class SitesDesc(ctypes.Structure):
     ]
 
 
+class Segment(ctypes.Structure):
+    """Mirror of maomi_segment."""
+    _fields_ = [("phase", c_void_p), ("H", c_int64), ("W", c_int64), ("Z", c_int64),
+                ("n", c_int64), ("steps", c_int64), ("n1", c_int64), ("n2f", c_int64),
+                ("q", c_void_p), ("qidx", c_void_p), ("edge", c_void_p), ("tw", c_void_p)]
+
+
+class SegmentedDesc(ctypes.Structure):
+    """Mirror of maomi_segmented_desc."""
+    _fields_ = [
+        ("seg", Segment * 3), ("base", c_void_p), ("site_x", c_void_p), ("site_y", c_void_p),
+        ("order_list", c_void_p),
+        ("P", c_int64), ("cut", c_int64), ("tails", c_int64), ("order", c_int64),
+        ("n", c_int64), ("n2", c_int64), ("fx", c_int64), ("fy", c_int64),
+        ("X", c_int64), ("Y", c_int64), ("TT", c_int64),
+        ("mfx", c_void_p), ("mfy", c_void_p), ("mt", c_void_p),
+        ("ax_focus", c_void_p), ("ax_top", c_void_p), ("ax_bot", c_void_p),
+        ("ss2", c_double), ("focal_scale", c_double), ("tail_scale", c_double),
+        ("prop_um", c_double),
+        ("psf", c_void_p), ("top_conv", c_void_p), ("top_prof", c_void_p), ("top_w", c_void_p),
+        ("bot_conv", c_void_p), ("bot_prof", c_void_p), ("bot_w", c_void_p),
+        ("work", c_void_p), ("work_stride", c_int64), ("grid", c_int64),
+    ]
+
+
 _SIGS = {
     "maomi_last_error": (c_char_p, []),
     "maomi_version": (c_char_p, []),
@@ -42,6 +67,12 @@ _SIGS = {
     "maomi_normalize_l2": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_edge_window": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "maomi_fresnel_q1d": (c_int, [c_void_p, c_int64, c_double, c_double, c_double, c_void_p]),
+    "maomi_fft2_c64": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "maomi_resize_phase": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                   c_int64, c_int64, c_void_p, c_void_p, c_int64, c_int64,
+                                   c_double, c_void_p, c_void_p, c_void_p]),
+    "maomi_segmented_sites": (c_int, [ctypes.POINTER(SegmentedDesc), c_void_p]),
+    "maomi_segmented_workspace_elems": (c_int64, [c_int64, c_int64, c_int64, c_int64, c_int64]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

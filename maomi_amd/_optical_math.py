@@ -1,0 +1,214 @@
+"""NAOMi-derived optical primitives (call surface of maomi/_optical_math.py).
+
+Host-side mirror: the same names, signatures and dataclass fields, written in
+torch so they run on "cpu" or a ROCm "cuda" device.  The hot path builds its
+pupil with the HIP kernel behind maomi_pupil_gaussian; these functions are
+the API for callers that compose pupils themselves.
+"""
+
+import math
+import typing
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+
+@dataclass
+class VolParams:
+    """Volume description (_optical_math.py:11-17)."""
+    vol_sz: np.ndarray = field(default_factory=lambda: np.array([100, 100, 30], dtype=np.float32))
+    vres: float = 2.0
+    vol_depth: float = 100.0
+    vasc_sz: typing.Optional[np.ndarray] = None
+    verbose: int = 1
+
+
+@dataclass
+class PsfParams:
+    """Beam and objective description (_optical_math.py:87-113)."""
+    NA: float = 0.6
+    objNA: float = 0.8
+    n: float = 1.35
+    lambda_exc: float = 0.92
+    obj_fl: float = 4.5
+    ss: int = 4
+    sampling: int = 1
+    zernikeDst: typing.Optional[np.ndarray] = None
+    zernikeWt: typing.Optional[typing.Any] = None
+    custom_phase: typing.Optional[np.ndarray] = None
+    scatter_sz: typing.Optional[np.ndarray] = None
+    scatter_wt: typing.Optional[np.ndarray] = None
+    type: str = "gaussian"
+    psf_sz: np.ndarray = field(default_factory=lambda: np.array([20.0, 20.0, 50.0], dtype=np.float32))
+    prop_sz: float = 10.0
+    n_diff: float = 0.02
+    fastmask: bool = True
+    propcrop: bool = True
+    taillength: typing.Optional[float] = 50.0
+    bprof: str = "gaussian"
+    length: typing.Optional[float] = None
+    cylscale: typing.Optional[np.ndarray] = None
+    scaling: typing.Optional[str] = None
+
+
+def masked_3DGP_test(grid_sz, l_scale, p_scale, mu=0, bin_mask=1, l_weights=None, device="cpu"):
+    """Spectral Gaussian-process field: a weighted sum of Gaussian spectral envelopes
+    times complex white noise, inverse-FFT'd, real part scaled by 2 p_scale sqrt(N)
+    (_optical_math.py:20-67).  Input generation; uses torch's global RNG."""
+    if l_weights is None:
+        l_weights = torch.ones(l_scale.shape[0], device=device)
+    dims = [int(s) for s in grid_sz]
+    half_pi = np.pi / 2
+    sq = [torch.linspace(-half_pi, half_pi, d, device=device, dtype=torch.float32).pow(2)
+          for d in dims]
+    gx, gy, gz = sq[0].view(-1, 1, 1), sq[1].view(1, -1, 1), sq[2].view(1, 1, -1)
+    env = torch.zeros(dims, device=device, dtype=torch.float32)
+    for i in range(l_scale.shape[0]):
+        k3 = (torch.exp(-gx * l_scale[i, 0] ** 2) * torch.exp(-gy * l_scale[i, 1] ** 2)
+              * torch.exp(-gz * l_scale[i, 2] ** 2))
+        env += torch.prod(l_scale[i]) * (k3 ** 2) * l_weights[i] ** 2
+    env = torch.sqrt(env)
+    noise = (torch.randn(dims, device=device, dtype=torch.float32)
+             + 1j * torch.randn(dims, device=device, dtype=torch.float32))
+    g = torch.fft.ifftshift(noise * env, dim=(0, 1, 2))
+    g = torch.fft.ifftshift(torch.fft.ifftn(g, dim=(0, 1, 2)), dim=(0, 1, 2))
+    return 2 * p_scale * bin_mask * np.sqrt(g.numel()) * torch.real(g) + mu
+
+
+def groupzproject(vol, chunk_size, mode="sum"):
+    """Sum or mean over consecutive groups of `chunk_size` planes (_optical_math.py:70-84)."""
+    Z = vol.shape[2]
+    nz = int(np.ceil(Z / chunk_size))
+    out = torch.zeros((vol.shape[0], vol.shape[1], nz), dtype=vol.dtype, device=vol.device)
+    for i in range(nz):
+        part = vol[:, :, i * chunk_size:min((i + 1) * chunk_size, Z)]
+        if mode == "sum":
+            out[:, :, i] = part.sum(2)
+        elif mode == "mean":
+            out[:, :, i] = part.mean(2)
+    return out
+
+
+def gaussian_beam_size(psf_params: PsfParams, dist: float, apod: float = 2.0) -> np.ndarray:
+    """[R, R, 0] with R = ceil(tan(asin(objNA/n)) * dist * 1.5) * apod (_optical_math.py:116-118)."""
+    r = np.ceil(np.tan(np.arcsin(psf_params.objNA / psf_params.n)) * dist * 1.5) * apod
+    return np.array([r, r, 0], dtype=np.float32)
+
+
+def generate_gaussian_profile(X, Y, rad, aper, k, fl, offset=(0.0, 0.0)):
+    """Focused Gaussian: exp(-|r - r0|^2/rad^2) [r^2 < aper^2] exp(-i k r^2/(2 fl))."""
+    rho2 = X ** 2 + Y ** 2
+    amp = torch.exp(-((X - offset[0]) ** 2 + (Y - offset[1]) ** 2) / (rad ** 2))
+    amp = amp * (rho2 < (aper ** 2))
+    return amp * torch.exp(-1j * k / (2 * fl) * rho2)
+
+
+def zrf(n, m, r):
+    """Radial Zernike polynomial R_n^m(r), accumulated in r's dtype."""
+    out = torch.zeros_like(r)
+    for s in range(int((n - m) / 2) + 1):
+        c = ((-1) ** s) * math.factorial(int(n - s)) / (
+            math.factorial(int(s)) * math.factorial(int((n + m) / 2 - s))
+            * math.factorial(int((n - m) / 2 - s)))
+        out += c * (r ** (n - 2 * s))
+    return out
+
+
+def zidx(i, j):
+    """Noll index i -> radial order n (j = 1) or azimuthal order m (j = 2)."""
+    n = math.ceil(math.sqrt(0.25 + 2 * i) - 1.5)
+    if j == 1:
+        return n
+    if j == 2:
+        rem = i - (n + 1) * n / 2
+        return ((n + 1) % 2) * 2 * math.floor(rem / 2) + (n % 2) * (2 * math.ceil(rem / 2) - 1)
+
+
+def zernike(i, x, y):
+    """Noll-normalised Zernike Z_i on Cartesian coordinates (cos for even i, sin for odd)."""
+    theta = torch.atan2(y, x)
+    r = torch.sqrt(x ** 2 + y ** 2)
+    n, m = zidx(i, 1), zidx(i, 2)
+    if m == 0:
+        return math.sqrt(n + 1) * zrf(n, 0, r)
+    trig = torch.cos(m * theta) if i % 2 == 0 else torch.sin(m * theta)
+    return math.sqrt(2 * (n + 1)) * zrf(n, m, r) * trig
+
+
+def generate_zernike(psf_params, offset=(0.0, 0.0)):
+    """Zernike weights scaled to metres (waves * lambda), or 0 when absent."""
+    if psf_params.zernikeWt is None:
+        return 0
+    return np.array(psf_params.zernikeWt) * psf_params.lambda_exc * 1e-6
+
+
+def apply_zernike(Uin, X, Y, k, abb):
+    """U * exp(i k (sum_i abb_i Z_{i+1} - mean)) on normalised pupil coordinates."""
+    if isinstance(abb, (int, float)) and abb == 0:
+        return Uin
+    phase = torch.zeros_like(X, dtype=torch.float32)
+    for i in np.nonzero(abb)[0]:
+        phase += abb[i] * zernike(i + 1, X, Y)
+    phase = phase - torch.mean(phase)
+    return Uin * torch.exp(1j * k * phase)
+
+
+def generate_bessel_profile(X, Y, rad, width, k, fl, bprof_type="gaussian", aper=float("inf"),
+                            offset=(0.0, 0.0)):
+    """Annular (Bessel-beam) pupil with uniform / posax / negax / gaussian ring profiles."""
+    rho = torch.sqrt((X - offset[0]) ** 2 + (Y - offset[1]) ** 2)
+    if bprof_type == "uniform":
+        amp = (((rho - rad) < width / 2) & ((rho - rad) > -width / 2)).to(torch.float32)
+    elif bprof_type in ("posax", "negax"):
+        w = width / 1.133
+        d = (rho - rad) if bprof_type == "posax" else (rad - rho)
+        t = d + w / math.sqrt(2)
+        amp = t * torch.exp(-((t / w) ** 2)) * (t > 0)
+    else:
+        w = width / (2 * math.sqrt(math.log(2)))
+        amp = torch.exp(-(((rho - rad) / w) ** 2))
+    rho2 = X ** 2 + Y ** 2
+    amp = amp * (rho2 < aper ** 2)
+    return amp * torch.exp(-1j * k / (2 * fl) * rho2)
+
+
+def generate_ba(vol_params: VolParams, psf_params: PsfParams, device=torch.device("cpu")):
+    """Back-aperture field (_optical_math.py:232-296): Gaussian, cylindrical or Bessel
+    profile on the N x N pupil grid, then a custom phase (nearest, mapped to the
+    excitation radius) or the Zernike screen."""
+    if vol_params.vasc_sz is None:
+        vol_params.vasc_sz = (gaussian_beam_size(psf_params, vol_params.vol_depth + vol_params.vol_sz[2] / 2)
+                              + vol_params.vol_sz + np.array([0, 0, 1]) * vol_params.vol_depth)
+    fl = float(psf_params.obj_fl / 1000.0)
+    D2 = float(1e-6 * (1.0 / vol_params.vres) / psf_params.ss)
+    N = np.float32(1e-6 * (vol_params.vasc_sz[:2] - vol_params.vol_sz[:2]) / D2)
+    D1 = float(max(gaussian_beam_size(psf_params, fl * 1e6)) / 1e6 / min(N))
+    nre = float(psf_params.n)
+    rad = float(np.tan(np.arcsin(psf_params.NA / nre)) * fl)
+    objrad = float(np.tan(np.arcsin(psf_params.objNA / nre)) * fl)
+    k = 2 * nre * np.pi / float(psf_params.lambda_exc * 1e-6)
+    xr = torch.arange(-N[0] / 2, N[0] / 2, device=device) * D1
+    yr = torch.arange(-N[1] / 2, N[1] / 2, device=device) * D1
+    Y, X = torch.meshgrid(yr, xr, indexing="ij")
+    if psf_params.type == "bessel":
+        length = psf_params.length if psf_params.length is not None else 60.0
+        width = 1.4e-10 * np.tan(np.arcsin(psf_params.NA / nre)) / (fl * 1e-6 * 2 * length)
+        U = generate_bessel_profile(X, Y, rad, width, k, fl, psf_params.bprof or "gaussian",
+                                    aper=objrad)
+    elif psf_params.type == "cylindrical":
+        cs = psf_params.cylscale if psf_params.cylscale is not None else np.array([1.0, 1.0])
+        U = generate_gaussian_profile(X / cs[0], Y / cs[1], rad, objrad, k, fl)
+    else:
+        U = generate_gaussian_profile(X, Y, rad, objrad, k, fl)
+    if psf_params.custom_phase is not None:
+        cp = torch.tensor(psf_params.custom_phase, dtype=torch.float32, device=device)[None, None]
+        grid = torch.stack((X / rad, Y / rad), dim=-1)[None]
+        ph = F.grid_sample(cp, grid, mode="nearest", padding_mode="zeros", align_corners=True).squeeze()
+        U = U * torch.exp(1j * ph * 2 * np.pi)
+    elif psf_params.zernikeWt is not None:
+        abb = generate_zernike(psf_params)
+        if isinstance(abb, torch.Tensor) or np.any(abb != 0):
+            U = apply_zernike(U, X / objrad, Y / objrad, k, abb)
+    return U

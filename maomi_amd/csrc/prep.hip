@@ -389,3 +389,65 @@ MAOMI_API int maomi_fresnel_q1d(void* q, int64_t N, double dx, double dz, double
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }
+
+// ---------------------------------------------------------------- phase resize
+// resize_xy (maomi/numerics.py:15-39) of a raw (X, Y, Z) field by the supersampling
+// factor, X then Y, from host-built tap tables, times the phase scale, written as
+// (Z, Xo, Yo) screens (batched.py:52-60 without the complex exponential).
+namespace {
+__global__ void resize_x_kernel(const float* __restrict__ src, int64_t YZ, const int32_t* __restrict__ idx,
+                                const float* __restrict__ w, int m, int K, float* __restrict__ tmp) {
+  const int64_t tot = (int64_t)m * YZ;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int xo = (int)(p / YZ);
+    const int64_t r = p % YZ;
+    float acc = 0.f;
+    for (int k = 0; k < K; ++k) acc += src[(int64_t)idx[xo * K + k] * YZ + r] * w[xo * K + k];
+    tmp[p] = acc;
+  }
+}
+__global__ void resize_y_kernel(const float* __restrict__ tmp, int m0, int64_t Y, int64_t Z,
+                                const int32_t* __restrict__ idx, const float* __restrict__ w, int m1,
+                                int K, float scale, float* __restrict__ dst) {
+  const int64_t tot = (int64_t)Z * m0 * m1;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int yo = (int)(p % m1);
+    const int xo = (int)((p / m1) % m0);
+    const int64_t z = p / ((int64_t)m0 * m1);
+    float acc = 0.f;
+    for (int k = 0; k < K; ++k) acc += tmp[((int64_t)xo * Y + idx[yo * K + k]) * Z + z] * w[yo * K + k];
+    dst[p] = acc * scale;
+  }
+}
+}  // namespace
+
+MAOMI_API int maomi_resize_phase(const float* src, int64_t X, int64_t Y, int64_t Z,
+                                 const int32_t* idx0, const float* w0, int64_t m0, int64_t K0,
+                                 const int32_t* idx1, const float* w1, int64_t m1, int64_t K1,
+                                 double scale, float* tmp, float* dst, maomi_stream_t stream) {
+  if (!src || !idx0 || !w0 || !idx1 || !w1 || !tmp || !dst || X < 1 || Y < 1 || Z < 1 || m0 < 1 ||
+      m1 < 1 || K0 < 1 || K1 < 1) {
+    maomi_set_error("maomi_resize_phase: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(resize_x_kernel, dim3(nblocks(m0 * Y * Z)), dim3(256), 0, st, src, Y * Z, idx0,
+                     w0, (int)m0, (int)K0, tmp);
+  MAOMI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(resize_y_kernel, dim3(nblocks(Z * m0 * m1)), dim3(256), 0, st, tmp, (int)m0, Y,
+                     Z, idx1, w1, (int)m1, (int)K1, (float)scale, dst);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_fft2_c64(void* u, int64_t B, int64_t N, int64_t inverse, const void* tw,
+                             maomi_stream_t stream) {
+  if (!u || !tw || B < 1 || N < 2 || N > 4096) {
+    maomi_set_error("maomi_fft2_c64: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  return maomi_fft2_generic((float2*)u, (int)B, (int)N, inverse ? 1 : 0, (const float2*)tw,
+                            (hipStream_t)stream);
+}

@@ -1,0 +1,254 @@
+"""MATLAB-compatible numerics of the optical stage (call surface of maomi/numerics.py).
+
+* resize_xy / resize_xy_region / translate_xy: the reference semantics, in torch
+  (plan-time helpers); the hot phase preparation runs maomi_resize_phase.
+* resize_plan / resample_matrix: the host-built tap tables that the HIP kernels
+  consume (the reference builds the same tables in NumPy, native.py:27-48).
+* fresnel: scaled multi-screen Fresnel propagation with HIP FFTs (maomi_fft2_c64).
+* v4_interpolate: MATLAB griddata 'v4' with the FP64 Green build and evaluation
+  on the GPU (maomi_v4_green_matrix_f64, maomi_v4_apply_f64) and the FP64 dense
+  solve from torch.linalg.
+"""
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def _cubic(x):
+    """Keys cubic kernel, a = -0.5."""
+    a = np.abs(x)
+    inner = ((1.5 * a - 2.5) * a * a + 1) * (a <= 1)
+    outer = (((-0.5 * a + 2.5) * a - 4) * a + 2) * ((a > 1) & (a < 2))
+    return inner + outer
+
+
+def resize_plan(n, scale, start=0, count=None):
+    """Tap indices and weights of imresize bicubic along one axis of length n.
+
+    Output samples start..start+count-1 sit at centres (i + 1/2)/scale - 1/2;
+    downscaling widens the kernel by 1/scale (antialiasing); indices reflect
+    symmetrically at the borders; weights are normalised per output sample.
+    """
+    m = int(np.ceil(n * scale)) if count is None else count
+    centres = (np.arange(start, start + m) + 0.5) / scale - 0.5
+    shrink = min(scale, 1)
+    width = 4 / shrink
+    idx = np.floor(centres - width / 2).astype(int)[:, None] + np.arange(int(np.ceil(width)) + 2)[None]
+    w = _cubic((centres[:, None] - idx) * shrink) * shrink
+    w /= w.sum(1, keepdims=True)
+    idx %= 2 * n
+    idx = np.where(idx < n, idx, 2 * n - 1 - idx)
+    return idx, w
+
+
+def resize_xy(a, scale):
+    """imresize(a, scale, 'bicubic') over the first two axes (numerics.py:15-39)."""
+    out = a
+    for axis in (0, 1):
+        idx, w = resize_plan(out.shape[axis], scale)
+        ids = torch.as_tensor(idx, device=out.device)
+        wt = torch.as_tensor(w, device=out.device, dtype=out.real.dtype)
+        moved = out.movedim(axis, 0)
+        g = moved[ids]
+        out = (g * wt.reshape(wt.shape + (1,) * (moved.ndim - 1))).sum(1).movedim(0, axis)
+    return out
+
+
+def resize_xy_region(a, scale, starts, shape, *, device=None):
+    """Region [starts, starts + shape) of resize_xy(a, scale), reading only its halo."""
+    plans = []
+    for axis in (0, 1):
+        idx, w = resize_plan(a.shape[axis], scale, starts[axis], shape[axis])
+        plans.append((idx, w, int(idx.min()), int(idx.max()) + 1))
+    out = a[plans[0][2]:plans[0][3], plans[1][2]:plans[1][3]].to(device or a.device)
+    for axis, (idx, w, lo, _) in enumerate(plans):
+        moved = out.movedim(axis, 0)
+        ids = torch.as_tensor(idx - lo, device=out.device)
+        wt = torch.as_tensor(w, device=out.device, dtype=out.real.dtype)
+        acc = None
+        for j in range(idx.shape[1]):
+            if not np.any(w[:, j]):
+                continue
+            term = moved.index_select(0, ids[:, j]) * wt[:, j].reshape((-1,) + (1,) * (moved.ndim - 1))
+            acc = term if acc is None else acc + term
+        out = acc.movedim(0, axis)
+    return out
+
+
+def translate_xy(a, shift):
+    """imtranslate(a, [shift shift]) with bilinear weights and zero fill: out[i, j] = a(i - s, j - s)."""
+    k = int(np.floor(shift))
+    f = float(shift - k)
+
+    def shifted(t, axis, s):
+        n = t.shape[axis]
+        res = torch.zeros_like(t)
+        if abs(s) >= n:
+            return res
+        if s >= 0:
+            res.narrow(axis, s, n - s).copy_(t.narrow(axis, 0, n - s))
+        else:
+            res.narrow(axis, 0, n + s).copy_(t.narrow(axis, -s, n + s))
+        return res
+
+    tmp = (1 - f) * shifted(a, 0, k) + f * shifted(a, 0, k + 1)
+    return (1 - f) * shifted(tmp, 1, k) + f * shifted(tmp, 1, k + 1)
+
+
+def resample_matrix(n, ss):
+    """translate(ss/2 - 1/2) followed by imresize(1/ss), as a dense (ceil(n/ss), n) matrix."""
+    scale = 1 / ss
+    shift = ss / 2 - 0.5
+    idx, w = resize_plan(n, scale)
+    mat = np.zeros((idx.shape[0], n))
+    for i in range(idx.shape[0]):
+        for j, wt in zip(idx[i], w[i]):
+            base = int(np.floor(j - shift))
+            frac = j - shift - base
+            if 0 <= base < n:
+                mat[i, base] += wt * (1 - frac)
+            if 0 <= base + 1 < n:
+                mat[i, base + 1] += wt * frac
+    return mat
+
+
+_TW = {}
+
+
+def twiddles(n, device):
+    """Device table W_n^j (cached per size and device)."""
+    key = (int(n), str(device))
+    if key not in _TW:
+        t = torch.empty(n, dtype=torch.complex64, device=device)
+        nat.check(nat.lib().maomi_twiddles(nat.ptr(t), n, nat.stream_handle(device)), "twiddles")
+        _TW[key] = t
+    return _TW[key]
+
+
+def fft2(u, inverse=False):
+    """Unnormalised 2-D DFT over the last two (equal) axes of a complex64 CUDA tensor."""
+    if u.dtype != torch.complex64 or not u.is_cuda:
+        raise ValueError("fft2 needs a complex64 tensor on a ROCm device")
+    h, w = u.shape[-2:]
+    if h != w:
+        raise ValueError("fft2 needs square fields")
+    out = u.contiguous().clone()
+    batch = out.numel() // (h * w)
+    nat.check(nat.lib().maomi_fft2_c64(nat.ptr(out), batch, h, int(inverse),
+                                       nat.ptr(twiddles(h, u.device)), nat.stream_handle(u.device)),
+              "fft2")
+    return out
+
+
+def fresnel(u, wavelength, dx, z, phi, nidx, *, crop=None, save_from=0, return_all=False):
+    """Scaled multi-screen Fresnel propagation (numerics.py:156-210) on the GPU.
+
+    Square even fields; shift-free FFT order (exact for even sizes).  u (..., X, Y);
+    phi (..., X, Y, S) with S = len(z) or len(z) - 1.
+    """
+    dx, z = np.asarray(dx, float).ravel(), np.asarray(z, float).ravel()
+    if len(z) < 2 or len(dx) != len(z) or np.any(np.diff(z) <= 0):
+        raise ValueError("Need increasing z and one dx per plane")
+    if phi.shape[-1] not in (len(z), len(z) - 1):
+        raise ValueError("Wrong number of phase screens")
+    h, w = u.shape[-2:]
+    if h != w or h % 2:
+        raise ValueError("fresnel on the GPU needs square fields of even size")
+    rdt = u.real.dtype
+    x = torch.arange(-h / 2, h / 2, device=u.device, dtype=rdt)[:, None]
+    y = torch.arange(-w / 2, w / 2, device=u.device, dtype=rdt)[None, :]
+    lam = float(wavelength) / float(nidx)
+    k = 2 * np.pi / lam
+    dz = np.diff(z)
+    sc = dx[1:] / dx[:-1]
+    u = u * torch.exp(1j * k * ((x * dx[0]) ** 2 + (y * dx[0]) ** 2) * (1 - sc[0]) / (2 * dz[0])) * phi[..., 0]
+    saved = []
+
+    def keep(i, f):
+        if return_all and i >= save_from:
+            saved.append(f[..., crop[0], crop[1]].clone() if crop else f.clone())
+
+    keep(0, u)
+    cache = {}
+    for i in range(len(dz)):
+        key = (float(dz[i]), float(dx[i]), float(sc[i]))
+        if key not in cache:
+            q = torch.exp(-1j * np.pi * lam * dz[i] * ((x / (h * dx[i])) ** 2 + (y / (w * dx[i])) ** 2) / sc[i])
+            cache[key] = torch.fft.ifftshift(q) / (h * w)
+        u = fft2(cache[key] * fft2(u / sc[i]), inverse=True)
+        if i + 1 < phi.shape[-1]:
+            u = u * phi[..., i + 1]
+        keep(i + 1, u)
+    out = u * torch.exp(1j * k / 2 * (sc[-1] - 1) / (sc[-1] * dz[-1]) * ((x * dx[-1]) ** 2 + (y * dx[-1]) ** 2))
+    return (out, torch.stack(saved, dim=-1)) if return_all else out
+
+
+def _green_cpu(x, y):
+    d = np.sqrt(((x[:, None, :] - y[None, :, :]) ** 2).sum(-1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = d * d * (np.log(d) - 1)
+    g[d == 0] = 0
+    return g
+
+
+def v4_interpolate(points, values, queries, chunk=16384, *, backend="auto", device=None):
+    """MATLAB griddata(..., 'v4') in FP64 with one solve shared by all value columns.
+
+    backend "auto" and "cuda" evaluate on the GPU (Green matrix, dense FP64 solve,
+    Green apply); "cpu" keeps the reference's NumPy evaluator for GPU-less callers.
+    """
+    if backend not in ("auto", "cpu", "cuda"):
+        raise ValueError("backend must be auto, cpu or cuda")
+    if not isinstance(chunk, (int, np.integer)) or chunk < 1:
+        raise ValueError("chunk must be a positive integer")
+    p, v, q = np.asarray(points, float), np.asarray(values, float), np.asarray(queries, float)
+    if p.ndim != 2 or p.shape[1] != 2 or not len(p):
+        raise ValueError("points must be a nonempty (N, 2) array")
+    if v.ndim not in (1, 2) or len(v) != len(p) or (v.ndim == 2 and v.shape[1] == 0):
+        raise ValueError("values must have shape (N,) or (N, C) with C > 0")
+    if q.ndim < 1 or q.shape[-1] != 2:
+        raise ValueError("queries must have shape (..., 2)")
+    shape = q.shape[:-1] + v.shape[1:]
+    if len(p) == 1:
+        return np.broadcast_to(v[0], shape).copy()
+    flat = q.reshape(-1, 2)
+    if not len(flat):
+        return np.empty(shape, dtype=np.float64)
+    if backend == "cpu":
+        from scipy.spatial.distance import cdist
+        coef = np.linalg.solve(_green_cpu(p, p), v)
+        res = np.empty((len(flat),) + v.shape[1:])
+        step = min(chunk, max(1, 8_388_608 // len(p)))
+        for s in range(0, len(flat), step):
+            d2 = cdist(flat[s:s + step], p, "sqeuclidean")
+            with np.errstate(divide="ignore", invalid="ignore"):
+                g = d2 * (0.5 * np.log(d2) - 1)
+            g[d2 == 0] = 0
+            res[s:s + step] = g @ coef
+        return res.reshape(shape)
+    if not torch.cuda.is_available():
+        raise RuntimeError("v4_interpolate backend 'cuda' needs a ROCm GPU")
+    dev = torch.device(device or "cuda")
+    out = v4_gpu(p, v.reshape(len(p), -1), flat, dev, chunk=chunk)
+    return out.cpu().numpy().reshape(shape)
+
+
+def v4_gpu(points, values, flat_queries, device, chunk=1 << 20):
+    """Device FP64 v4: returns a (Q, C) float64 tensor on `device`."""
+    lib = nat.lib()
+    st = nat.stream_handle(device)
+    p = torch.as_tensor(np.ascontiguousarray(points, np.float64), device=device)
+    v = torch.as_tensor(values, dtype=torch.float64, device=device).reshape(len(p), -1).contiguous()
+    q = torch.as_tensor(np.ascontiguousarray(flat_queries, np.float64), device=device)
+    P, C = v.shape
+    G = torch.empty((P, P), dtype=torch.float64, device=device)
+    nat.check(lib.maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G), st), "green matrix")
+    coef = torch.linalg.solve(G, v).contiguous()
+    out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
+    for s in range(0, q.shape[0], chunk):
+        qs, os_ = q[s:s + chunk], out[s:s + chunk]
+        nat.check(lib.maomi_v4_apply_f64(nat.ptr(p), nat.ptr(coef), nat.ptr(qs), nat.ptr(os_), P,
+                                         qs.shape[0], C, st), "v4 apply")
+    return out

@@ -1,0 +1,112 @@
+"""GPU parity of the reference-geometry stage (OpticalPlan / propagate / run_fixed).
+
+Compared with (i) the CPU oracle's propagate on the same inputs and (ii) the
+reference's own outputs recorded in tests/golden (rescaled by the recorded
+ratio of the reference's float32 base-beam norm to the FP64 norm).  The bar
+is the reference's own GPU-vs-CPU tolerance, rel L2 < 2e-5
+(tests/test_fixed.py:147-172 of the reference).
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.conftest import rel_l2
+from tests.golden.make_golden import golden_raw_fields
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+KEYS = ("psf_3d", "psf", "psf_raw", "mask_raw", "energies")
+TAIL_KEYS = ("convmask", "weight", "psfZ", "mask_raw")
+
+
+def _cfgs(kw):
+    from maomi_amd.optics import OpticalConfig
+    kw = {k: tuple(v) if isinstance(v, list) else v for k, v in kw.items()}
+    return OpticalConfig(**kw), O.OracleConfig(**kw)
+
+
+@pytest.mark.parametrize("name", ["scene2p", "scene_tf", "scene1p", "partial"])
+def test_propagate_matches_oracle_and_reference(gpu_lib, golden, name):
+    from maomi_amd.optics import propagate
+    meta = json.loads((GOLD / "propagate_cases.json").read_text())[name]
+    cfg, ocfg = _cfgs(meta["config"])
+    raw = golden_raw_fields(O.field_shapes(ocfg), seed=meta["seed"])
+    pts = O.locations(ocfg.volume_um, meta["count"])
+    print(f"[segmented] {name}: {len(pts)} sites", flush=True)
+    res = propagate(cfg, raw, pts, device="cuda")
+    ref = O.propagate(ocfg, raw, pts)
+    f = (meta["base_norm_f64"] / meta["base_norm_ref"]) ** (2 * O.excitation_order(ocfg))
+    for key in KEYS:
+        e = rel_l2(res[key], ref[key])
+        print(f"  {key}: vs oracle {e:.2e}, vs reference {rel_l2(res[key] * f, golden[f'prop_{name}_{key}']):.2e}")
+        assert e < 2e-5, key
+        assert rel_l2(res[key] * f, golden[f"prop_{name}_{key}"]) < 2e-5, key
+    for t in ("psfT", "psfB"):
+        for key in TAIL_KEYS:
+            scale = 1.0 if key == "psfZ" else f
+            assert rel_l2(res[t][key], ref[t][key]) < 2e-5, (t, key)
+            assert rel_l2(np.asarray(res[t][key]) * scale, golden[f"prop_{name}_{t}_{key}"]) < 2e-5
+        np.testing.assert_allclose(res[t]["mask"].mean(), 1, rtol=1e-12)
+
+
+def test_demo_geometry_custom_phase_matches_reference(gpu_lib, golden):
+    """The reference's own GPU parity case: demo geometry, no scattering, custom phase."""
+    from maomi_amd.batched import OpticalPlan
+    from maomi_amd.fixed import assemble
+    kw = dict(volume_um=(16.0, 16.0, 10.0), depth_um=20.0, vres=2.0, tail_um=10.0)
+    cfg, ocfg = _cfgs(kw)
+    raw = [np.zeros(s, np.float32) for s in O.field_shapes(ocfg)]
+    pts = O.locations(ocfg.volume_um, 3)
+    plan = OpticalPlan(cfg, raw, uin=golden["demo_uin"], device="cuda")
+    assert (plan.n, plan.n2) == (224, 144)
+    data = plan.evaluate(pts, batch_size=4, tails=True)
+    res = assemble(cfg, data, (16, 16), mask_mode="matlab_v4", z_positions=plan.z_positions,
+                   cut=plan.cut)
+    nref, n64 = golden["demo_base_norms"]
+    f = (n64 / nref) ** 4
+    ref = O.propagate(ocfg, raw, pts, uin=golden["demo_uin"])
+    for key in ("psf", "mask_raw", "psf_raw"):
+        assert rel_l2(res[key], ref[key]) < 2e-5
+        assert rel_l2(res[key] * f, golden[f"demo_{key}"]) < 2e-5
+    for t in ("psfT", "psfB"):
+        for key in TAIL_KEYS:
+            scale = 1.0 if key == "psfZ" else f
+            assert rel_l2(np.asarray(res[t][key]) * scale, golden[f"demo_{t}_{key}"]) < 2e-5
+
+
+def test_graph_replay_and_cache_match_evaluate(gpu_lib):
+    from maomi_amd.batched import OpticalPlan
+    meta = json.loads((GOLD / "propagate_cases.json").read_text())["scene2p"]
+    cfg, ocfg = _cfgs(meta["config"])
+    raw = golden_raw_fields(O.field_shapes(ocfg), seed=meta["seed"])
+    plan = OpticalPlan(cfg, raw, device="cuda")
+    pts = O.locations(ocfg.volume_um, 3)
+    ref = plan.evaluate(pts)["psfs"]
+    for _ in range(3):  # several replays, eager work in between
+        g = plan.evaluate_graph(pts, batch_size=4)
+        np.testing.assert_array_equal(g["psfs"], ref)
+        assert g["graph_padding_slots"] == 3
+        plan.evaluate(pts[:2])
+    np.testing.assert_array_equal(plan.cached(pts), ref)
+    assert plan.paths_evaluated >= 9 * 4
+
+
+def test_rejections(gpu_lib):
+    from maomi_amd.batched import OpticalPlan
+    meta = json.loads((GOLD / "propagate_cases.json").read_text())["scene2p"]
+    cfg, ocfg = _cfgs(meta["config"])
+    raw = golden_raw_fields(O.field_shapes(ocfg), seed=meta["seed"])
+    with pytest.raises(ValueError, match="CPU"):
+        OpticalPlan(cfg, raw, device="cpu")
+    plan = OpticalPlan(cfg, raw, device="cuda")
+    with pytest.raises(ValueError, match="Outside FOV"):
+        plan.evaluate([[-1.0, 0.0]])
+    with pytest.raises(ValueError, match="supersampled"):
+        plan.evaluate([[0.3, 0.0]])
+    import torch
+    with pytest.raises(ValueError, match="complex64"):
+        OpticalPlan(cfg, raw, uin=torch.zeros((92, 92), dtype=torch.complex128), device="cuda")
