@@ -144,7 +144,9 @@ typedef struct maomi_segmented_desc {
 } maomi_segmented_desc;
 
 int maomi_segmented_sites(const maomi_segmented_desc* desc, maomi_stream_t stream);
-int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy, int64_t X);
+/* complex64 elements of workspace per persistent block; TT is the tail output side. */
+int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy, int64_t X,
+                                        int64_t TT);
 
 /* ---------------------------------------------------------------------------
  * Beam preparation (once per run, inside the timed "plan" stage).

@@ -5,10 +5,12 @@ operation of the package raises if the HIP library is missing or a call fails.
 """
 
 import ctypes
+import os
 from ctypes import c_char_p, c_double, c_int, c_int64, c_void_p
 from pathlib import Path
 
-_PATH = Path(__file__).resolve().with_name("libmaomi_amd.so")
+_PATH = Path(os.environ.get("MAOMI_AMD_LIBRARY") or
+             Path(__file__).resolve().with_name("libmaomi_amd.so"))
 _LIB = None
 
 
@@ -72,7 +74,8 @@ _SIGS = {
                                    c_int64, c_int64, c_void_p, c_void_p, c_int64, c_int64,
                                    c_double, c_void_p, c_void_p, c_void_p]),
     "maomi_segmented_sites": (c_int, [ctypes.POINTER(SegmentedDesc), c_void_p]),
-    "maomi_segmented_workspace_elems": (c_int64, [c_int64, c_int64, c_int64, c_int64, c_int64]),
+    "maomi_segmented_workspace_elems": (c_int64, [c_int64, c_int64, c_int64, c_int64, c_int64,
+                                                  c_int64]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

@@ -135,7 +135,13 @@ class OpticalPlan:
             raise ValueError("Outside FOV")
         if not np.allclose(off, np.round(off)):
             raise ValueError("Points must lie on supersampled grid")
-        return pts, np.round(off).astype(np.int32)
+        off = np.round(off).astype(np.int32)
+        if len(off):
+            for k, ph in enumerate(self.phases):
+                size = self.n2 if k == 1 else self.n
+                if off[:, 0].max() + size > ph.shape[1] or off[:, 1].max() + size > ph.shape[2]:
+                    raise ValueError(f"site windows exceed the {'ABC'[k]} phase field {tuple(ph.shape[1:])}")
+        return pts, off
 
     def _outputs(self, P, tails):
         d = self.device
@@ -192,7 +198,8 @@ class OpticalPlan:
             desc.bot_conv, desc.bot_prof = outs["bot_conv"].data_ptr(), outs["bot_prof"].data_ptr()
             desc.bot_w = outs["bot_w"].data_ptr()
         stride = int(nat.lib().maomi_segmented_workspace_elems(self.n, self.n2, *self.fshape,
-                                                               self.mfx.shape[0]))
+                                                               self.mfx.shape[0],
+                                                               self.mt.shape[0]))
         need = grid * stride
         if self._work is None or self._work.numel() < need:
             self._work = torch.empty(need, dtype=torch.complex64, device=self.device)

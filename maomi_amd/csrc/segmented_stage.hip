@@ -264,8 +264,11 @@ __global__ __launch_bounds__(NT) void segmented_sites_kernel(SegArgs a) {
 }  // namespace
 
 MAOMI_API int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy,
-                                                  int64_t X) {
-  const int64_t floats = n2 * n2 + fx * fy + X * (fy > n2 ? fy : n2) + 64;
+                                                  int64_t X, int64_t TT) {
+  // field + A's last plane + B's last plane (complex), then tail accumulator,
+  // focal staging and the resample intermediate (floats)
+  const int64_t tmp = X * fy > TT * n2 ? X * fy : TT * n2;
+  const int64_t floats = n2 * n2 + fx * fy + tmp + 64;
   return 2 * n * n + n2 * n2 + (floats + 1) / 2;
 }
 
@@ -308,11 +311,12 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
   a.psf = d->psf; a.top_conv = d->top_conv; a.top_prof = d->top_prof; a.top_w = d->top_w;
   a.bot_conv = d->bot_conv; a.bot_prof = d->bot_prof; a.bot_w = d->bot_w;
   a.work = (float2*)d->work; a.work_stride = d->work_stride;
-  if (a.work_stride < maomi_segmented_workspace_elems(n, n2, d->fx, d->fy, d->X)) {
+  if (a.work_stride < maomi_segmented_workspace_elems(n, n2, d->fx, d->fy, d->X, d->TT)) {
     maomi_set_error("maomi_segmented_sites: work_stride too small");
     return MAOMI_EINVAL;
   }
-  a.L = (int)(3584 / n);
+  // two line buffers of L x n complex plus tables, kept under 60 KiB of LDS
+  a.L = (int)((61440 - 16 * n - 4 * n - 96) / (16 * n));
   if (a.L < 1) a.L = 1;
   const size_t shm = ((size_t)2 * a.L * n + 2 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
                      (NT / 64) * sizeof(double) + 16;
