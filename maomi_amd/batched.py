@@ -160,7 +160,7 @@ class OpticalPlan:
                      bot_w=torch.empty(P, dtype=torch.float64, device=d))
         return o
 
-    def _desc(self, off_x, off_y, order, P, tails, outs, grid):
+    def _desc(self, off_x, off_y, order, P, tails, outs, grid, work=None):
         cfg = self.config
         desc = nat.SegmentedDesc()
         for k in range(3):
@@ -201,9 +201,13 @@ class OpticalPlan:
                                                                self.mfx.shape[0],
                                                                self.mt.shape[0]))
         need = grid * stride
-        if self._work is None or self._work.numel() < need:
-            self._work = torch.empty(need, dtype=torch.complex64, device=self.device)
-        desc.work = self._work.data_ptr()
+        if work is None:
+            if self._work is None or self._work.numel() < need:
+                self._work = torch.empty(need, dtype=torch.complex64, device=self.device)
+            work = self._work
+        if work.numel() < need:
+            raise ValueError("workspace too small")
+        desc.work = work.data_ptr()
         desc.work_stride = stride
         desc.grid = grid
         return desc
@@ -249,7 +253,13 @@ class OpticalPlan:
         oy = torch.zeros(batch_size, dtype=torch.int32, device=d)
         order = torch.arange(batch_size, dtype=torch.int32, device=d)
         outs = self._outputs(batch_size, False)
-        desc = self._desc(ox, oy, order, batch_size, False, outs, self._grid(batch_size))
+        grid = self._grid(batch_size)
+        stride = int(nat.lib().maomi_segmented_workspace_elems(self.n, self.n2, *self.fshape,
+                                                               self.mfx.shape[0],
+                                                               self.mt.shape[0]))
+        # the graph owns every buffer its launch reads or writes
+        gwork = torch.empty(grid * stride, dtype=torch.complex64, device=d)
+        desc = self._desc(ox, oy, order, batch_size, False, outs, grid, work=gwork)
         stream = torch.cuda.Stream(device=d)
         stream.wait_stream(torch.cuda.current_stream(d))
         with torch.cuda.stream(stream):
@@ -259,7 +269,7 @@ class OpticalPlan:
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
             nat.check(nat.lib().maomi_segmented_sites(desc, nat.stream_handle(d)), "capture")
-        self.graphs[batch_size] = (graph, (ox, oy), outs["psf"], desc)
+        self.graphs[batch_size] = (graph, (ox, oy), outs["psf"], (desc, order, outs, gwork))
 
     @torch.inference_mode()
     def evaluate_graph(self, points_um, *, batch_size=32):

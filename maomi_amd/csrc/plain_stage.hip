@@ -16,7 +16,24 @@
 
 namespace {
 
-constexpr int NT = 512;  // threads per block
+// Workgroup barrier that orders LDS only: global loads and stores stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+#ifndef MAOMI_SITES_NT
+#define MAOMI_SITES_NT 512
+#endif
+#ifndef MAOMI_SITES_WAVES
+#define MAOMI_SITES_WAVES 0
+#endif
+#ifndef MAOMI_SITES_PREFETCH
+#define MAOMI_SITES_PREFETCH 0
+#endif
+constexpr int NT = MAOMI_SITES_NT;  // threads per block
+constexpr bool kPrefetch = MAOMI_SITES_PREFETCH != 0;  // register prefetch of the next tile
 
 template <int N> struct ClassIdx;
 template <> struct ClassIdx<64> { static constexpr int v = 0; };
@@ -87,21 +104,41 @@ __device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
     const int64_t pslice = (int64_t)s * a.H;
 
     // ---------------- row pass: [IFFT_y] -> screen, |u|^4 -> [FFT_y]
+    // Barriers inside a pass only order LDS; each thread's mask pixels and the
+    // phase volume need no cross-thread ordering, and the next tile's rows are
+    // prefetched into registers while the current tile is transformed.
+    const bool p1 = tid < L * N1;
+    const int l1 = tid / N1, n1 = tid % N1;
+    float2 pre[N2];
+    if (kPrefetch && !first && p1) {
+      const float2* src = fld + (int64_t)l1 * N + n1;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
+    }
     for (int t = 0; t < N / L; ++t) {
       const int row0 = t * L;
       if (!first) {
-        if (tid < L * N1) {  // stage 1 of the inverse transform along y
-          const int l = tid / N1, n1 = tid % N1;
-          const float2* src = fld + (int64_t)(row0 + l) * N + n1;
+        if (p1) {  // stage 1 of the inverse transform along y
           float2 v[N2];
+          if (kPrefetch) {
 #pragma unroll
-          for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
+            for (int n2 = 0; n2 < N2; ++n2) v[n2] = pre[n2];
+            if (t + 1 < N / L) {
+              const float2* src = fld + (int64_t)(row0 + L + l1) * N + n1;
+#pragma unroll
+              for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
+            }
+          } else {
+            const float2* src = fld + (int64_t)(row0 + l1) * N + n1;
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
+          }
           fftk::fft<N2, true>(v);
-          float2* dst = sm.x + l * LS + n1 * (N2 + 1);
+          float2* dst = sm.x + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
           for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmulc(v[k2], sm.tw[n1 * k2]);
         }
-        __syncthreads();
+        lds_barrier();
       }
       float2 v[N1];
       const bool p2 = tid < L * N2;
@@ -115,7 +152,7 @@ __device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
         } else {
           const float2* src = sm.x + l * LS + k2;
 #pragma unroll
-          for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[n1 * (N2 + 1)];
+          for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
           fftk::fft<N1, true>(v);
         }
         const float ex = sm.eg[x];
@@ -144,14 +181,14 @@ __device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
           for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
         }
       }
-      __syncthreads();
+      lds_barrier();
       if (!last) {
         if (p2) {
           float2* dst = sm.x + l * LS + k2 * (N1 + 1);
 #pragma unroll
           for (int j = 0; j < N1; ++j) dst[j] = v[j];
         }
-        __syncthreads();
+        lds_barrier();
         if (tid < L * N1) {  // stage 2 of the forward transform along y
           const int l3 = tid / N1, k2p = tid % N1;
           float2 w[N2];
@@ -163,61 +200,82 @@ __device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
 #pragma unroll
           for (int k1 = 0; k1 < N2; ++k1) dst[N1 * k1] = w[k1];
         }
-        __syncthreads();
+        lds_barrier();
       }
     }
+    __syncthreads();  // the row pass's stores are read by other threads below
     if (last) break;
 
     // ---------------- column pass: FFT_x -> * Q -> IFFT_x
-    for (int t = 0; t < N / L; ++t) {
-      const int col0 = t * L;
-      if (tid < L * N1) {
-        const int l = tid % L, n1 = tid / L;
-        const float2* src = fld + (int64_t)n1 * N + col0 + l;
-        float2 v[N2];
+    {
+      const bool c1 = tid < L * N1;
+      const int lc = tid % L, nc = tid / L;
+      float2 cpre[N2];
+      if (kPrefetch && c1) {
+        const float2* src = fld + (int64_t)nc * N + lc;
 #pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[(int64_t)N1 * N * n2];
-        fftk::fft<N2, false>(v);
-        float2* dst = sm.x + l * LS + n1 * (N2 + 1);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmul(v[k2], sm.tw[n1 * k2]);
+        for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
       }
-      __syncthreads();
-      float2 v[N1];
-      const bool p2 = tid < L * N2;
-      const int l = tid % L, k2 = tid / L;
-      if (p2) {
-        const float2* src = sm.x + l * LS + k2;
+      for (int t = 0; t < N / L; ++t) {
+        const int col0 = t * L;
+        if (c1) {
+          float2 v[N2];
+          if (kPrefetch) {
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[n1 * (N2 + 1)];
-        fftk::fft<N1, false>(v);
-        const float2 qc = sm.q[col0 + l];
+            for (int n2 = 0; n2 < N2; ++n2) v[n2] = cpre[n2];
+            if (t + 1 < N / L) {
+              const float2* src = fld + (int64_t)nc * N + col0 + L + lc;
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
-        fftk::fft<N1, true>(v);
+              for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
+            }
+          } else {
+            const float2* src = fld + (int64_t)nc * N + col0 + lc;
 #pragma unroll
-        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+            for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[(int64_t)N1 * N * n2];
+          }
+          fftk::fft<N2, false>(v);
+          float2* dst = sm.x + lc * LS + nc * (N2 + 1);
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmul(v[k2], sm.tw[nc * k2]);
+        }
+        lds_barrier();
+        float2 v[N1];
+        const bool p2 = tid < L * N2;
+        const int l = tid % L, k2 = tid / L;
+        if (p2) {
+          const float2* src = sm.x + l * LS + k2;
+#pragma unroll
+          for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+          fftk::fft<N1, false>(v);
+          const float2 qc = sm.q[col0 + l];
+#pragma unroll
+          for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
+          fftk::fft<N1, true>(v);
+#pragma unroll
+          for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+        }
+        lds_barrier();
+        if (p2) {
+          float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+#pragma unroll
+          for (int j = 0; j < N1; ++j) dst[j] = v[j];
+        }
+        lds_barrier();
+        if (tid < L * N1) {
+          const int l3 = tid % L, k2p = tid / L;
+          float2 w[N2];
+          const float2* src = sm.x + l3 * LS + k2p;
+#pragma unroll
+          for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
+          fftk::fft<N2, true>(w);
+          float2* dst = fld + (int64_t)k2p * N + col0 + l3;
+#pragma unroll
+          for (int k1 = 0; k1 < N2; ++k1) dst[(int64_t)N1 * N * k1] = w[k1];
+        }
+        lds_barrier();
       }
-      __syncthreads();
-      if (p2) {
-        float2* dst = sm.x + l * LS + k2 * (N1 + 1);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) dst[j] = v[j];
-      }
-      __syncthreads();
-      if (tid < L * N1) {
-        const int l3 = tid % L, k2p = tid / L;
-        float2 w[N2];
-        const float2* src = sm.x + l3 * LS + k2p;
-#pragma unroll
-        for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
-        fftk::fft<N2, true>(w);
-        float2* dst = fld + (int64_t)k2p * N + col0 + l3;
-#pragma unroll
-        for (int k1 = 0; k1 < N2; ++k1) dst[(int64_t)N1 * N * k1] = w[k1];
-      }
-      __syncthreads();
     }
+    __syncthreads();  // the column pass's stores are read by other threads in the next row pass
   }
   const double tot = block_sum_double(esum, sm.red);
   if (tid == 0) a.energy[site] = tot;

@@ -64,7 +64,7 @@ def test_demo_geometry_custom_phase_matches_reference(gpu_lib, golden):
     plan = OpticalPlan(cfg, raw, uin=golden["demo_uin"], device="cuda")
     assert (plan.n, plan.n2) == (224, 144)
     data = plan.evaluate(pts, batch_size=4, tails=True)
-    res = assemble(cfg, data, (16, 16), mask_mode="matlab_v4", z_positions=plan.z_positions,
+    res = assemble(cfg, data, (32, 32), mask_mode="matlab_v4", z_positions=plan.z_positions,
                    cut=plan.cut)
     nref, n64 = golden["demo_base_norms"]
     f = (n64 / nref) ** 4
