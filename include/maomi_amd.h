@@ -68,6 +68,7 @@ typedef struct maomi_sites_desc {
   int64_t P, H, W, Z;
   int64_t grid;               /* persistent blocks (0 = choose)                          */
   int64_t uniform_n;          /* N shared by every site, or 0 for a mixed-N launch       */
+  int64_t block_threads;      /* 512 (32-line tiles at N=256) or 256 (16-line); 0 = 512  */
   int64_t class_edge[5];      /* offsets for N = 64,128,256,512,(unused)                 */
   int64_t class_q[5];
   int64_t class_tw[5];

@@ -23,7 +23,7 @@ class SitesDesc(ctypes.Structure):
         ("order", c_void_p), ("mask", c_void_p), ("mask_off", c_void_p), ("energy", c_void_p),
         ("workspace", c_void_p), ("scratch_stride", c_int64),
         ("P", c_int64), ("H", c_int64), ("W", c_int64), ("Z", c_int64),
-        ("grid", c_int64), ("uniform_n", c_int64),
+        ("grid", c_int64), ("uniform_n", c_int64), ("block_threads", c_int64),
         ("class_edge", c_int64 * 5), ("class_q", c_int64 * 5), ("class_tw", c_int64 * 5),
     ]
 

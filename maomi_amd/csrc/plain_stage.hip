@@ -23,16 +23,9 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-#ifndef MAOMI_SITES_NT
-#define MAOMI_SITES_NT 512
-#endif
-#ifndef MAOMI_SITES_WAVES
-#define MAOMI_SITES_WAVES 0
-#endif
 #ifndef MAOMI_SITES_PREFETCH
 #define MAOMI_SITES_PREFETCH 0
 #endif
-constexpr int NT = MAOMI_SITES_NT;  // threads per block
 constexpr bool kPrefetch = MAOMI_SITES_PREFETCH != 0;  // register prefetch of the next tile
 
 template <int N> struct ClassIdx;
@@ -41,8 +34,12 @@ template <> struct ClassIdx<128> { static constexpr int v = 1; };
 template <> struct ClassIdx<256> { static constexpr int v = 2; };
 template <> struct ClassIdx<512> { static constexpr int v = 3; };
 
-constexpr int kLdsSlots = fftk::cmax(fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2),
-                                     fftk::cmax(fftk::Tile<256, NT>::LDS2, fftk::Tile<512, NT>::LDS2));
+// Block shapes: NT threads transform L = NT / max(N1, N2) lines per LDS tile.
+template <int NT>
+constexpr int lds_slots() {
+  return fftk::cmax(fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2),
+                    fftk::cmax(fftk::Tile<256, NT>::LDS2, fftk::Tile<512, NT>::LDS2));
+}
 constexpr int kNmax = 512;
 
 struct SitesArgs {
@@ -68,16 +65,17 @@ struct SitesArgs {
   int64_t cedge[5], cq[5], ctw[5];
 };
 
+template <int NT>
 struct Smem {
-  float2 x[kLdsSlots];  // line exchange buffer
-  float2 tw[kNmax];     // W_N^j
-  float2 q[kNmax];      // 1-D Fresnel factor (FFT order, times 1/N)
-  float eg[kNmax];      // 1-D edge window
+  float2 x[lds_slots<NT>()];  // line exchange buffer
+  float2 tw[kNmax];           // W_N^j
+  float2 q[kNmax];            // 1-D Fresnel factor (FFT order, times 1/N)
+  float eg[kNmax];            // 1-D edge window
   double red[NT / 64];
 };
 
-template <int N>
-__device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
+template <int N, int NT>
+__device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
   constexpr int C = ClassIdx<N>::v;
@@ -283,22 +281,23 @@ __device__ void run_site(const SitesArgs& a, int site, Smem& sm) {
 }
 
 // One field side for every site of the launch.
-template <int N>
+template <int N, int NT>
 __global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
-  __shared__ Smem sm;
-  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N>(a, a.order[i], sm);
+  __shared__ Smem<NT> sm;
+  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT>(a, a.order[i], sm);
 }
 
 // Sites of mixed field sides in one launch (the order list puts large sites first).
+template <int NT>
 __global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
-  __shared__ Smem sm;
+  __shared__ Smem<NT> sm;
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
     const int site = a.order[i];
     switch (a.site_n[site]) {
-      case 64: run_site<64>(a, site, sm); break;
-      case 128: run_site<128>(a, site, sm); break;
-      case 256: run_site<256>(a, site, sm); break;
-      case 512: run_site<512>(a, site, sm); break;
+      case 64: run_site<64, NT>(a, site, sm); break;
+      case 128: run_site<128, NT>(a, site, sm); break;
+      case 256: run_site<256, NT>(a, site, sm); break;
+      case 512: run_site<512, NT>(a, site, sm); break;
       default: break;  // rejected on the host
     }
   }
@@ -354,7 +353,7 @@ MAOMI_API int64_t maomi_sites_default_grid(void) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256>, NT, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256, 512>, 512, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -394,18 +393,30 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   int64_t grid = d->grid > 0 ? d->grid : maomi_sites_default_grid();
   if (grid <= 0) grid = 256;
   if (grid > d->P) grid = d->P;
-  const dim3 g((unsigned)grid), b(NT);
-  hipStream_t st = (hipStream_t)stream;
-  switch (d->uniform_n) {
-    case 64: hipLaunchKernelGGL(sites_kernel<64>, g, b, 0, st, a); break;
-    case 128: hipLaunchKernelGGL(sites_kernel<128>, g, b, 0, st, a); break;
-    case 256: hipLaunchKernelGGL(sites_kernel<256>, g, b, 0, st, a); break;
-    case 512: hipLaunchKernelGGL(sites_kernel<512>, g, b, 0, st, a); break;
-    case 0: hipLaunchKernelGGL(sites_mixed_kernel, g, b, 0, st, a); break;
-    default:
-      maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");
-      return MAOMI_EUNSUPPORTED;
+  const int nt = d->block_threads ? (int)d->block_threads : 512;
+  if (nt != 512 && nt != 256) {
+    maomi_set_error("maomi_persistent_sites: block_threads must be 0, 256 or 512");
+    return MAOMI_EUNSUPPORTED;
   }
+  const dim3 g((unsigned)grid), b(nt);
+  hipStream_t st = (hipStream_t)stream;
+#define MAOMI_LAUNCH(NTV)                                                                  \
+  switch (d->uniform_n) {                                                                  \
+    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV>), g, b, 0, st, a); break;           \
+    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV>), g, b, 0, st, a); break;         \
+    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV>), g, b, 0, st, a); break;         \
+    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV>), g, b, 0, st, a); break;         \
+    case 0: hipLaunchKernelGGL((sites_mixed_kernel<NTV>), g, b, 0, st, a); break;          \
+    default:                                                                               \
+      maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512"); \
+      return MAOMI_EUNSUPPORTED;                                                           \
+  }
+  if (nt == 512) {
+    MAOMI_LAUNCH(512)
+  } else {
+    MAOMI_LAUNCH(256)
+  }
+#undef MAOMI_LAUNCH
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

@@ -1,0 +1,236 @@
+"""Uneven plain stage: sites with mixed field sides and depths in ONE persistent launch.
+
+Every site s has its own field side N_s in {128, 256, 512} and slice count S_s
+(focal depth d_s = S_s dz, focus on the last plane).  All sites run in one
+launch of the mixed-size persistent kernel (maomi_persistent_sites with
+uniform_n = 0); blocks take sites in the order of a list sorted by cost
+(S N^2 log N, largest first) and the launch is captured once in a HIP graph
+and replayed.  An autotuner picks the grid size, the tile shape (256- or
+512-thread blocks: 16- or 32-line LDS tiles) and the ordering policy by
+timing the captured candidates on the device.
+"""
+
+import itertools
+import time
+from dataclasses import dataclass, replace
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .plain import SiteTables, _CLASSES, _SLOT, base_beam, make_pupil, v4_masks
+from .workloads import PlainConfig, beam_parameters, zernike_weights
+
+
+@dataclass(frozen=True)
+class UnevenConfig:
+    base: PlainConfig = PlainConfig(fov=512, n=512, slices=60, sites_per_axis=65, corr_vox=3.0)
+    sizes: tuple = (128, 256, 512)
+    size_probs: tuple = (0.5, 0.4, 0.1)
+    depth_um: tuple = (20.0, 120.0)
+
+
+CONFIG3 = UnevenConfig()
+
+
+def draw_sites(ucfg: UnevenConfig):
+    """Seeded per-site field sides and slice counts (depth ~ U(lo, hi), S = round(d / dz))."""
+    cfg = ucfg.base
+    rng = np.random.default_rng(cfg.seed + 2)
+    P = cfg.num_sites
+    depth = rng.uniform(*ucfg.depth_um, P)
+    slices = np.clip(np.round(depth / cfg.dz_um).astype(int), 1, cfg.slices)
+    sizes = rng.choice(np.asarray(ucfg.sizes), size=P, p=np.asarray(ucfg.size_probs))
+    return sizes.astype(np.int64), slices.astype(np.int64)
+
+
+def site_windows(cfg: PlainConfig, centres, sizes):
+    """Top-left of each centred window in a volume padded by cfg.n (the largest side)."""
+    return np.asarray(centres, np.int64) + (cfg.n - np.asarray(sizes)[:, None]) // 2
+
+
+def bytes_per_site(sizes, slices):
+    n2 = np.asarray(sizes, np.float64) ** 2
+    return (20 * np.asarray(slices) + 12) * n2
+
+
+class UnevenStage:
+    def __init__(self, cfg: PlainConfig, index_volume, centres, sizes, slices, *, device="cuda"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("UnevenStage needs a ROCm GPU")
+        self.cfg, self.device = cfg, torch.device(device)
+        self.centres = np.asarray(centres, np.int64)
+        self.sizes = np.asarray(sizes, np.int64)
+        self.slices = np.asarray(slices, np.int64)
+        P = len(self.centres)
+        if not (len(self.sizes) == len(self.slices) == P):
+            raise ValueError("one size and one slice count per site")
+        for n in np.unique(self.sizes):
+            if int(n) not in _CLASSES or n > cfg.n:
+                raise ValueError(f"site field side {n} unsupported (max {cfg.n})")
+        if np.any(self.slices < 1) or np.any(self.slices > cfg.slices):
+            raise ValueError("slice counts must lie in [1, cfg.slices]")
+        if np.any(self.centres < 0) or np.any(self.centres > cfg.fov):
+            raise ValueError("site centres must lie in the field of view")
+        vol = torch.as_tensor(index_volume, dtype=torch.float32, device=self.device)
+        exp = (cfg.padded, cfg.padded, cfg.slices * cfg.voxels_per_slice)
+        if tuple(vol.shape) != exp:
+            raise ValueError(f"index volume must have shape {exp}")
+        self.volume = vol.contiguous()
+        self.windows = site_windows(cfg, self.centres, self.sizes)
+        self.bp = dict(beam_parameters(cfg), wvl_um=cfg.wavelength_um)
+        self.zern = zernike_weights(cfg)
+        d = self.device
+        self.site_x = torch.as_tensor(self.windows[:, 0].astype(np.int32), device=d)
+        self.site_y = torch.as_tensor(self.windows[:, 1].astype(np.int32), device=d)
+        self.site_n = torch.as_tensor(self.sizes.astype(np.int32), device=d)
+        self.site_s = torch.as_tensor(self.slices.astype(np.int32), device=d)
+        moff = np.concatenate(([0], np.cumsum(self.sizes ** 2)[:-1]))
+        self.mask_off = torch.as_tensor(moff.astype(np.int64), device=d)
+        self.mask_total = int(np.sum(self.sizes ** 2))
+        cost = self.slices * self.sizes ** 2 * np.log2(self.sizes)
+        self.orders = {
+            "lpt": np.argsort(-cost, kind="stable"),
+            "natural": np.arange(P),
+            "class_interleaved": np.lexsort((-self.slices, -self.sizes)),
+        }
+        self.tuned = None
+        self.timing = {}
+        self._graph = None
+
+    def plan(self):
+        cfg, d = self.cfg, self.device
+        lib = nat.lib()
+        st = nat.stream_handle(d)
+        Z, H = cfg.slices, cfg.padded
+        # buffers are allocated once: a captured launch keeps reading the same addresses
+        if not hasattr(self, "phase"):
+            self.phase = torch.empty((Z, H, H), dtype=torch.float32, device=d)
+        nat.check(lib.maomi_phase_prepare(nat.ptr(self.volume), H, H, Z * cfg.voxels_per_slice,
+                                          cfg.voxels_per_slice, float(cfg.n0),
+                                          float(self.bp["k0dz"]), nat.ptr(self.phase), None, None,
+                                          None, st), "phase prepare")
+        uniq = sorted({int(n) for n in self.sizes})
+        tables = SiteTables(uniq, self.bp["dx"], self.bp["dz"],
+                            float(self.bp["wvl"]) / float(self.bp["nr"]), d)
+        if not hasattr(self, "tables"):
+            self.tables = tables
+        else:
+            for name in ("edge", "q", "tw"):
+                getattr(self.tables, name).copy_(getattr(tables, name))
+        # one base beam per (N, S): focus on the last plane, S dz below the entry plane
+        keys = sorted({(int(n), int(s)) for n, s in zip(self.sizes, self.slices)})
+        blocks, offs, pos = [], {}, 0
+        for n, s in keys:
+            c = _CLASSES[n]
+            sub = replace(cfg, n=n, slices=s, focus_um=s * cfg.dz_um)
+            bp = dict(beam_parameters(sub), wvl_um=cfg.wavelength_um)
+            pup, _ = make_pupil(n, bp, self.zern, d)
+            tw = self.tables.tw[c * _SLOT:c * _SLOT + n]
+            blocks.append(base_beam(pup, bp, self.tables.edge2d[n], tw, d).reshape(-1))
+            offs[(n, s)] = pos
+            pos += n * n
+        if not hasattr(self, "bases"):
+            self.bases = torch.empty(pos, dtype=torch.complex64, device=d)
+            boff = np.array([offs[(int(n), int(s))] for n, s in zip(self.sizes, self.slices)])
+            self.base_off = torch.as_tensor(boff.astype(np.int64), device=d)
+            self.masks = torch.empty(self.mask_total, dtype=torch.float32, device=d)
+            self.energies = torch.empty(len(self.sizes), dtype=torch.float64, device=d)
+        torch.cat(blocks, out=self.bases)
+        return self
+
+    def _desc(self, order, grid, threads, work):
+        cfg = self.cfg
+        desc = nat.SitesDesc()
+        desc.bases, desc.base_off = self.bases.data_ptr(), self.base_off.data_ptr()
+        desc.phase = self.phase.data_ptr()
+        desc.edge, desc.q1d = self.tables.edge.data_ptr(), self.tables.q.data_ptr()
+        desc.twiddle = self.tables.tw.data_ptr()
+        desc.site_x, desc.site_y = self.site_x.data_ptr(), self.site_y.data_ptr()
+        desc.site_n, desc.site_s = self.site_n.data_ptr(), self.site_s.data_ptr()
+        desc.order = order.data_ptr()
+        desc.mask, desc.mask_off = self.masks.data_ptr(), self.mask_off.data_ptr()
+        desc.energy = self.energies.data_ptr()
+        desc.workspace = work.data_ptr()
+        desc.scratch_stride = int(self.sizes.max()) ** 2
+        desc.P, desc.H, desc.W, desc.Z = len(self.sizes), cfg.padded, cfg.padded, cfg.slices
+        desc.grid, desc.uniform_n, desc.block_threads = grid, 0, threads
+        for i in range(5):
+            desc.class_edge[i] = desc.class_q[i] = desc.class_tw[i] = self.tables.offsets[i]
+        return desc
+
+    def _capture(self, order_name, grid, threads):
+        d = self.device
+        order = torch.as_tensor(self.orders[order_name].astype(np.int32), device=d)
+        work = torch.empty(grid * int(self.sizes.max()) ** 2, dtype=torch.complex64, device=d)
+        desc = self._desc(order, grid, threads, work)
+        lib = nat.lib()
+        nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "warm-up launch")
+        torch.cuda.synchronize(d)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "captured launch")
+        return dict(graph=graph, keep=(order, work, desc), order=order_name, grid=grid,
+                    threads=threads)
+
+    def autotune(self, grids=None, threads=(512, 256), orders=("lpt", "class_interleaved"),
+                 reps=2):
+        """Time captured candidates and keep the fastest (grid, block shape, ordering)."""
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        grids = grids or (cus, 2 * cus)
+        results = []
+        best = None
+        for g, t, o in itertools.product(grids, threads, orders):
+            g = min(int(g), len(self.sizes))
+            cand = self._capture(o, g, t)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ms = []
+            for _ in range(reps):
+                ev0.record()
+                cand["graph"].replay()
+                ev1.record()
+                torch.cuda.synchronize(self.device)
+                ms.append(ev0.elapsed_time(ev1))
+            cand["ms"] = min(ms)
+            results.append({k: cand[k] for k in ("grid", "threads", "order", "ms")})
+            if best is None or cand["ms"] < best["ms"]:
+                best = cand
+        self._graph = best
+        self.tuned = {k: best[k] for k in ("grid", "threads", "order", "ms")}
+        self.tune_log = results
+        return self.tuned
+
+    def propagate(self):
+        if self._graph is None:
+            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+            self._graph = self._capture("lpt", min(cus, len(self.sizes)), 512)
+        self._graph["graph"].replay()
+        return self.masks, self.energies
+
+    def site_mask(self, i):
+        off, n = int(self.mask_off[i]), int(self.sizes[i])
+        return self.masks[off:off + n * n].reshape(n, n)
+
+    def aggregate(self):
+        cfg = self.cfg
+        m = v4_masks(self.centres.astype(np.float64), self.energies, (cfg.fov, cfg.fov),
+                     self.device)[..., 0]
+        self.mask_raw, self.mask_mean = m, m.mean()
+        self.mask = m / self.mask_mean
+        return self.mask
+
+    def run(self):
+        t = time.perf_counter()
+        self.plan()
+        torch.cuda.synchronize(self.device)
+        self.timing["plan"] = time.perf_counter() - t
+        t = time.perf_counter()
+        self.propagate()
+        torch.cuda.synchronize(self.device)
+        self.timing["propagation"] = time.perf_counter() - t
+        t = time.perf_counter()
+        self.aggregate()
+        torch.cuda.synchronize(self.device)
+        self.timing["aggregation"] = time.perf_counter() - t
+        self.timing["total"] = sum(self.timing[k] for k in ("plan", "propagation", "aggregation"))
+        return self

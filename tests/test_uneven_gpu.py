@@ -1,0 +1,58 @@
+"""Uneven sites (mixed field sides and slice counts) in one captured persistent launch."""
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.conftest import rel_l2
+
+
+
+
+def _oracle_masks(cfg, vol, windows, sizes, slices, zw):
+    bases = {}
+    for n, s in sorted({(int(a), int(b)) for a, b in zip(sizes, slices)}):
+        sub = replace(cfg, n=n, slices=s, focus_um=s * cfg.dz_um)
+        bases.setdefault(n, {})[s] = O.plain_beam(sub, zw)[0]
+    k0dz = 2 * np.pi / cfg.wavelength_um * cfg.dz_um
+    phase = O.plain_phase(vol, cfg.n0, k0dz)
+    return O.plain_stage_masks(bases, phase, windows, wvl=np.float32(cfg.wavelength_um * 1e-6),
+                               nr=np.float32(cfg.n0), dx=np.float32(cfg.dx_um * 1e-6),
+                               dz=cfg.dz_um * 1e-6, sizes=sizes, slices=slices)
+
+
+@pytest.mark.gpu
+def test_mixed_sizes_one_launch_matches_oracle(gpu_lib):
+    from maomi_amd.uneven import UnevenStage
+    from maomi_amd.workloads import PlainConfig, index_volume, zernike_weights
+    cfg = PlainConfig(fov=64, n=256, slices=12, sites_per_axis=3, corr_vox=2.0,
+                      zernike_std_rad=0.05)
+    vol = index_volume(cfg)
+    centres = np.array([[0, 0], [32, 32], [64, 0], [16, 48], [48, 16], [64, 64]])
+    sizes = np.array([256, 128, 64, 128, 256, 64])
+    slices = np.array([12, 5, 9, 12, 3, 7])
+    print("[uneven] 6 mixed sites", flush=True)
+    st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
+    tuned = st.autotune(grids=(2, 6), reps=1)
+    print("autotuned:", tuned, flush=True)
+    for _ in range(2):  # replays with eager work between them
+        st.propagate()
+        st.energies.sum().item()
+    ref = _oracle_masks(cfg, vol, st.windows, sizes, slices, zernike_weights(cfg))
+    for i in range(len(sizes)):
+        got = st.site_mask(i).cpu().numpy()
+        assert got.shape == (sizes[i], sizes[i])
+        assert rel_l2(got, ref[i]) <= 1e-5, i
+    np.testing.assert_allclose(st.energies.cpu().numpy(), [m.sum(dtype=np.float64) for m in ref],
+                               rtol=1e-5)
+
+
+def test_uneven_draw_statistics():
+    from maomi_amd.uneven import CONFIG3, draw_sites
+    sizes, slices = draw_sites(CONFIG3)
+    assert len(sizes) == 4225 and set(np.unique(sizes)) <= {128, 256, 512}
+    frac = [np.mean(sizes == n) for n in (128, 256, 512)]
+    assert abs(frac[0] - 0.5) < 0.03 and abs(frac[1] - 0.4) < 0.03 and abs(frac[2] - 0.1) < 0.02
+    assert slices.min() >= 10 and slices.max() <= 60
