@@ -72,6 +72,9 @@ typedef struct maomi_sites_desc {
   int64_t class_edge[5];      /* offsets for N = 64,128,256,512,(unused)                 */
   int64_t class_q[5];
   int64_t class_tw[5];
+  float* psf_out;             /* optional [psf_nz][P][psf_w][psf_w]: |u|^4 of planes      */
+  int64_t psf_z0, psf_nz;     /*   psf_z0 .. psf_z0+psf_nz-1, crop [psf_c0, psf_c0+psf_w)^2 */
+  int64_t psf_c0, psf_w;      /*   of each site window (training targets of the surrogate) */
 } maomi_sites_desc;
 
 int maomi_persistent_sites(const maomi_sites_desc* desc, maomi_stream_t stream);
@@ -214,6 +217,48 @@ int maomi_v4_green_matrix_f64(const double* points, int64_t P, double* G, maomi_
  * out[t][c] = sum_i g(|q_t - p_i|^2) coef[i][c], g(r2) = r2 (ln(r2)/2 - 1), 1 <= C <= 8. */
 int maomi_v4_apply_f64(const double* points, const double* coef, const double* queries,
                        double* out, int64_t P, int64_t Q, int64_t C, maomi_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned PSF surrogate (no counterpart in the reference: the reference has no
+ * ML).  BF16 MFMA training and inference kernels; activations are FP32
+ * channel-major [C][B][H][W], operands of the GEMM are BF16.
+ * ------------------------------------------------------------------------- */
+
+/* C[m][n] = alpha sum_k A(m,k) B(k,n) + beta C[m][n] with A(m,k) = A[m sam + k sak],
+ * B(k,n) = B[k sbk + n sbn] (BF16), C FP32 row stride ldc; v_mfma_f32_16x16x32_bf16. */
+int maomi_gemm_bf16(const void* A, const void* B, float* C, int64_t M, int64_t N, int64_t K,
+                    int64_t sam, int64_t sak, int64_t sbk, int64_t sbn, int64_t ldc, float alpha,
+                    float beta, maomi_stream_t stream);
+/* col[(c k + ky) k + kx][(b Ho + oy) Wo + ox] = X[c][b][s oy + ky - p][s ox + kx - p] (BF16) */
+int maomi_im2col_bf16(const float* X, void* col, int64_t C, int64_t B, int64_t H, int64_t W,
+                      int64_t k, int64_t s, int64_t p, maomi_stream_t stream);
+/* adjoint of im2col: dX = sum of the dcol entries that read each input sample */
+int maomi_col2im(const float* dcol, float* dX, int64_t C, int64_t B, int64_t H, int64_t W,
+                 int64_t k, int64_t s, int64_t p, maomi_stream_t stream);
+/* Y[c][n] = act(Y[c][n] + bias[c]), act = leaky ReLU with negative slope `slope` */
+int maomi_bias_act(float* Y, const float* bias, int64_t C, int64_t n, float slope,
+                   maomi_stream_t stream);
+/* dY <- dY act'(Y) in place; db[c] = sum_n dY[c][n] */
+int maomi_act_bwd(float* dY, const float* Y, int64_t C, int64_t n, float slope, float* db,
+                  maomi_stream_t stream);
+int maomi_to_bf16(const float* x, void* y, int64_t n, maomi_stream_t stream);
+/* nearest 2x upsampling of CB planes of H x W (backward = 2 x 2 sums) */
+int maomi_upsample2(const float* in, float* out, int64_t CB, int64_t H, int64_t W,
+                    int64_t backward, maomi_stream_t stream);
+/* per sample b: loss[b] = |y - t|^2 / |t|^2; grad = 2 (y - t) / |t|^2 / B (grad may be NULL) */
+int maomi_rel_sq_loss(const float* y, const float* t, float* grad, float* loss, int64_t C,
+                      int64_t B, int64_t HW, maomi_stream_t stream);
+/* Adam step (bias-corrected, decoupled weight decay wd) on FP32 w; wb (BF16 copy) may be NULL */
+int maomi_adam(float* w, void* wb, const float* g, float* m, float* v, int64_t n, float lr,
+               float b1, float b2, float eps, int64_t step, float wd, maomi_stream_t stream);
+
+/* Network input per site p: out[c][p][y][x] (c < zbins) = mean of (n - n0) * inv_std over the
+ * pool x pool x (Zv/zbins) voxels of the side x side window at (wx[p], wy[p]) of the
+ * [X][Y][Zv] index volume; out[zbins][p][.][.] = depth[p]. */
+int maomi_patch_features(const float* index, int64_t X, int64_t Y, int64_t Zv, const int32_t* wx,
+                         const int32_t* wy, const float* depth, int64_t P, int64_t side,
+                         int64_t pool, int64_t zbins, double n0, double inv_std, float* out,
+                         maomi_stream_t stream);
 
 #ifdef __cplusplus
 }

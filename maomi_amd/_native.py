@@ -6,7 +6,7 @@ operation of the package raises if the HIP library is missing or a call fails.
 
 import ctypes
 import os
-from ctypes import c_char_p, c_double, c_int, c_int64, c_void_p
+from ctypes import c_char_p, c_double, c_float, c_int, c_int64, c_void_p
 from pathlib import Path
 
 _PATH = Path(os.environ.get("MAOMI_AMD_LIBRARY") or
@@ -25,6 +25,8 @@ class SitesDesc(ctypes.Structure):
         ("P", c_int64), ("H", c_int64), ("W", c_int64), ("Z", c_int64),
         ("grid", c_int64), ("uniform_n", c_int64), ("block_threads", c_int64),
         ("class_edge", c_int64 * 5), ("class_q", c_int64 * 5), ("class_tw", c_int64 * 5),
+        ("psf_out", c_void_p), ("psf_z0", c_int64), ("psf_nz", c_int64), ("psf_c0", c_int64),
+        ("psf_w", c_int64),
     ]
 
 
@@ -76,6 +78,22 @@ _SIGS = {
     "maomi_segmented_sites": (c_int, [ctypes.POINTER(SegmentedDesc), c_void_p]),
     "maomi_segmented_workspace_elems": (c_int64, [c_int64, c_int64, c_int64, c_int64, c_int64,
                                                   c_int64]),
+    "maomi_gemm_bf16": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64,
+                                c_int64, c_int64, c_int64, c_int64, c_int64, c_float, c_float,
+                                c_void_p]),
+    "maomi_im2col_bf16": (c_int, [c_void_p, c_void_p] + [c_int64] * 7 + [c_void_p]),
+    "maomi_col2im": (c_int, [c_void_p, c_void_p] + [c_int64] * 7 + [c_void_p]),
+    "maomi_bias_act": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p]),
+    "maomi_act_bwd": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p, c_void_p]),
+    "maomi_to_bf16": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "maomi_upsample2": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p]),
+    "maomi_rel_sq_loss": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                  c_int64, c_void_p]),
+    "maomi_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float,
+                           c_float, c_float, c_float, c_int64, c_float, c_void_p]),
+    "maomi_patch_features": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                     c_void_p, c_int64, c_int64, c_int64, c_int64, c_double,
+                                     c_double, c_void_p, c_void_p]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

@@ -23,6 +23,7 @@ SOURCES = {
     "prep.hip": ["-ffp-contract=off"],
     "v4.hip": ["-ffp-contract=off"],
     "segmented_stage.hip": [],
+    "surrogate.hip": [],
 }
 
 

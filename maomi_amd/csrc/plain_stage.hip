@@ -63,6 +63,8 @@ struct SitesArgs {
   int64_t scratch_stride;
   int64_t P, H, W, Z;
   int64_t cedge[5], cq[5], ctw[5];
+  float* psf_out;  // optional [psf_nz][P][psf_w][psf_w] planes |u|^4 around focus
+  int psf_z0, psf_nz, psf_c0, psf_w;
 };
 
 template <int NT>
@@ -168,6 +170,12 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
             v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
           }
           const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
+          if (a.psf_out && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
+            const int cx = x - a.psf_c0, cy = y - a.psf_c0;
+            if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
+              a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + site) * a.psf_w + cx) * a.psf_w + cy] =
+                  pw * pw;
+          }
           const float m = wgt * (pw * pw);
           const float acc = first ? m : mrow[N2 * k1] + m;
           mrow[N2 * k1] = acc;
@@ -390,6 +398,13 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   a.scratch = (float2*)d->workspace; a.scratch_stride = d->scratch_stride;
   a.P = d->P; a.H = d->H; a.W = d->W; a.Z = d->Z;
   for (int i = 0; i < 5; ++i) { a.cedge[i] = d->class_edge[i]; a.cq[i] = d->class_q[i]; a.ctw[i] = d->class_tw[i]; }
+  a.psf_out = d->psf_out;
+  a.psf_z0 = (int)d->psf_z0; a.psf_nz = (int)d->psf_nz; a.psf_c0 = (int)d->psf_c0;
+  a.psf_w = (int)d->psf_w;
+  if (a.psf_out && (a.psf_nz < 1 || a.psf_w < 1 || a.psf_z0 < 0 || a.psf_c0 < 0)) {
+    maomi_set_error("maomi_persistent_sites: bad focal-plane output window");
+    return MAOMI_EINVAL;
+  }
   int64_t grid = d->grid > 0 ? d->grid : maomi_sites_default_grid();
   if (grid <= 0) grid = 256;
   if (grid > d->P) grid = d->P;

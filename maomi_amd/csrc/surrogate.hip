@@ -1,0 +1,389 @@
+// Kernels of the learned PSF surrogate: BF16 MFMA GEMM with FP32 accumulation
+// (forward and both backward products of every convolution), im2col / col2im,
+// bias + leaky-ReLU forward and backward, nearest 2x upsampling, the relative
+// squared-error loss and an Adam step on FP32 master weights with BF16 copies.
+//
+// Activations are channel-major: [C][B][H][W], so a convolution is
+//   Y[co][n] = sum_k W[co][k] col[k][n],  k = (ci, ky, kx),  n = (b, y, x),
+// dcol = W^T dY and dW = dY col^T, each one call of the GEMM below.
+#include "common.hpp"
+#include "maomi_amd.h"
+#include <hip/hip_bf16.h>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 32;  // block tile
+constexpr int GT = 256;                   // threads per GEMM block (4 waves, 2 x 2 of 32 x 32)
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+
+// C[m][n] = alpha * sum_k A(m, k) B(k, n) + beta * C[m][n]
+// A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn], C row stride ldc.
+__global__ __launch_bounds__(GT) void gemm_bf16_kernel(const __bf16* __restrict__ A,
+                                                       const __bf16* __restrict__ B,
+                                                       float* __restrict__ C, int M, int N, int K,
+                                                       int64_t sam, int64_t sak, int64_t sbk,
+                                                       int64_t sbn, int64_t ldc, float alpha,
+                                                       float beta) {
+  __shared__ __bf16 As[TM][TK + 8];
+  __shared__ __bf16 Bs[TN][TK + 8];  // stored transposed: Bs[n][k]
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    // 64 x 32 elements of each operand, 8 per thread
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = tid + r * GT;
+      const int mm = e / TK, kk = e % TK;
+      const int gm = m0 + mm, gk = k0 + kk;
+      As[mm][kk] = (gm < M && gk < K) ? A[gm * sam + gk * sak] : to_bf16(0.f);
+      const int nn = e % TN, kb = e / TN;
+      const int gn = n0 + nn, gkb = k0 + kb;
+      Bs[nn][kb] = (gn < N && gkb < K) ? B[gkb * sbk + gn * sbn] : to_bf16(0.f);
+    }
+    __syncthreads();
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      bf16x8 a;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) a[t] = As[wm + 16 * i + row][kq + t];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        bf16x8 b;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) b[t] = Bs[wn + 16 * j + row][kq + t];
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D[4 (lane / 16) + t][lane % 16] of each 16 x 16 tile
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + t;
+        const int gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          float* c = C + gm * ldc + gn;
+          *c = alpha * acc[i][j][t] + (beta != 0.f ? beta * *c : 0.f);
+        }
+      }
+}
+
+// col[(c*k + ky)*k + kx][n], n = (b*Ho + oy)*Wo + ox, from X[c][b][iy][ix] (float -> bf16)
+__global__ void im2col_kernel(const float* __restrict__ X, __bf16* __restrict__ col, int C, int Bn,
+                              int H, int W, int k, int s, int p, int Ho, int Wo) {
+  const int64_t Ncol = (int64_t)Bn * Ho * Wo;
+  const int64_t tot = (int64_t)C * k * k * Ncol;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = e % Ncol;
+    const int r = (int)(e / Ncol);
+    const int kx = r % k, ky = (r / k) % k, c = r / (k * k);
+    const int ox = (int)(n % Wo), oy = (int)((n / Wo) % Ho), b = (int)(n / ((int64_t)Wo * Ho));
+    const int iy = s * oy + ky - p, ix = s * ox + kx - p;
+    float v = 0.f;
+    if (iy >= 0 && iy < H && ix >= 0 && ix < W) v = X[(((int64_t)c * Bn + b) * H + iy) * W + ix];
+    col[e] = to_bf16(v);
+  }
+}
+
+// dX[c][b][iy][ix] = sum over (ky, kx, oy, ox) with iy = s oy + ky - p, ix = s ox + kx - p
+__global__ void col2im_kernel(const float* __restrict__ dcol, float* __restrict__ dX, int C, int Bn,
+                              int H, int W, int k, int s, int p, int Ho, int Wo) {
+  const int64_t tot = (int64_t)C * Bn * H * W;
+  const int64_t Ncol = (int64_t)Bn * Ho * Wo;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(e % W), iy = (int)((e / W) % H);
+    const int b = (int)((e / ((int64_t)W * H)) % Bn), c = (int)(e / ((int64_t)W * H * Bn));
+    float acc = 0.f;
+    for (int ky = 0; ky < k; ++ky) {
+      const int ty = iy + p - ky;
+      if (ty < 0 || ty % s) continue;
+      const int oy = ty / s;
+      if (oy >= Ho) continue;
+      for (int kx = 0; kx < k; ++kx) {
+        const int tx = ix + p - kx;
+        if (tx < 0 || tx % s) continue;
+        const int ox = tx / s;
+        if (ox >= Wo) continue;
+        acc += dcol[(int64_t)((c * k + ky) * k + kx) * Ncol + ((int64_t)b * Ho + oy) * Wo + ox];
+      }
+    }
+    dX[e] = acc;
+  }
+}
+
+// Y[c][n] += bias[c]; leaky ReLU with negative slope `slope` (slope = 1: identity)
+__global__ void bias_act_kernel(float* __restrict__ Y, const float* __restrict__ bias, int C,
+                                int64_t n, float slope) {
+  const int64_t tot = (int64_t)C * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float v = Y[e] + bias[e / n];
+    Y[e] = v > 0.f ? v : slope * v;
+  }
+}
+
+// dZ = dY * act'(Y) in place (Y is the activated output; sign is preserved), and
+// db[c] = sum_n dZ[c][n] (one block per channel).
+__global__ void act_bwd_kernel(float* __restrict__ dY, const float* __restrict__ Y, int64_t n,
+                               float slope, float* __restrict__ db) {
+  __shared__ double red[16];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const int64_t e = (int64_t)c * n + i;
+    const float g = Y[e] > 0.f ? dY[e] : slope * dY[e];
+    dY[e] = g;
+    acc += g;
+  }
+  const double t = block_sum_double(acc, red);
+  if (threadIdx.x == 0) db[c] = (float)t;
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = to_bf16(x[e]);
+}
+
+// nearest 2x: out[c][b][y][x] = in[c][b][y/2][x/2]
+__global__ void up2_fwd_kernel(const float* __restrict__ in, float* __restrict__ out, int64_t CB,
+                               int H, int W) {
+  const int64_t tot = CB * 4 * H * W;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % (2 * W)), y = (int)((e / (2 * W)) % (2 * H));
+    const int64_t cb = e / (4 * (int64_t)H * W);
+    out[e] = in[(cb * H + y / 2) * W + x / 2];
+  }
+}
+
+__global__ void up2_bwd_kernel(const float* __restrict__ dout, float* __restrict__ din, int64_t CB,
+                               int H, int W) {
+  const int64_t tot = CB * H * W;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % W), y = (int)((e / W) % H);
+    const int64_t cb = e / ((int64_t)H * W);
+    const float* d = dout + (cb * 2 * H + 2 * y) * (2 * W) + 2 * x;
+    din[e] = d[0] + d[1] + d[2 * W] + d[2 * W + 1];
+  }
+}
+
+// Per sample b (columns b*HW .. of channel-major [C][B][HW] tensors):
+//   L_b = sum (y - t)^2 / sum t^2;  grad = 2 (y - t) / sum t^2 / B.  loss[b] written.
+__global__ void rel_sq_loss_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                                   float* __restrict__ grad, float* __restrict__ loss, int C,
+                                   int Bn, int HW) {
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  double num = 0.0, den = 0.0;
+  for (int i = threadIdx.x; i < C * HW; i += blockDim.x) {
+    const int c = i / HW, q = i % HW;
+    const int64_t e = ((int64_t)c * Bn + b) * HW + q;
+    const double d = (double)y[e] - t[e];
+    num += d * d;
+    den += (double)t[e] * t[e];
+  }
+  num = block_sum_double(num, red);
+  den = block_sum_double(den, red);
+  const float inv = den > 0.0 ? (float)(1.0 / den) : 0.f;
+  for (int i = threadIdx.x; i < C * HW; i += blockDim.x) {
+    const int c = i / HW, q = i % HW;
+    const int64_t e = ((int64_t)c * Bn + b) * HW + q;
+    if (grad) grad[e] = 2.f * (y[e] - t[e]) * inv / (float)Bn;
+  }
+  if (threadIdx.x == 0) loss[b] = (float)(den > 0.0 ? num / den : 0.0);
+}
+
+// Adam on FP32 master weights; refreshes the BF16 copy.
+__global__ void adam_kernel(float* __restrict__ w, __bf16* __restrict__ wb, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
+                            float b1, float b2, float eps, float c1, float c2, float wd) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float gr = g[e] + wd * w[e];
+    const float mm = b1 * m[e] + (1.f - b1) * gr;
+    const float vv = b2 * v[e] + (1.f - b2) * gr * gr;
+    m[e] = mm;
+    v[e] = vv;
+    const float upd = lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    w[e] -= upd;
+    if (wb) wb[e] = to_bf16(w[e]);
+  }
+}
+
+inline unsigned blocks_for(int64_t n, int bs = 256) {
+  const int64_t b = (n + bs - 1) / bs;
+  return (unsigned)(b < 16384 ? (b > 0 ? b : 1) : 16384);
+}
+
+}  // namespace
+
+MAOMI_API int maomi_gemm_bf16(const void* A, const void* B, float* C, int64_t M, int64_t N,
+                              int64_t K, int64_t sam, int64_t sak, int64_t sbk, int64_t sbn,
+                              int64_t ldc, float alpha, float beta, maomi_stream_t stream) {
+  if (!A || !B || !C || M < 1 || N < 1 || K < 1 || M > (1ll << 30) || N > (1ll << 30) ||
+      K > (1ll << 30)) {
+    maomi_set_error("maomi_gemm_bf16: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  const dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
+  if (grid.y > 65535) { maomi_set_error("maomi_gemm_bf16: M too large"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(GT), 0, (hipStream_t)stream, (const __bf16*)A,
+                     (const __bf16*)B, C, (int)M, (int)N, (int)K, sam, sak, sbk, sbn, ldc, alpha,
+                     beta);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_im2col_bf16(const float* X, void* col, int64_t C, int64_t Bn, int64_t H,
+                                int64_t W, int64_t k, int64_t s, int64_t p, maomi_stream_t stream) {
+  if (!X || !col || C < 1 || Bn < 1 || H < 1 || W < 1 || k < 1 || s < 1 || p < 0) {
+    maomi_set_error("maomi_im2col_bf16: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  const int64_t Ho = (H + 2 * p - k) / s + 1, Wo = (W + 2 * p - k) / s + 1;
+  hipLaunchKernelGGL(im2col_kernel, dim3(blocks_for(C * k * k * Bn * Ho * Wo)), dim3(256), 0,
+                     (hipStream_t)stream, X, (__bf16*)col, (int)C, (int)Bn, (int)H, (int)W, (int)k,
+                     (int)s, (int)p, (int)Ho, (int)Wo);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_col2im(const float* dcol, float* dX, int64_t C, int64_t Bn, int64_t H,
+                           int64_t W, int64_t k, int64_t s, int64_t p, maomi_stream_t stream) {
+  if (!dcol || !dX || C < 1 || Bn < 1 || H < 1 || W < 1 || k < 1 || s < 1 || p < 0) {
+    maomi_set_error("maomi_col2im: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  const int64_t Ho = (H + 2 * p - k) / s + 1, Wo = (W + 2 * p - k) / s + 1;
+  hipLaunchKernelGGL(col2im_kernel, dim3(blocks_for(C * Bn * H * W)), dim3(256), 0,
+                     (hipStream_t)stream, dcol, dX, (int)C, (int)Bn, (int)H, (int)W, (int)k, (int)s,
+                     (int)p, (int)Ho, (int)Wo);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_bias_act(float* Y, const float* bias, int64_t C, int64_t n, float slope,
+                             maomi_stream_t stream) {
+  if (!Y || !bias || C < 1 || n < 1) { maomi_set_error("maomi_bias_act: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(bias_act_kernel, dim3(blocks_for(C * n)), dim3(256), 0, (hipStream_t)stream,
+                     Y, bias, (int)C, n, slope);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_act_bwd(float* dY, const float* Y, int64_t C, int64_t n, float slope,
+                            float* db, maomi_stream_t stream) {
+  if (!dY || !Y || !db || C < 1 || n < 1) { maomi_set_error("maomi_act_bwd: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(act_bwd_kernel, dim3((unsigned)C), dim3(256), 0, (hipStream_t)stream, dY, Y, n,
+                     slope, db);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_to_bf16(const float* x, void* y, int64_t n, maomi_stream_t stream) {
+  if (!x || !y || n < 1) { maomi_set_error("maomi_to_bf16: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, x,
+                     (__bf16*)y, n);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_upsample2(const float* in, float* out, int64_t CB, int64_t H, int64_t W,
+                              int64_t backward, maomi_stream_t stream) {
+  if (!in || !out || CB < 1 || H < 1 || W < 1) { maomi_set_error("maomi_upsample2: bad arguments"); return MAOMI_EINVAL; }
+  if (backward)
+    hipLaunchKernelGGL(up2_bwd_kernel, dim3(blocks_for(CB * H * W)), dim3(256), 0,
+                       (hipStream_t)stream, in, out, CB, (int)H, (int)W);
+  else
+    hipLaunchKernelGGL(up2_fwd_kernel, dim3(blocks_for(CB * 4 * H * W)), dim3(256), 0,
+                       (hipStream_t)stream, in, out, CB, (int)H, (int)W);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_rel_sq_loss(const float* y, const float* t, float* grad, float* loss, int64_t C,
+                                int64_t Bn, int64_t HW, maomi_stream_t stream) {
+  if (!y || !t || !loss || C < 1 || Bn < 1 || HW < 1) { maomi_set_error("maomi_rel_sq_loss: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(rel_sq_loss_kernel, dim3((unsigned)Bn), dim3(256), 0, (hipStream_t)stream, y,
+                     t, grad, loss, (int)C, (int)Bn, (int)HW);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_adam(float* w, void* wb, const float* g, float* m, float* v, int64_t n,
+                         float lr, float b1, float b2, float eps, int64_t step, float wd,
+                         maomi_stream_t stream) {
+  if (!w || !g || !m || !v || n < 1 || step < 1) { maomi_set_error("maomi_adam: bad arguments"); return MAOMI_EINVAL; }
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w,
+                     (__bf16*)wb, g, m, v, n, lr, b1, b2, eps, c1, c2, wd);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+// ---------------------------------------------------------------- input patches
+namespace {
+// out[c][p][y][x], c < zbins: mean over the pool x pool x (z-bin) voxels of
+// (n - n0) * inv_std in the site window; channel zbins holds the site's depth scalar.
+__global__ void patch_kernel(const float* __restrict__ index, int64_t X, int64_t Y, int64_t Zv,
+                             const int32_t* __restrict__ wx, const int32_t* __restrict__ wy,
+                             const float* __restrict__ depth, int P, int side, int pool, int zbins,
+                             float n0, float inv_std, float* __restrict__ out) {
+  const int S = side / pool;
+  const int64_t plane = (int64_t)P * S * S;
+  const int64_t tot = (int64_t)zbins * plane;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int xx = (int)(e % S), yy = (int)((e / S) % S);
+    const int p = (int)((e / ((int64_t)S * S)) % P), c = (int)(e / plane);
+    const int z0 = (int)((int64_t)c * Zv / zbins), z1 = (int)((int64_t)(c + 1) * Zv / zbins);
+    double acc = 0.0;
+    for (int i = 0; i < pool; ++i) {
+      const int64_t gx = wx[p] + (int64_t)yy * pool + i;
+      for (int j = 0; j < pool; ++j) {
+        const int64_t gy = wy[p] + (int64_t)xx * pool + j;
+        const float* col = index + (gx * Y + gy) * Zv;
+        for (int z = z0; z < z1; ++z) acc += (double)col[z] - n0;
+      }
+    }
+    out[e] = (float)(acc / ((double)pool * pool * (z1 - z0)) * inv_std);
+    if (c == 0) out[(int64_t)zbins * plane + (int64_t)p * S * S + (int64_t)yy * S + xx] = depth[p];
+  }
+}
+}  // namespace
+
+MAOMI_API int maomi_patch_features(const float* index, int64_t X, int64_t Y, int64_t Zv,
+                                   const int32_t* wx, const int32_t* wy, const float* depth,
+                                   int64_t P, int64_t side, int64_t pool, int64_t zbins, double n0,
+                                   double inv_std, float* out, maomi_stream_t stream) {
+  if (!index || !wx || !wy || !depth || !out || P < 1 || side < pool || pool < 1 || zbins < 1 ||
+      zbins > Zv || side % pool) {
+    maomi_set_error("maomi_patch_features: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  const int64_t S = side / pool;
+  hipLaunchKernelGGL(patch_kernel, dim3(blocks_for(zbins * P * S * S)), dim3(256), 0,
+                     (hipStream_t)stream, index, X, Y, Zv, wx, wy, depth, (int)P, (int)side,
+                     (int)pool, (int)zbins, (float)n0, (float)inv_std, out);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}

@@ -164,10 +164,23 @@ class PlainStage:
         self.base = base_beam(self.pupil, self.bp, self.tables.edge2d[cfg.n], tw, d)
         return self
 
-    def propagate(self):
+    def propagate(self, subset=None, psf=None):
+        """All sites (or the indices `subset`) in one persistent launch.
+
+        psf = (out, z0, nz, c0, w) also writes |u|^4 of planes z0..z0+nz-1 cropped to
+        [c0, c0+w)^2 into out[nz][P][w][w] (surrogate training targets and audits).
+        """
         cfg, d = self.cfg, self.device
         lib = nat.lib()
-        P = len(self.offsets)
+        if subset is None:
+            site_x, site_y, order = self.site_x, self.site_y, self.order
+            P = len(self.offsets)
+        else:
+            sub = np.asarray(subset, np.int64)
+            P = len(sub)
+            site_x = torch.as_tensor(self.offsets[sub, 0].astype(np.int32), device=d)
+            site_y = torch.as_tensor(self.offsets[sub, 1].astype(np.int32), device=d)
+            order = torch.arange(P, dtype=torch.int32, device=d)
         grid = self.grid or int(lib.maomi_sites_default_grid()) or 256
         grid = min(grid, P)
         stride = cfg.n * cfg.n
@@ -184,9 +197,9 @@ class PlainStage:
         desc.edge = self.tables.edge.data_ptr()
         desc.q1d = self.tables.q.data_ptr()
         desc.twiddle = self.tables.tw.data_ptr()
-        desc.site_x, desc.site_y = self.site_x.data_ptr(), self.site_y.data_ptr()
+        desc.site_x, desc.site_y = site_x.data_ptr(), site_y.data_ptr()
         desc.site_n, desc.site_s = self.site_n.data_ptr(), self.site_s.data_ptr()
-        desc.order = self.order.data_ptr()
+        desc.order = order.data_ptr()
         desc.mask, desc.mask_off = self.masks.data_ptr(), self.mask_off.data_ptr()
         desc.energy = self.energies.data_ptr()
         desc.workspace = self.workspace.data_ptr()
@@ -195,7 +208,16 @@ class PlainStage:
         desc.grid, desc.uniform_n = grid, cfg.n
         for i in range(5):
             desc.class_edge[i] = desc.class_q[i] = desc.class_tw[i] = self.tables.offsets[i]
+        if psf is not None:
+            out, z0, nz, c0, w = psf
+            if tuple(out.shape) != (nz, P, w, w) or out.dtype != torch.float32:
+                raise ValueError("psf output must be float32 (nz, P, w, w)")
+            if z0 < 0 or z0 + nz > cfg.slices + 1 or c0 < 0 or c0 + w > cfg.n:
+                raise ValueError("psf window outside the propagated planes or field")
+            desc.psf_out = out.data_ptr()
+            desc.psf_z0, desc.psf_nz, desc.psf_c0, desc.psf_w = z0, nz, c0, w
         self._desc = desc
+        self._keep = (site_x, site_y, order)
         nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "persistent sites")
         return self.masks, self.energies
 

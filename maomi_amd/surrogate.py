@@ -1,0 +1,293 @@
+"""Learned site-PSF surrogate with audit and fallback (no counterpart in the reference).
+
+A convolutional encoder-decoder maps a site's downsampled refractive-index
+patch (8 depth bins x 64 x 64, plus a focal-depth channel) to its local
+two-photon PSF (9 planes x 33 x 33 around focus).  Every layer runs on the
+hand-written kernels of libmaomi_amd.so: im2col + BF16 MFMA GEMM with FP32
+accumulation for the forward and both backward products, bias + leaky ReLU,
+nearest 2x upsampling, a relative squared-error loss and Adam on FP32 master
+weights with BF16 copies.  The training data are site PSFs produced by the
+propagator itself.  At inference a seeded random fraction of sites is re-run
+with the FP32 propagator; an audited site whose prediction is off by more than
+the tolerance is replaced by the propagator, and when too many audited sites
+fail the whole run falls back to the propagator.
+"""
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def _st(dev):
+    return nat.stream_handle(dev)
+
+
+def gemm(A, B, C, M, N, K, sam, sak, sbk, sbn, ldc, alpha=1.0, beta=0.0):
+    nat.check(nat.lib().maomi_gemm_bf16(nat.ptr(A), nat.ptr(B), nat.ptr(C), M, N, K, sam, sak,
+                                        sbk, sbn, ldc, float(alpha), float(beta),
+                                        _st(C.device)), "gemm bf16")
+
+
+def to_bf16(x, out=None):
+    out = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device) if out is None else out
+    nat.check(nat.lib().maomi_to_bf16(nat.ptr(x.contiguous()), nat.ptr(out), x.numel(),
+                                      _st(x.device)), "to bf16")
+    return out
+
+
+class Conv:
+    """k x k convolution, stride s, padding p, activation leaky ReLU(slope) (slope 1: linear)."""
+
+    def __init__(self, cin, cout, k, s, p, slope, gen, device):
+        self.cin, self.cout, self.k, self.s, self.p, self.slope = cin, cout, k, s, p, slope
+        fan = cin * k * k
+        std = np.sqrt(2.0 / fan) if slope < 1 else np.sqrt(1.0 / fan)
+        w = torch.randn((cout, fan), generator=gen, dtype=torch.float32) * std
+        self.w = w.to(device)
+        self.b = torch.zeros(cout, dtype=torch.float32, device=device)
+        self.wb = to_bf16(self.w)
+        self.gw, self.gb = torch.zeros_like(self.w), torch.zeros_like(self.b)
+        self.state = [torch.zeros_like(self.w), torch.zeros_like(self.w),
+                      torch.zeros_like(self.b), torch.zeros_like(self.b)]
+
+    def params(self):
+        return self.w.numel() + self.b.numel()
+
+    def out_hw(self, H, W):
+        return (H + 2 * self.p - self.k) // self.s + 1, (W + 2 * self.p - self.k) // self.s + 1
+
+    def forward(self, x, train):
+        C, B, H, W = x.shape
+        Ho, Wo = self.out_hw(H, W)
+        n = B * Ho * Wo
+        K = self.cin * self.k * self.k
+        col = torch.empty((K, n), dtype=torch.bfloat16, device=x.device)
+        nat.check(nat.lib().maomi_im2col_bf16(nat.ptr(x), nat.ptr(col), C, B, H, W, self.k, self.s,
+                                              self.p, _st(x.device)), "im2col")
+        y = torch.empty((self.cout, B, Ho, Wo), dtype=torch.float32, device=x.device)
+        gemm(self.wb, col, y, self.cout, n, K, K, 1, n, 1, n)
+        nat.check(nat.lib().maomi_bias_act(nat.ptr(y), nat.ptr(self.b), self.cout, n, self.slope,
+                                           _st(x.device)), "bias act")
+        if train:
+            self.cache = (x.shape, col, y)
+        return y
+
+    def backward(self, dy):
+        (C, B, H, W), col, y = self.cache
+        Ho, Wo = y.shape[2], y.shape[3]
+        n = B * Ho * Wo
+        K = self.cin * self.k * self.k
+        nat.check(nat.lib().maomi_act_bwd(nat.ptr(dy), nat.ptr(y), self.cout, n, self.slope,
+                                          nat.ptr(self.gb), _st(dy.device)), "act bwd")
+        dyb = to_bf16(dy)
+        # dW[co][kk] = sum_n dY[co][n] col[kk][n]
+        gemm(dyb, col, self.gw, self.cout, K, n, n, 1, 1, n, K)
+        # dcol[kk][n] = sum_co W[co][kk] dY[co][n]
+        dcol = torch.empty((K, n), dtype=torch.float32, device=dy.device)
+        gemm(self.wb, dyb, dcol, K, n, self.cout, 1, K, n, 1, n)
+        dx = torch.empty((C, B, H, W), dtype=torch.float32, device=dy.device)
+        nat.check(nat.lib().maomi_col2im(nat.ptr(dcol), nat.ptr(dx), C, B, H, W, self.k, self.s,
+                                         self.p, _st(dy.device)), "col2im")
+        self.cache = None
+        return dx
+
+    def step(self, lr, t, wd=0.0, b1=0.9, b2=0.999, eps=1e-8):
+        lib = nat.lib()
+        st = _st(self.w.device)
+        m, v, mb, vb = self.state
+        nat.check(lib.maomi_adam(nat.ptr(self.w), nat.ptr(self.wb), nat.ptr(self.gw), nat.ptr(m),
+                                 nat.ptr(v), self.w.numel(), lr, b1, b2, eps, t, wd, st), "adam w")
+        nat.check(lib.maomi_adam(nat.ptr(self.b), None, nat.ptr(self.gb), nat.ptr(mb), nat.ptr(vb),
+                                 self.b.numel(), lr, b1, b2, eps, t, 0.0, st), "adam b")
+
+
+class Up2:
+    def forward(self, x, train):
+        C, B, H, W = x.shape
+        y = torch.empty((C, B, 2 * H, 2 * W), dtype=torch.float32, device=x.device)
+        nat.check(nat.lib().maomi_upsample2(nat.ptr(x), nat.ptr(y), C * B, H, W, 0, _st(x.device)),
+                  "upsample")
+        self.shape = x.shape
+        return y
+
+    def backward(self, dy):
+        C, B, H, W = self.shape
+        dx = torch.empty((C, B, H, W), dtype=torch.float32, device=dy.device)
+        nat.check(nat.lib().maomi_upsample2(nat.ptr(dy), nat.ptr(dx), C * B, H, W, 1,
+                                            _st(dy.device)), "upsample bwd")
+        return dx
+
+    def params(self):
+        return 0
+
+    def step(self, *a, **k):
+        pass
+
+
+def default_layers(cin=9, widths=(32, 64, 128, 384, 384, 96, 48), out_ch=9, slope=0.1):
+    """Encoder 64 -> 32 -> 16 -> 8 (two 8x8 convs), decoder 8 -> 16 -> 32, 2x2 conv to 33x33."""
+    w = widths
+    return [("conv", cin, w[0], 3, 2, 1, slope), ("conv", w[0], w[1], 3, 2, 1, slope),
+            ("conv", w[1], w[2], 3, 2, 1, slope), ("conv", w[2], w[3], 3, 1, 1, slope),
+            ("conv", w[3], w[4], 3, 1, 1, slope), ("up",), ("conv", w[4], w[5], 3, 1, 1, slope),
+            ("up",), ("conv", w[5], w[6], 3, 1, 1, slope), ("conv", w[6], out_ch, 2, 1, 1, 1.0)]
+
+
+class SurrogateNet:
+    def __init__(self, layers=None, seed=0, device="cuda"):
+        self.device = torch.device(device)
+        gen = torch.Generator().manual_seed(seed)
+        self.spec = layers or default_layers()
+        self.layers = [Up2() if l[0] == "up" else Conv(*l[1:], gen=gen, device=self.device)
+                       for l in self.spec]
+        self.t = 0
+
+    def num_params(self):
+        return sum(l.params() for l in self.layers)
+
+    def forward(self, x, train=False):
+        for l in self.layers:
+            x = l.forward(x, train)
+        return x
+
+    def backward(self, dy):
+        for l in reversed(self.layers):
+            dy = l.backward(dy)
+        return dy
+
+    def loss(self, y, t, grad=True):
+        C, B, H, W = y.shape
+        g = torch.empty_like(y) if grad else None
+        per = torch.empty(B, dtype=torch.float32, device=y.device)
+        nat.check(nat.lib().maomi_rel_sq_loss(nat.ptr(y), nat.ptr(t), nat.ptr(g), nat.ptr(per), C,
+                                              B, H * W, _st(y.device)), "loss")
+        return per, g
+
+    def train_step(self, x, t, lr, wd=0.0):
+        y = self.forward(x, train=True)
+        per, g = self.loss(y, t)
+        self.backward(g)
+        self.t += 1
+        for l in self.layers:
+            l.step(lr, self.t, wd)
+        return per
+
+    def predict(self, x, batch=512):
+        outs = []
+        for s in range(0, x.shape[1], batch):
+            outs.append(self.forward(x[:, s:s + batch].contiguous()))
+        return torch.cat(outs, 1)
+
+
+def to_cbhw(a):
+    """(B, C, H, W) -> channel-major (C, B, H, W) contiguous."""
+    return a.permute(1, 0, 2, 3).contiguous()
+
+
+def train(net, x, t, *, epochs=10, batch=128, lr=1e-3, wd=0.0, seed=0, log=None):
+    """x, t: channel-major tensors (C, P, H, W) on the device."""
+    P = x.shape[1]
+    rng = np.random.default_rng(seed)
+    hist = []
+    for ep in range(epochs):
+        order = torch.as_tensor(rng.permutation(P), device=x.device)
+        tot, cnt = 0.0, 0
+        lr_ep = lr * 0.5 * (1 + np.cos(np.pi * ep / max(1, epochs)))
+        for s in range(0, P - batch + 1, batch):
+            idx = order[s:s + batch]
+            per = net.train_step(x[:, idx].contiguous(), t[:, idx].contiguous(), lr_ep, wd)
+            tot += float(per.sum())
+            cnt += batch
+        hist.append(tot / max(cnt, 1))
+        if log:
+            log(f"epoch {ep}: mean rel sq error {hist[-1]:.4e} (lr {lr_ep:.2e})")
+    return hist
+
+
+def rel_l2_per_site(pred, true):
+    """Per-site relative L2 of channel-major tensors (C, P, H, W)."""
+    d = (pred - true).permute(1, 0, 2, 3).reshape(pred.shape[1], -1)
+    r = true.permute(1, 0, 2, 3).reshape(true.shape[1], -1)
+    return torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(r, dim=1).clamp_min(1e-30)
+
+
+# ---------------------------------------------------------------------- data and stage
+PSF_PLANES, PSF_SIDE, PATCH_POOL, PATCH_BINS = 9, 33, 4, 8
+
+
+def site_features(volume, cfg, offsets, depth_um, *, pool=PATCH_POOL, zbins=PATCH_BINS):
+    """Network input (zbins + 1, P, N/pool, N/pool): pooled (n - n0)/std over each site window."""
+    d = volume.device
+    offs = np.asarray(offsets, np.int64)
+    P = len(offs)
+    wx = torch.as_tensor(offs[:, 0].astype(np.int32), device=d)
+    wy = torch.as_tensor(offs[:, 1].astype(np.int32), device=d)
+    dep = torch.full((P,), float(depth_um) / 100.0, dtype=torch.float32, device=d)
+    S = cfg.n // pool
+    out = torch.empty((zbins + 1, P, S, S), dtype=torch.float32, device=d)
+    X, Y, Zv = volume.shape
+    nat.check(nat.lib().maomi_patch_features(nat.ptr(volume), X, Y, Zv, nat.ptr(wx), nat.ptr(wy),
+                                             nat.ptr(dep), P, cfg.n, pool, zbins, float(cfg.n0),
+                                             1.0 / cfg.dn_std, nat.ptr(out), _st(d)),
+              "patch features")
+    return out
+
+
+def focal_window(cfg, planes=PSF_PLANES, side=PSF_SIDE):
+    """(z0, nz, c0, w): planes centred on the focal plane, crop centred on the site."""
+    zf = int(round(cfg.focus / cfg.dz_um))
+    return zf - planes // 2, planes, cfg.n // 2 - side // 2, side
+
+
+def site_psfs(stage, subset=None):
+    """Propagator PSFs (nz, P, w, w) of all sites of a planned PlainStage (or a subset)."""
+    z0, nz, c0, w = focal_window(stage.cfg)
+    P = len(stage.offsets) if subset is None else len(subset)
+    out = torch.empty((nz, P, w, w), dtype=torch.float32, device=stage.device)
+    stage.propagate(subset=subset, psf=(out, z0, nz, c0, w))
+    return out
+
+
+class SurrogateStage:
+    """Surrogate inference on every site, seeded audit, per-site and whole-run fallback."""
+
+    def __init__(self, net, scale, stage, *, audit_frac=0.05, tol=0.01, max_fail_frac=0.1,
+                 seed=0):
+        self.net, self.scale, self.stage = net, float(scale), stage
+        self.audit_frac, self.tol, self.max_fail_frac = audit_frac, tol, max_fail_frac
+        self.rng = np.random.default_rng(seed)
+        self.timing, self.report = {}, {}
+
+    def run(self):
+        import time
+        st, cfg = self.stage, self.stage.cfg
+        P = len(st.offsets)
+        t0 = time.perf_counter()
+        x = site_features(st.volume, cfg, st.offsets, cfg.focus)
+        pred = self.net.predict(x) * self.scale
+        torch.cuda.synchronize(st.device)
+        t1 = time.perf_counter()
+        k = max(1, int(np.ceil(self.audit_frac * P)))
+        audit = np.sort(self.rng.choice(P, k, replace=False))
+        true = site_psfs(st, audit)
+        rel = rel_l2_per_site(pred[:, audit], true)
+        fails = (rel > self.tol).cpu().numpy()
+        if fails.mean() > self.max_fail_frac:
+            mode = "fallback_all"
+            pred = site_psfs(st)
+        else:
+            mode = "surrogate"
+            idx = torch.as_tensor(audit[fails], device=pred.device)
+            if len(idx):
+                pred[:, idx] = true[:, torch.as_tensor(np.nonzero(fails)[0], device=pred.device)]
+        torch.cuda.synchronize(st.device)
+        t2 = time.perf_counter()
+        self.timing = {"inference": t1 - t0, "audit_and_fallback": t2 - t1, "total": t2 - t0}
+        rel_np = rel.cpu().numpy()
+        self.report = {"mode": mode, "audited": int(k), "audit_fail": int(fails.sum()),
+                       "audit_rel_l2_median": float(np.median(rel_np)),
+                       "audit_rel_l2_max": float(rel_np.max()),
+                       "audit_rel_l2_mean": float(rel_np.mean())}
+        self.psfs = pred
+        return pred

@@ -19,3 +19,4 @@ the reference's float32/complex64 arithmetic (the "reference CPU path"), and
 """
 
 from .optics_np import *  # noqa: F401,F403
+from . import surrogate_np  # noqa: F401

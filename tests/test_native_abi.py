@@ -44,5 +44,5 @@ def test_error_reporting_without_gpu_work():
 
 
 def test_sites_descriptor_layout_matches_header():
-    # 17 pointers + 8 int64 scalars + 3 x 5 int64 class tables, no padding
-    assert ctypes.sizeof(_native.SitesDesc) == 8 * (17 + 8 + 15)
+    # 18 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding
+    assert ctypes.sizeof(_native.SitesDesc) == 8 * (18 + 12 + 15)

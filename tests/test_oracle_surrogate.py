@@ -1,0 +1,48 @@
+"""Pin the FP64 layer oracle of the surrogate by central finite differences."""
+
+import numpy as np
+
+from oracle import surrogate_np as S
+
+
+def _num_grad(f, x, eps=1e-6):
+    g = np.zeros_like(x)
+    it = np.nditer(x, flags=["multi_index"])
+    for _ in it:
+        i = it.multi_index
+        old = x[i]
+        x[i] = old + eps
+        fp = f()
+        x[i] = old - eps
+        fm = f()
+        x[i] = old
+        g[i] = (fp - fm) / (2 * eps)
+    return g
+
+
+def test_conv_backward_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    for k, s, p, slope in ((3, 2, 1, 0.1), (3, 1, 1, 0.1), (2, 1, 1, 1.0)):
+        x = rng.standard_normal((3, 2, 6, 5))
+        w = rng.standard_normal((4, 3 * k * k)) * 0.3
+        b = rng.standard_normal(4) * 0.1
+        y, cache = S.conv_fwd(x, w, b, k, s, p, slope)
+        r = rng.standard_normal(y.shape)
+        loss = lambda: float((S.conv_fwd(x, w, b, k, s, p, slope)[0] * r).sum())
+        dx, dw, db = S.conv_bwd(r, w, cache, k, s, p, slope)
+        np.testing.assert_allclose(dw, _num_grad(loss, w), rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(db, _num_grad(loss, b), rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(dx, _num_grad(loss, x), rtol=1e-5, atol=1e-7)
+
+
+def test_upsample_and_loss_gradients():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((2, 3, 3, 4))
+    r = rng.standard_normal((2, 3, 6, 8))
+    np.testing.assert_allclose(S.up2_bwd(r), _num_grad(lambda: float((S.up2_fwd(x) * r).sum()), x),
+                               rtol=1e-6)
+    y, t = rng.standard_normal((2, 3, 4, 4)), rng.standard_normal((2, 3, 4, 4))
+    per, g = S.rel_sq_loss(y, t)
+    np.testing.assert_allclose(g, _num_grad(lambda: float(S.rel_sq_loss(y, t)[0].mean()), y),
+                               rtol=1e-5, atol=1e-8)
+    assert per.shape == (3,)

@@ -1,0 +1,158 @@
+"""Surrogate kernels on the GPU: MFMA GEMM, layer gradients vs the FP64 oracle, data, gain."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from oracle import surrogate_np as S
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_round(a):
+    return torch.as_tensor(a, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(1, 1, 1, 0, 0), (70, 33, 45, 0, 0), (128, 256, 96, 1, 0),
+                                         (37, 129, 300, 0, 1), (64, 64, 32, 1, 1)])
+def test_gemm_bf16_matches_exact_product(gpu_lib, M, N, K, ta, tb):
+    from maomi_amd.surrogate import gemm
+    rng = np.random.default_rng(M * 7 + N)
+    a = _bf16_round(rng.standard_normal((M, K)))
+    b = _bf16_round(rng.standard_normal((K, N)))
+    ref = a @ b
+    A = torch.as_tensor(a.T.copy() if ta else a, dtype=torch.bfloat16, device="cuda")
+    B = torch.as_tensor(b.T.copy() if tb else b, dtype=torch.bfloat16, device="cuda")
+    sam, sak = (1, M) if ta else (K, 1)
+    sbk, sbn = (1, K) if tb else (N, 1)
+    C = torch.full((M, N), 7.0, dtype=torch.float32, device="cuda")
+    print(f"[gemm] {M}x{N}x{K} ta={ta} tb={tb}", flush=True)
+    gemm(A, B, C, M, N, K, sam, sak, sbk, sbn, N)
+    assert rel_l2(C.cpu().numpy(), ref) < 1e-6
+    gemm(A, B, C, M, N, K, sam, sak, sbk, sbn, N, alpha=0.5, beta=1.0)
+    assert rel_l2(C.cpu().numpy(), 1.5 * ref) < 1e-6
+
+
+def test_layer_gradients_match_fp64_oracle(gpu_lib):
+    from maomi_amd.surrogate import SurrogateNet
+    spec = [("conv", 3, 5, 3, 2, 1, 0.1), ("conv", 5, 4, 3, 1, 1, 0.1), ("up",),
+            ("conv", 4, 2, 2, 1, 1, 1.0)]
+    net = SurrogateNet(spec, seed=3)
+    rng = np.random.default_rng(5)
+    x = _bf16_round(rng.standard_normal((3, 2, 8, 8)))
+    t = rng.standard_normal((2, 2, 9, 9))
+    ws = []
+    for l in net.layers:
+        if hasattr(l, "w"):
+            w = _bf16_round(rng.standard_normal(tuple(l.w.shape)) * 0.4)
+            b = rng.standard_normal(l.b.shape[0]) * 0.1
+            l.w.copy_(torch.as_tensor(w, dtype=torch.float32))
+            l.wb.copy_(l.w.to(torch.bfloat16))
+            l.b.copy_(torch.as_tensor(b, dtype=torch.float32))
+            ws.append((w, b))
+    xd = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+    td = torch.as_tensor(t, dtype=torch.float32, device="cuda")
+    y = net.forward(xd, train=True)
+    per, g = net.loss(y, td)
+    dx = net.backward(g)
+    # FP64 oracle
+    caches, h, wi = [], x, 0
+    for spec_l in spec:
+        if spec_l[0] == "up":
+            caches.append(None)
+            h = S.up2_fwd(h)
+        else:
+            _, ci, co, k, s, p, sl = spec_l
+            h, c = S.conv_fwd(h, *ws[wi], k, s, p, sl)
+            caches.append(c)
+            wi += 1
+    assert rel_l2(y.cpu().numpy(), h) < 2e-2
+    lp, gh = S.rel_sq_loss(h, t)
+    np.testing.assert_allclose(per.cpu().numpy(), lp, rtol=5e-2)
+    grads = []
+    for spec_l, c in zip(reversed(spec), reversed(caches)):
+        if spec_l[0] == "up":
+            gh = S.up2_bwd(gh)
+        else:
+            _, ci, co, k, s, p, sl = spec_l
+            wi -= 1
+            gh, dw, db = S.conv_bwd(gh, ws[wi][0], c, k, s, p, sl)
+            grads.append((dw, db))
+    grads.reverse()
+    convs = [l for l in net.layers if hasattr(l, "w")]
+    for l, (dw, db) in zip(convs, grads):
+        assert rel_l2(l.gw.cpu().numpy(), dw) < 3e-2
+        assert rel_l2(l.gb.cpu().numpy(), db) < 3e-2
+    assert rel_l2(dx.cpu().numpy(), gh) < 3e-2
+
+
+def test_patch_features_match_numpy(gpu_lib):
+    from maomi_amd.surrogate import site_features
+    from maomi_amd.workloads import PlainConfig, index_volume
+    cfg = PlainConfig(fov=32, n=64, slices=16, sites_per_axis=2)
+    vol = index_volume(cfg)
+    offs = np.array([[0, 0], [32, 16], [5, 30]])
+    out = site_features(torch.as_tensor(vol, device="cuda"), cfg, offs, 12.0).cpu().numpy()
+    for p, (x, y) in enumerate(offs):
+        win = (vol[x:x + 64, y:y + 64].astype(np.float64) - cfg.n0) / cfg.dn_std
+        pooled = win.reshape(16, 4, 16, 4, 8, 2).mean((1, 3, 5))
+        np.testing.assert_allclose(out[:8, p], np.moveaxis(pooled, -1, 0), rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(out[8, p], 0.12)
+
+
+def test_focal_planes_from_propagator_match_oracle(gpu_lib):
+    from maomi_amd.plain import PlainStage
+    from maomi_amd.surrogate import focal_window, site_psfs
+    from maomi_amd.workloads import CONFIG1, index_volume, site_offsets, zernike_weights
+    vol = index_volume(CONFIG1)
+    st = PlainStage(CONFIG1, vol).plan()
+    sub = [0, 4, 8]
+    psf = site_psfs(st, sub).cpu().numpy()
+    z0, nz, c0, w = focal_window(CONFIG1)
+    ref = O.plain_stage(CONFIG1, vol, site_offsets(CONFIG1), zernike_weights(CONFIG1),
+                        sites=np.array(sub), with_masks=False)
+    base, _ = O.plain_beam(CONFIG1, zernike_weights(CONFIG1))
+    k0dz = 2 * np.pi / CONFIG1.wavelength_um * CONFIG1.dz_um
+    phase = O.plain_phase(vol, CONFIG1.n0, k0dz)
+    _, planes = O.plain_stage_masks(base, phase, site_offsets(CONFIG1)[sub],
+                                    wvl=np.float32(0.92e-6), nr=np.float32(1.35),
+                                    dx=np.float32(0.5e-6), dz=2e-6, return_planes=True)
+    for i in range(len(sub)):
+        want = np.abs(planes[i][c0:c0 + w, c0:c0 + w, z0:z0 + nz]) ** 4
+        assert rel_l2(psf[:, i], np.moveaxis(want, -1, 0)) < 1e-5
+    assert ref["masks"].shape[0] == 3
+
+
+def test_trained_surrogate_beats_mean_psf_predictor(gpu_lib):
+    """The learned part's gain: validation error of the trained network against the
+    method without it (predicting the mean training PSF at every site)."""
+    from maomi_amd.plain import PlainStage
+    from maomi_amd.surrogate import (SurrogateNet, rel_l2_per_site, site_features, site_psfs,
+                                     train)
+    from maomi_amd.workloads import PlainConfig, index_volume, site_offsets
+    xs, ts = [], []
+    for seed in (11, 12, 13):
+        cfg = PlainConfig(fov=96, n=256, slices=12, sites_per_axis=13, corr_vox=3.0, seed=seed,
+                          zernike_std_rad=0.05)
+        vol = torch.as_tensor(index_volume(cfg), device="cuda")
+        st = PlainStage(cfg, vol).plan()
+        xs.append(site_features(vol, cfg, st.offsets, cfg.focus))
+        ts.append(site_psfs(st))
+    x, t = torch.cat(xs, 1), torch.cat(ts, 1)
+    scale = float(t.mean() * 4)
+    t = t / scale
+    P = x.shape[1]
+    ntr = P - 64
+    print(f"[surrogate] {P} sites, training on {ntr}", flush=True)
+    net = SurrogateNet(seed=0)
+    hist = train(net, x[:, :ntr].contiguous(), t[:, :ntr].contiguous(), epochs=12, batch=64,
+                 lr=2e-3)
+    assert hist[-1] < hist[0]
+    pred = net.predict(x[:, ntr:].contiguous())
+    err_net = rel_l2_per_site(pred, t[:, ntr:]).mean().item()
+    mean_psf = t[:, :ntr].mean(1, keepdim=True).expand(-1, P - ntr, -1, -1)
+    err_mean = rel_l2_per_site(mean_psf, t[:, ntr:]).mean().item()
+    print(f"validation rel L2: trained {err_net:.4f}, mean-PSF predictor {err_mean:.4f}", flush=True)
+    assert err_net < err_mean
