@@ -241,6 +241,12 @@ int maomi_bias_act(float* Y, const float* bias, int64_t C, int64_t n, float slop
 /* dY <- dY act'(Y) in place; db[c] = sum_n dY[c][n] */
 int maomi_act_bwd(float* dY, const float* Y, int64_t C, int64_t n, float slope, float* db,
                   maomi_stream_t stream);
+/* learned output planes: out[c][b][i] = in[c][b][i] + T[c][i] (in may equal out) */
+int maomi_plane_bias(const float* in, const float* T, float* out, int64_t C, int64_t B, int64_t HW,
+                     maomi_stream_t stream);
+/* gT[c][i] = scale sum_b dY[c][b][i] (the planes' gradient; scale 1/B gives the batch mean) */
+int maomi_plane_bias_grad(const float* dY, float* gT, int64_t C, int64_t B, int64_t HW,
+                          float scale, maomi_stream_t stream);
 int maomi_to_bf16(const float* x, void* y, int64_t n, maomi_stream_t stream);
 /* nearest 2x upsampling of CB planes of H x W (backward = 2 x 2 sums) */
 int maomi_upsample2(const float* in, float* out, int64_t CB, int64_t H, int64_t W,
@@ -248,6 +254,10 @@ int maomi_upsample2(const float* in, float* out, int64_t CB, int64_t H, int64_t 
 /* per sample b: loss[b] = |y - t|^2 / |t|^2; grad = 2 (y - t) / |t|^2 / B (grad may be NULL) */
 int maomi_rel_sq_loss(const float* y, const float* t, float* grad, float* loss, int64_t C,
                       int64_t B, int64_t HW, maomi_stream_t stream);
+/* per sample b: loss[b] = |y - t|^2 / E with E = mean_b |t_b|^2 (the batch's mean target
+ * energy); grad = 2 (y - t) / (E B) (may be NULL); work holds 2 B doubles */
+int maomi_energy_sq_loss(const float* y, const float* t, float* grad, float* loss, double* work,
+                         int64_t C, int64_t B, int64_t HW, maomi_stream_t stream);
 /* Adam step (bias-corrected, decoupled weight decay wd) on FP32 w; wb (BF16 copy) may be NULL */
 int maomi_adam(float* w, void* wb, const float* g, float* m, float* v, int64_t n, float lr,
                float b1, float b2, float eps, int64_t step, float wd, maomi_stream_t stream);

@@ -85,10 +85,16 @@ _SIGS = {
     "maomi_col2im": (c_int, [c_void_p, c_void_p] + [c_int64] * 7 + [c_void_p]),
     "maomi_bias_act": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p]),
     "maomi_act_bwd": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p, c_void_p]),
+    "maomi_plane_bias": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64,
+                                 c_void_p]),
+    "maomi_plane_bias_grad": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_float,
+                                      c_void_p]),
     "maomi_to_bf16": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "maomi_upsample2": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p]),
     "maomi_rel_sq_loss": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                   c_int64, c_void_p]),
+    "maomi_energy_sq_loss": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                     c_int64, c_int64, c_void_p]),
     "maomi_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float,
                            c_float, c_float, c_float, c_int64, c_float, c_void_p]),
     "maomi_patch_features": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,

@@ -156,6 +156,29 @@ __global__ void act_bwd_kernel(float* __restrict__ dY, const float* __restrict__
   if (threadIdx.x == 0) db[c] = (float)t;
 }
 
+// out[c][b][i] = in[c][b][i] + T[c][i]
+__global__ void plane_bias_kernel(const float* __restrict__ in, const float* __restrict__ T,
+                                  float* __restrict__ out, int64_t B, int64_t HW, int64_t total) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / (B * HW), i = e % HW;
+    out[e] = in[e] + T[c * HW + i];
+  }
+}
+
+// gT[c][i] = scale * sum_b dY[c][b][i]; consecutive threads take consecutive i (coalesced)
+__global__ void plane_bias_grad_kernel(const float* __restrict__ dY, float* __restrict__ gT,
+                                       int64_t B, int64_t HW, int64_t total, float scale) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / HW, i = e % HW;
+    const float* src = dY + c * B * HW + i;
+    double acc = 0.0;
+    for (int64_t b = 0; b < B; ++b) acc += src[b * HW];
+    gT[e] = (float)(acc * scale);
+  }
+}
+
 __global__ void to_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -212,18 +235,59 @@ __global__ void rel_sq_loss_kernel(const float* __restrict__ y, const float* __r
   if (threadIdx.x == 0) loss[b] = (float)(den > 0.0 ? num / den : 0.0);
 }
 
+// Energy-normalised squared error, pass 1: per sample b, work[b] = |y_b - t_b|^2 and
+// work[Bn + b] = |t_b|^2.
+__global__ void sq_terms_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                                double* __restrict__ work, int C, int Bn, int HW) {
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  double num = 0.0, den = 0.0;
+  for (int i = threadIdx.x; i < C * HW; i += blockDim.x) {
+    const int c = i / HW, q = i % HW;
+    const int64_t e = ((int64_t)c * Bn + b) * HW + q;
+    const double d = (double)y[e] - t[e];
+    num += d * d;
+    den += (double)t[e] * t[e];
+  }
+  num = block_sum_double(num, red);
+  den = block_sum_double(den, red);
+  if (threadIdx.x == 0) {
+    work[b] = num;
+    work[Bn + b] = den;
+  }
+}
+
+// Pass 2: E = mean_b |t_b|^2 (each block sums the Bn terms itself); loss[b] = work[b] / E and
+// grad = 2 (y - t) / (E Bn), the gradient of the mean loss.
+__global__ void energy_loss_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                                   const double* __restrict__ work, float* __restrict__ grad,
+                                   float* __restrict__ loss, int Bn, int64_t total) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < Bn; b += blockDim.x) s += work[Bn + b];
+  const double E = block_sum_double(s, red) / Bn;
+  const float inv = E > 0.0 ? (float)(1.0 / E) : 0.f;
+  if (blockIdx.x == 0)
+    for (int b = threadIdx.x; b < Bn; b += blockDim.x) loss[b] = (float)(work[b] * inv);
+  if (!grad) return;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    grad[e] = 2.f * (y[e] - t[e]) * inv / (float)Bn;
+}
+
 // Adam on FP32 master weights; refreshes the BF16 copy.
 __global__ void adam_kernel(float* __restrict__ w, __bf16* __restrict__ wb, const float* __restrict__ g,
                             float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
                             float b1, float b2, float eps, float c1, float c2, float wd) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const float gr = g[e] + wd * w[e];
+    const float gr = g[e];
     const float mm = b1 * m[e] + (1.f - b1) * gr;
     const float vv = b2 * v[e] + (1.f - b2) * gr * gr;
     m[e] = mm;
     v[e] = vv;
-    const float upd = lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    // decoupled decay (AdamW): the decay term does not pass through the moment estimates
+    const float upd = lr * ((mm / c1) / (sqrtf(vv / c2) + eps) + wd * w[e]);
     w[e] -= upd;
     if (wb) wb[e] = to_bf16(w[e]);
   }
@@ -299,6 +363,24 @@ MAOMI_API int maomi_act_bwd(float* dY, const float* Y, int64_t C, int64_t n, flo
   return MAOMI_OK;
 }
 
+MAOMI_API int maomi_plane_bias(const float* in, const float* T, float* out, int64_t C, int64_t B,
+                               int64_t HW, maomi_stream_t stream) {
+  if (!in || !T || !out || C < 1 || B < 1 || HW < 1) { maomi_set_error("maomi_plane_bias: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(plane_bias_kernel, dim3(blocks_for(C * B * HW)), dim3(256), 0,
+                     (hipStream_t)stream, in, T, out, B, HW, C * B * HW);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_plane_bias_grad(const float* dY, float* gT, int64_t C, int64_t B, int64_t HW,
+                                    float scale, maomi_stream_t stream) {
+  if (!dY || !gT || C < 1 || B < 1 || HW < 1) { maomi_set_error("maomi_plane_bias_grad: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(plane_bias_grad_kernel, dim3(blocks_for(C * HW)), dim3(256), 0,
+                     (hipStream_t)stream, dY, gT, B, HW, C * HW, scale);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
 MAOMI_API int maomi_to_bf16(const float* x, void* y, int64_t n, maomi_stream_t stream) {
   if (!x || !y || n < 1) { maomi_set_error("maomi_to_bf16: bad arguments"); return MAOMI_EINVAL; }
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, x,
@@ -325,6 +407,20 @@ MAOMI_API int maomi_rel_sq_loss(const float* y, const float* t, float* grad, flo
   if (!y || !t || !loss || C < 1 || Bn < 1 || HW < 1) { maomi_set_error("maomi_rel_sq_loss: bad arguments"); return MAOMI_EINVAL; }
   hipLaunchKernelGGL(rel_sq_loss_kernel, dim3((unsigned)Bn), dim3(256), 0, (hipStream_t)stream, y,
                      t, grad, loss, (int)C, (int)Bn, (int)HW);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_energy_sq_loss(const float* y, const float* t, float* grad, float* loss,
+                                   double* work, int64_t C, int64_t Bn, int64_t HW,
+                                   maomi_stream_t stream) {
+  if (!y || !t || !loss || !work || C < 1 || Bn < 1 || HW < 1) { maomi_set_error("maomi_energy_sq_loss: bad arguments"); return MAOMI_EINVAL; }
+  hipLaunchKernelGGL(sq_terms_kernel, dim3((unsigned)Bn), dim3(256), 0, (hipStream_t)stream, y, t,
+                     work, (int)C, (int)Bn, (int)HW);
+  MAOMI_CHECK_LAUNCH();
+  const int64_t total = C * Bn * HW;
+  hipLaunchKernelGGL(energy_loss_kernel, dim3(grad ? blocks_for(total) : 1u), dim3(256), 0,
+                     (hipStream_t)stream, y, t, work, grad, loss, (int)Bn, total);
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

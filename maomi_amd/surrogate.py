@@ -31,7 +31,8 @@ def gemm(A, B, C, M, N, K, sam, sak, sbk, sbn, ldc, alpha=1.0, beta=0.0):
 
 def to_bf16(x, out=None):
     out = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device) if out is None else out
-    nat.check(nat.lib().maomi_to_bf16(nat.ptr(x.contiguous()), nat.ptr(out), x.numel(),
+    x = x.contiguous()  # kept alive until the launch is enqueued
+    nat.check(nat.lib().maomi_to_bf16(nat.ptr(x), nat.ptr(out), x.numel(),
                                       _st(x.device)), "to bf16")
     return out
 
@@ -102,6 +103,48 @@ class Conv:
                                  self.b.numel(), lr, b1, b2, eps, t, 0.0, st), "adam b")
 
 
+class Template:
+    """Learned output planes T (C, H, W) added to every sample's output; the network then
+    predicts the site-specific part on top of T.  Set to the training-mean PSF before training."""
+
+    def __init__(self, C, H, W, device):
+        self.C, self.H, self.W = C, H, W
+        self.w = torch.zeros((C, H * W), dtype=torch.float32, device=device)
+        self.gw = torch.zeros_like(self.w)
+        self.state = [torch.zeros_like(self.w), torch.zeros_like(self.w)]
+
+    def params(self):
+        return self.w.numel()
+
+    def set_mean(self, t):
+        """T = mean over samples of channel-major targets t (C, P, H, W)."""
+        C, P, H, W = t.shape
+        t = t.contiguous()
+        nat.check(nat.lib().maomi_plane_bias_grad(nat.ptr(t), nat.ptr(self.w), C, P,
+                                                  H * W, 1.0 / P, _st(t.device)), "template mean")
+
+    def forward(self, x, train):
+        C, B, H, W = x.shape
+        if (C, H, W) != (self.C, self.H, self.W):
+            raise ValueError(f"template planes {(self.C, self.H, self.W)} vs output {(C, H, W)}")
+        y = torch.empty_like(x)
+        nat.check(nat.lib().maomi_plane_bias(nat.ptr(x), nat.ptr(self.w), nat.ptr(y), C, B, H * W,
+                                             _st(x.device)), "template")
+        return y
+
+    def backward(self, dy):
+        C, B, H, W = dy.shape
+        nat.check(nat.lib().maomi_plane_bias_grad(nat.ptr(dy), nat.ptr(self.gw), C, B, H * W, 1.0,
+                                                  _st(dy.device)), "template grad")
+        return dy
+
+    def step(self, lr, t, wd=0.0, b1=0.9, b2=0.999, eps=1e-8):
+        m, v = self.state
+        nat.check(nat.lib().maomi_adam(nat.ptr(self.w), None, nat.ptr(self.gw), nat.ptr(m),
+                                       nat.ptr(v), self.w.numel(), lr, b1, b2, eps, t, 0.0,
+                                       _st(self.w.device)), "adam template")
+
+
 class Up2:
     def forward(self, x, train):
         C, B, H, W = x.shape
@@ -125,13 +168,25 @@ class Up2:
         pass
 
 
-def default_layers(cin=9, widths=(32, 64, 128, 384, 384, 96, 48), out_ch=9, slope=0.1):
-    """Encoder 64 -> 32 -> 16 -> 8 (two 8x8 convs), decoder 8 -> 16 -> 32, 2x2 conv to 33x33."""
+def default_layers(cin=11, widths=(32, 64, 128, 384, 384, 96, 48), out_ch=9, side=33,
+                   slope=0.1):
+    """Encoder 64 -> 32 -> 16 -> 8 (two 8x8 convs), decoder 8 -> 16 -> 32, 2x2 conv to 33x33,
+    plus learned output planes.  Input: 8 index bins, depth, and two coordinate channels (the
+    convolutions alone cannot place the focal spot at the window centre)."""
     w = widths
     return [("conv", cin, w[0], 3, 2, 1, slope), ("conv", w[0], w[1], 3, 2, 1, slope),
             ("conv", w[1], w[2], 3, 2, 1, slope), ("conv", w[2], w[3], 3, 1, 1, slope),
             ("conv", w[3], w[4], 3, 1, 1, slope), ("up",), ("conv", w[4], w[5], 3, 1, 1, slope),
-            ("up",), ("conv", w[5], w[6], 3, 1, 1, slope), ("conv", w[6], out_ch, 2, 1, 1, 1.0)]
+            ("up",), ("conv", w[5], w[6], 3, 1, 1, slope), ("conv", w[6], out_ch, 2, 1, 1, 1.0),
+            ("template", out_ch, side, side)]
+
+
+def _make_layer(l, gen, device):
+    if l[0] == "up":
+        return Up2()
+    if l[0] == "template":
+        return Template(*l[1:], device=device)
+    return Conv(*l[1:], gen=gen, device=device)
 
 
 class SurrogateNet:
@@ -139,9 +194,17 @@ class SurrogateNet:
         self.device = torch.device(device)
         gen = torch.Generator().manual_seed(seed)
         self.spec = layers or default_layers()
-        self.layers = [Up2() if l[0] == "up" else Conv(*l[1:], gen=gen, device=self.device)
-                       for l in self.spec]
+        self.layers = [_make_layer(l, gen, self.device) for l in self.spec]
+        if isinstance(self.layers[-1], Template):
+            # the convolutional part starts near zero: the net starts at the template
+            last = self.layers[-2]
+            last.w.mul_(0.1)
+            last.wb = to_bf16(last.w)
         self.t = 0
+
+    @property
+    def template(self):
+        return self.layers[-1] if isinstance(self.layers[-1], Template) else None
 
     def num_params(self):
         return sum(l.params() for l in self.layers)
@@ -156,12 +219,20 @@ class SurrogateNet:
             dy = l.backward(dy)
         return dy
 
-    def loss(self, y, t, grad=True):
+    def loss(self, y, t, grad=True, kind="energy"):
+        """Per-sample loss and its gradient.  "energy": |y - t|^2 over the batch's mean target
+        energy (training: low-energy sites do not dominate); "relative": |y - t|^2 / |t|^2."""
         C, B, H, W = y.shape
         g = torch.empty_like(y) if grad else None
         per = torch.empty(B, dtype=torch.float32, device=y.device)
-        nat.check(nat.lib().maomi_rel_sq_loss(nat.ptr(y), nat.ptr(t), nat.ptr(g), nat.ptr(per), C,
-                                              B, H * W, _st(y.device)), "loss")
+        if kind == "energy":
+            work = torch.empty(2 * B, dtype=torch.float64, device=y.device)
+            nat.check(nat.lib().maomi_energy_sq_loss(nat.ptr(y), nat.ptr(t), nat.ptr(g),
+                                                     nat.ptr(per), nat.ptr(work), C, B, H * W,
+                                                     _st(y.device)), "energy loss")
+        else:
+            nat.check(nat.lib().maomi_rel_sq_loss(nat.ptr(y), nat.ptr(t), nat.ptr(g), nat.ptr(per),
+                                                  C, B, H * W, _st(y.device)), "loss")
         return per, g
 
     def train_step(self, x, t, lr, wd=0.0):
@@ -172,6 +243,17 @@ class SurrogateNet:
         for l in self.layers:
             l.step(lr, self.t, wd)
         return per
+
+    def forward_flops(self, H, W):
+        """Multiply-add flops (2 per MAC) of one sample's forward pass on an H x W input."""
+        f = 0
+        for l in self.layers:
+            if isinstance(l, Conv):
+                H, W = l.out_hw(H, W)
+                f += 2 * l.cout * l.cin * l.k * l.k * H * W
+            elif isinstance(l, Up2):
+                H, W = 2 * H, 2 * W
+        return f
 
     def predict(self, x, batch=512):
         outs = []
@@ -185,10 +267,18 @@ def to_cbhw(a):
     return a.permute(1, 0, 2, 3).contiguous()
 
 
-def train(net, x, t, *, epochs=10, batch=128, lr=1e-3, wd=0.0, seed=0, log=None):
-    """x, t: channel-major tensors (C, P, H, W) on the device."""
+def train(net, x, t, *, epochs=10, batch=64, lr=1e-3, wd=0.0, seed=0, warmup=0.05, log=None):
+    """Adam with linear warm-up over the first `warmup` fraction of steps, then cosine decay
+    per epoch.  x, t: channel-major tensors (C, P, H, W) on the device."""
     P = x.shape[1]
+    if P < batch:
+        raise ValueError(f"{P} training samples for batch {batch}")
     rng = np.random.default_rng(seed)
+    if net.template is not None and net.t == 0:
+        net.template.set_mean(t)
+    steps = epochs * (P // batch)
+    warm = max(1, int(warmup * steps))
+    done = 0
     hist = []
     for ep in range(epochs):
         order = torch.as_tensor(rng.permutation(P), device=x.device)
@@ -196,7 +286,9 @@ def train(net, x, t, *, epochs=10, batch=128, lr=1e-3, wd=0.0, seed=0, log=None)
         lr_ep = lr * 0.5 * (1 + np.cos(np.pi * ep / max(1, epochs)))
         for s in range(0, P - batch + 1, batch):
             idx = order[s:s + batch]
-            per = net.train_step(x[:, idx].contiguous(), t[:, idx].contiguous(), lr_ep, wd)
+            done += 1
+            per = net.train_step(x[:, idx].contiguous(), t[:, idx].contiguous(),
+                                 lr_ep * min(1.0, done / warm), wd)
             tot += float(per.sum())
             cnt += batch
         hist.append(tot / max(cnt, 1))
@@ -206,10 +298,15 @@ def train(net, x, t, *, epochs=10, batch=128, lr=1e-3, wd=0.0, seed=0, log=None)
 
 
 def rel_l2_per_site(pred, true):
-    """Per-site relative L2 of channel-major tensors (C, P, H, W)."""
-    d = (pred - true).permute(1, 0, 2, 3).reshape(pred.shape[1], -1)
-    r = true.permute(1, 0, 2, 3).reshape(true.shape[1], -1)
-    return torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(r, dim=1).clamp_min(1e-30)
+    """Per-site relative L2 |pred - true| / |true| of channel-major tensors (C, P, H, W)."""
+    C, P, H, W = true.shape
+    per = torch.empty(P, dtype=torch.float32, device=true.device)
+    # both operands bound to names: a temporary freed before the launch can hand its block to
+    # the next temporary
+    pred, true = pred.contiguous(), true.contiguous()
+    nat.check(nat.lib().maomi_rel_sq_loss(nat.ptr(pred), nat.ptr(true), None, nat.ptr(per), C, P,
+                                          H * W, _st(true.device)), "rel l2")
+    return per.sqrt_()
 
 
 # ---------------------------------------------------------------------- data and stage
@@ -217,7 +314,8 @@ PSF_PLANES, PSF_SIDE, PATCH_POOL, PATCH_BINS = 9, 33, 4, 8
 
 
 def site_features(volume, cfg, offsets, depth_um, *, pool=PATCH_POOL, zbins=PATCH_BINS):
-    """Network input (zbins + 1, P, N/pool, N/pool): pooled (n - n0)/std over each site window."""
+    """Network input (zbins + 3, P, N/pool, N/pool): pooled (n - n0)/std over each site window,
+    the depth scalar, then x and y coordinates in [-1, 1]."""
     d = volume.device
     offs = np.asarray(offsets, np.int64)
     P = len(offs)
@@ -225,7 +323,10 @@ def site_features(volume, cfg, offsets, depth_um, *, pool=PATCH_POOL, zbins=PATC
     wy = torch.as_tensor(offs[:, 1].astype(np.int32), device=d)
     dep = torch.full((P,), float(depth_um) / 100.0, dtype=torch.float32, device=d)
     S = cfg.n // pool
-    out = torch.empty((zbins + 1, P, S, S), dtype=torch.float32, device=d)
+    out = torch.empty((zbins + 3, P, S, S), dtype=torch.float32, device=d)
+    ramp = torch.linspace(-1.0, 1.0, S, dtype=torch.float32, device=d)
+    out[zbins + 1].copy_(ramp[None, :, None].expand(P, S, S))
+    out[zbins + 2].copy_(ramp[None, None, :].expand(P, S, S))
     X, Y, Zv = volume.shape
     nat.check(nat.lib().maomi_patch_features(nat.ptr(volume), X, Y, Zv, nat.ptr(wx), nat.ptr(wy),
                                              nat.ptr(dep), P, cfg.n, pool, zbins, float(cfg.n0),

@@ -54,3 +54,29 @@ def rel_sq_loss(y, t):
     num = (d ** 2).sum((0, 2, 3))
     den = (t ** 2).sum((0, 2, 3))
     return num / den, 2 * d / den[None, :, None, None] / B
+
+
+def energy_sq_loss(y, t):
+    """Per-sample |y - t|^2 / E, E = mean over samples of |t|^2, and the gradient of the mean."""
+    B = y.shape[1]
+    d = y - t
+    E = (t ** 2).sum((0, 2, 3)).mean()
+    return (d ** 2).sum((0, 2, 3)) / E, 2 * d / E / B
+
+
+def plane_bias_fwd(x, T):
+    """Learned output planes: y[c, b] = x[c, b] + T[c] (channel-major (C, B, H, W))."""
+    return x + T[:, None]
+
+
+def plane_bias_bwd(g):
+    """Gradient of the planes: sum over samples."""
+    return g.sum(1)
+
+
+def adam_step(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    """One AdamW step (bias-corrected moments, decoupled decay); returns new (w, m, v)."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh, vh = m / (1 - b1 ** t), v / (1 - b2 ** t)
+    return w - lr * (mh / (np.sqrt(vh) + eps) + wd * w), m, v

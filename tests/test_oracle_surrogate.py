@@ -46,3 +46,33 @@ def test_upsample_and_loss_gradients():
     np.testing.assert_allclose(g, _num_grad(lambda: float(S.rel_sq_loss(y, t)[0].mean()), y),
                                rtol=1e-5, atol=1e-8)
     assert per.shape == (3,)
+
+
+def test_plane_bias_gradient():
+    rng = np.random.default_rng(2)
+    x, T = rng.standard_normal((2, 3, 4, 5)), rng.standard_normal((2, 4, 5))
+    r = rng.standard_normal((2, 3, 4, 5))
+    loss = lambda: float((S.plane_bias_fwd(x, T) * r).sum())
+    np.testing.assert_allclose(S.plane_bias_bwd(r), _num_grad(loss, T), rtol=1e-6)
+    np.testing.assert_allclose(S.plane_bias_fwd(x, T)[:, 1], x[:, 1] + T)
+
+
+def test_energy_loss_gradient():
+    rng = np.random.default_rng(3)
+    y, t = rng.standard_normal((2, 4, 3, 3)), rng.standard_normal((2, 4, 3, 3))
+    per, g = S.energy_sq_loss(y, t)
+    np.testing.assert_allclose(g, _num_grad(lambda: float(S.energy_sq_loss(y, t)[0].mean()), y),
+                               rtol=1e-5, atol=1e-8)
+    # equal target energies: identical to the per-sample relative loss
+    t1 = t / np.sqrt((t ** 2).sum((0, 2, 3)))[None, :, None, None]
+    np.testing.assert_allclose(S.energy_sq_loss(y, t1)[0], S.rel_sq_loss(y, t1)[0])
+
+
+def test_adam_first_step_closed_form():
+    # the bias-corrected first step moves every weight by lr g / (|g| + eps) plus the decay
+    rng = np.random.default_rng(4)
+    w, g = rng.standard_normal(50), rng.standard_normal(50)
+    w1, m, v = S.adam_step(w, g, np.zeros(50), np.zeros(50), 1, 0.01, eps=1e-8, wd=0.1)
+    np.testing.assert_allclose(w1, w - 0.01 * (g / (np.abs(g) + 1e-8) + 0.1 * w), rtol=1e-12)
+    np.testing.assert_allclose(m, 0.1 * g)
+    np.testing.assert_allclose(v, 0.001 * g * g)

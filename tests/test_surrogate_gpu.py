@@ -36,26 +36,32 @@ def test_gemm_bf16_matches_exact_product(gpu_lib, M, N, K, ta, tb):
 
 
 def test_layer_gradients_match_fp64_oracle(gpu_lib):
-    from maomi_amd.surrogate import SurrogateNet
+    from maomi_amd.surrogate import Conv, SurrogateNet
     spec = [("conv", 3, 5, 3, 2, 1, 0.1), ("conv", 5, 4, 3, 1, 1, 0.1), ("up",),
-            ("conv", 4, 2, 2, 1, 1, 1.0)]
+            ("conv", 4, 2, 2, 1, 1, 1.0), ("template", 2, 9, 9)]
     net = SurrogateNet(spec, seed=3)
     rng = np.random.default_rng(5)
     x = _bf16_round(rng.standard_normal((3, 2, 8, 8)))
     t = rng.standard_normal((2, 2, 9, 9))
     ws = []
     for l in net.layers:
-        if hasattr(l, "w"):
+        if isinstance(l, Conv):
             w = _bf16_round(rng.standard_normal(tuple(l.w.shape)) * 0.4)
             b = rng.standard_normal(l.b.shape[0]) * 0.1
             l.w.copy_(torch.as_tensor(w, dtype=torch.float32))
             l.wb.copy_(l.w.to(torch.bfloat16))
             l.b.copy_(torch.as_tensor(b, dtype=torch.float32))
             ws.append((w, b))
+    T = rng.standard_normal((2, 9, 9)) * 0.5
+    net.template.w.copy_(torch.as_tensor(T.reshape(2, 81), dtype=torch.float32))
     xd = torch.as_tensor(x, dtype=torch.float32, device="cuda")
     td = torch.as_tensor(t, dtype=torch.float32, device="cuda")
     y = net.forward(xd, train=True)
-    per, g = net.loss(y, td)
+    per_e, g_e = net.loss(y, td)
+    lp_e, gh_e = S.energy_sq_loss(y.cpu().numpy().astype(np.float64), t)
+    np.testing.assert_allclose(per_e.cpu().numpy(), lp_e, rtol=1e-5)
+    assert rel_l2(g_e.cpu().numpy(), gh_e) < 1e-5
+    per, g = net.loss(y, td, kind="relative")
     dx = net.backward(g)
     # FP64 oracle
     caches, h, wi = [], x, 0
@@ -63,6 +69,9 @@ def test_layer_gradients_match_fp64_oracle(gpu_lib):
         if spec_l[0] == "up":
             caches.append(None)
             h = S.up2_fwd(h)
+        elif spec_l[0] == "template":
+            caches.append(None)
+            h = S.plane_bias_fwd(h, T)
         else:
             _, ci, co, k, s, p, sl = spec_l
             h, c = S.conv_fwd(h, *ws[wi], k, s, p, sl)
@@ -75,17 +84,39 @@ def test_layer_gradients_match_fp64_oracle(gpu_lib):
     for spec_l, c in zip(reversed(spec), reversed(caches)):
         if spec_l[0] == "up":
             gh = S.up2_bwd(gh)
+        elif spec_l[0] == "template":
+            gT = S.plane_bias_bwd(gh)
         else:
             _, ci, co, k, s, p, sl = spec_l
             wi -= 1
             gh, dw, db = S.conv_bwd(gh, ws[wi][0], c, k, s, p, sl)
             grads.append((dw, db))
     grads.reverse()
-    convs = [l for l in net.layers if hasattr(l, "w")]
+    convs = [l for l in net.layers if isinstance(l, Conv)]
     for l, (dw, db) in zip(convs, grads):
         assert rel_l2(l.gw.cpu().numpy(), dw) < 3e-2
         assert rel_l2(l.gb.cpu().numpy(), db) < 3e-2
     assert rel_l2(dx.cpu().numpy(), gh) < 3e-2
+    assert rel_l2(net.template.gw.cpu().numpy().reshape(2, 9, 9), gT) < 3e-2
+
+
+def test_adam_steps_match_oracle(gpu_lib):
+    from maomi_amd import _native as nat
+    rng = np.random.default_rng(9)
+    n = 1000
+    w, m, v = rng.standard_normal(n), np.zeros(n), np.zeros(n)
+    W = torch.as_tensor(w, dtype=torch.float32, device="cuda")
+    Wb = torch.empty(n, dtype=torch.bfloat16, device="cuda")
+    M, V = torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda")
+    for t in (1, 2, 3):
+        g = rng.standard_normal(n)
+        G = torch.as_tensor(g, dtype=torch.float32, device="cuda")
+        nat.check(nat.lib().maomi_adam(nat.ptr(W), nat.ptr(Wb), nat.ptr(G), nat.ptr(M), nat.ptr(V),
+                                       n, 1e-2, 0.9, 0.999, 1e-8, t, 0.05,
+                                       nat.stream_handle(W.device)), "adam")
+        w, m, v = S.adam_step(w, g, m, v, t, 1e-2, wd=0.05)
+    np.testing.assert_allclose(W.cpu().numpy(), w, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(Wb.float().cpu().numpy(), w, rtol=1e-2, atol=1e-2)
 
 
 def test_patch_features_match_numpy(gpu_lib):
@@ -100,6 +131,9 @@ def test_patch_features_match_numpy(gpu_lib):
         pooled = win.reshape(16, 4, 16, 4, 8, 2).mean((1, 3, 5))
         np.testing.assert_allclose(out[:8, p], np.moveaxis(pooled, -1, 0), rtol=1e-5, atol=1e-5)
         np.testing.assert_allclose(out[8, p], 0.12)
+        ramp = np.linspace(-1, 1, 16)
+        np.testing.assert_allclose(out[9, p], np.broadcast_to(ramp[:, None], (16, 16)), atol=1e-6)
+        np.testing.assert_allclose(out[10, p], np.broadcast_to(ramp[None, :], (16, 16)), atol=1e-6)
 
 
 def test_focal_planes_from_propagator_match_oracle(gpu_lib):
@@ -126,33 +160,39 @@ def test_focal_planes_from_propagator_match_oracle(gpu_lib):
 
 
 def test_trained_surrogate_beats_mean_psf_predictor(gpu_lib):
-    """The learned part's gain: validation error of the trained network against the
-    method without it (predicting the mean training PSF at every site)."""
+    """What the learned part gains: validation error of the trained network against the
+    method without it, one PSF for every site (the mean training PSF), on sites of an
+    independent volume.  Weakly scattering volumes (index std 0.005, correlation 6 voxels),
+    where the PSF varies mostly through low-order aberrations that the index patch carries;
+    under the full-shape statistics (std 0.02, correlation 3) the focal PSF is speckle and
+    neither predictor comes near the propagator."""
     from maomi_amd.plain import PlainStage
-    from maomi_amd.surrogate import (SurrogateNet, rel_l2_per_site, site_features, site_psfs,
-                                     train)
-    from maomi_amd.workloads import PlainConfig, index_volume, site_offsets
-    xs, ts = [], []
-    for seed in (11, 12, 13):
-        cfg = PlainConfig(fov=96, n=256, slices=12, sites_per_axis=13, corr_vox=3.0, seed=seed,
-                          zernike_std_rad=0.05)
+    from maomi_amd.surrogate import SurrogateNet, rel_l2_per_site, site_features, site_psfs, train
+    from maomi_amd.workloads import PlainConfig, index_volume
+
+    def volume_data(seed, subset=None):
+        cfg = PlainConfig(fov=512, n=256, slices=60, sites_per_axis=65, corr_vox=6.0,
+                          dn_std=0.005, seed=seed, zernike_std_rad=0.05)
         vol = torch.as_tensor(index_volume(cfg), device="cuda")
         st = PlainStage(cfg, vol).plan()
-        xs.append(site_features(vol, cfg, st.offsets, cfg.focus))
-        ts.append(site_psfs(st))
-    x, t = torch.cat(xs, 1), torch.cat(ts, 1)
+        offs = st.offsets if subset is None else st.offsets[subset]
+        return site_features(vol, cfg, offs, cfg.focus), site_psfs(st, subset)
+
+    parts = [volume_data(s) for s in (21, 22, 23)]
+    x, t = torch.cat([p[0] for p in parts], 1), torch.cat([p[1] for p in parts], 1)
+    del parts
     scale = float(t.mean() * 4)
-    t = t / scale
-    P = x.shape[1]
-    ntr = P - 64
-    print(f"[surrogate] {P} sites, training on {ntr}", flush=True)
+    t /= scale
+    val = np.sort(np.random.default_rng(0).choice(4225, 512, replace=False))
+    xv, tv = volume_data(24, val)
+    tv /= scale
+    print(f"[surrogate] training on {x.shape[1]} sites", flush=True)
     net = SurrogateNet(seed=0)
-    hist = train(net, x[:, :ntr].contiguous(), t[:, :ntr].contiguous(), epochs=12, batch=64,
-                 lr=2e-3)
-    assert hist[-1] < hist[0]
-    pred = net.predict(x[:, ntr:].contiguous())
-    err_net = rel_l2_per_site(pred, t[:, ntr:]).mean().item()
-    mean_psf = t[:, :ntr].mean(1, keepdim=True).expand(-1, P - ntr, -1, -1)
-    err_mean = rel_l2_per_site(mean_psf, t[:, ntr:]).mean().item()
+    hist = train(net, x, t, epochs=8, lr=1e-3, wd=0.05)
+    print("train loss per epoch", " ".join(f"{h:.3f}" for h in hist), flush=True)
+    assert hist[-1] < 0.5 * hist[0]
+    err_net = rel_l2_per_site(net.predict(xv), tv).mean().item()
+    mean_psf = t.mean(1, keepdim=True).expand(-1, len(val), -1, -1).contiguous()
+    err_mean = rel_l2_per_site(mean_psf, tv).mean().item()
     print(f"validation rel L2: trained {err_net:.4f}, mean-PSF predictor {err_mean:.4f}", flush=True)
-    assert err_net < err_mean
+    assert err_net < 0.85 * err_mean
