@@ -83,6 +83,180 @@ __global__ __launch_bounds__(GT) void gemm_bf16_kernel(const __bf16* __restrict_
       }
 }
 
+// Tiled GEMM for operands with one unit stride each (every product of the network):
+//   AK: A(m, k) = A[m lda + k] (k contiguous), else A(m, k) = A[k lda + m] (m contiguous);
+//   BK: B(k, n) = B[n ldb + k] (k contiguous), else B(k, n) = B[k ldb + n] (n contiguous).
+// BM x BN block tile, K in slabs of 32, 256 threads = 2 x 2 waves of (BM/2) x (BN/2).
+// Global loads are 8-element (16-byte when VEC) runs along the contiguous direction; the
+// next slab is fetched into registers while the MFMAs consume the current one from LDS.
+// LDS rows hold 32 k + 8 pad (80 bytes): the 16 rows one MFMA fragment read touches fall
+// in distinct 16-byte bank groups.
+union Vec8 {
+  uint4 v;
+  unsigned short h[8];
+};
+
+template <bool VEC>
+__device__ __forceinline__ uint4 load_run(const unsigned short* __restrict__ p, int valid) {
+  Vec8 r;
+  if (VEC && valid >= 8) {
+    r.v = *reinterpret_cast<const uint4*>(p);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) r.h[t] = t < valid ? p[t] : (unsigned short)0;
+  }
+  return r.v;
+}
+
+constexpr int SK = 32, SKP = SK + 8;
+
+template <int BM, int BN, bool AK, bool BK, bool VEC>
+__global__ __launch_bounds__(256) void gemm_tiled_kernel(
+    const unsigned short* __restrict__ A, const unsigned short* __restrict__ B,
+    float* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb, int64_t ldc,
+    float alpha, float beta) {
+  constexpr int RA = BM * SK / 8 / 256, RB = BN * SK / 8 / 256;
+  constexpr int FM = BM / 32, FN = BN / 32;
+  __shared__ __attribute__((aligned(16))) unsigned short As[BM][SKP];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[BN][SKP];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  uint4 ra[RA], rb[RB];
+
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int v = tid + r * 256;
+      if (AK) {
+        const int row = v / (SK / 8), kv = (v % (SK / 8)) * 8;
+        const int gm = m0 + row, gk = k0 + kv;
+        ra[r] = load_run<VEC>(A + (int64_t)gm * lda + gk, gm < M ? K - gk : 0);
+      } else {
+        const int k = v / (BM / 8), mv = (v % (BM / 8)) * 8;
+        const int gm = m0 + mv, gk = k0 + k;
+        ra[r] = load_run<VEC>(A + (int64_t)gk * lda + gm, gk < K ? M - gm : 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int v = tid + r * 256;
+      if (BK) {
+        const int col = v / (SK / 8), kv = (v % (SK / 8)) * 8;
+        const int gn = n0 + col, gk = k0 + kv;
+        rb[r] = load_run<VEC>(B + (int64_t)gn * ldb + gk, gn < N ? K - gk : 0);
+      } else {
+        const int k = v / (BN / 8), nv = (v % (BN / 8)) * 8;
+        const int gn = n0 + nv, gk = k0 + k;
+        rb[r] = load_run<VEC>(B + (int64_t)gk * ldb + gn, gk < K ? N - gn : 0);
+      }
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int v = tid + r * 256;
+      if (AK) {
+        *reinterpret_cast<uint4*>(&As[v / (SK / 8)][(v % (SK / 8)) * 8]) = ra[r];
+      } else {
+        Vec8 u;
+        u.v = ra[r];
+        const int k = v / (BM / 8), mv = (v % (BM / 8)) * 8;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) As[mv + t][k] = u.h[t];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int v = tid + r * 256;
+      if (BK) {
+        *reinterpret_cast<uint4*>(&Bs[v / (SK / 8)][(v % (SK / 8)) * 8]) = rb[r];
+      } else {
+        Vec8 u;
+        u.v = rb[r];
+        const int k = v / (BN / 8), nv = (v % (BN / 8)) * 8;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) Bs[nv + t][k] = u.h[t];
+      }
+    }
+  };
+
+  f32x4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  fetch(0);
+  const int row = lane & 15, kq = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < K; k0 += SK) {
+    stage();
+    __syncthreads();
+    if (k0 + SK < K) fetch(k0 + SK);
+    bf16x8 af[FM], bfr[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+      af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&As[wm + 16 * i + row][kq]));
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+      bfr[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[wn + 16 * j + row][kq]));
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + t;
+        const int gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          float* c = C + (int64_t)gm * ldc + gn;
+          *c = alpha * acc[i][j][t] + (beta != 0.f ? beta * *c : 0.f);
+        }
+      }
+}
+
+template <int BM, bool AK, bool BK, bool VEC>
+void launch_tiled(const void* A, const void* B, float* C, int64_t M, int64_t N, int64_t K,
+                  int64_t lda, int64_t ldb, int64_t ldc, float alpha, float beta,
+                  hipStream_t stream) {
+  constexpr int BN = 128;
+  const dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
+  hipLaunchKernelGGL((gemm_tiled_kernel<BM, BN, AK, BK, VEC>), grid, dim3(256), 0, stream,
+                     (const unsigned short*)A, (const unsigned short*)B, C, (int)M, (int)N,
+                     (int)K, lda, ldb, ldc, alpha, beta);
+}
+
+template <int BM, bool AK, bool BK>
+void launch_tiled_vec(bool vec, const void* A, const void* B, float* C, int64_t M, int64_t N,
+                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, float alpha, float beta,
+                      hipStream_t stream) {
+  if (vec)
+    launch_tiled<BM, AK, BK, true>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, stream);
+  else
+    launch_tiled<BM, AK, BK, false>(A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, stream);
+}
+
+template <int BM>
+void launch_tiled_layout(bool ak, bool bk, bool vec, const void* A, const void* B, float* C,
+                         int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc,
+                         float alpha, float beta, hipStream_t stream) {
+  if (ak && bk)
+    launch_tiled_vec<BM, true, true>(vec, A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, stream);
+  else if (ak)
+    launch_tiled_vec<BM, true, false>(vec, A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, stream);
+  else if (bk)
+    launch_tiled_vec<BM, false, true>(vec, A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, stream);
+  else
+    launch_tiled_vec<BM, false, false>(vec, A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, stream);
+}
+
 // col[(c*k + ky)*k + kx][n], n = (b*Ho + oy)*Wo + ox, from X[c][b][iy][ix] (float -> bf16)
 __global__ void im2col_kernel(const float* __restrict__ X, __bf16* __restrict__ col, int C, int Bn,
                               int H, int W, int k, int s, int p, int Ho, int Wo) {
@@ -308,11 +482,25 @@ MAOMI_API int maomi_gemm_bf16(const void* A, const void* B, float* C, int64_t M,
     maomi_set_error("maomi_gemm_bf16: bad arguments");
     return MAOMI_EINVAL;
   }
-  const dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
-  if (grid.y > 65535) { maomi_set_error("maomi_gemm_bf16: M too large"); return MAOMI_EINVAL; }
-  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(GT), 0, (hipStream_t)stream, (const __bf16*)A,
-                     (const __bf16*)B, C, (int)M, (int)N, (int)K, sam, sak, sbk, sbn, ldc, alpha,
-                     beta);
+  if ((M + TM - 1) / TM > 65535) { maomi_set_error("maomi_gemm_bf16: M too large"); return MAOMI_EINVAL; }
+  const bool ak = sak == 1, am = sam == 1, bk = sbk == 1, bn = sbn == 1;
+  if ((ak || am) && (bk || bn)) {
+    // one unit stride per operand: the tiled kernel; its leading dimensions
+    const int64_t lda = ak ? sam : sak, ldb = bk ? sbn : sbk;
+    const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 8 == 0 &&
+                     ldb % 8 == 0;
+    if (M > 64)
+      launch_tiled_layout<128>(ak, bk, vec, A, B, C, M, N, K, lda, ldb, ldc, alpha, beta,
+                               (hipStream_t)stream);
+    else
+      launch_tiled_layout<64>(ak, bk, vec, A, B, C, M, N, K, lda, ldb, ldc, alpha, beta,
+                              (hipStream_t)stream);
+  } else {
+    const dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(GT), 0, (hipStream_t)stream,
+                       (const __bf16*)A, (const __bf16*)B, C, (int)M, (int)N, (int)K, sam, sak,
+                       sbk, sbn, ldc, alpha, beta);
+  }
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

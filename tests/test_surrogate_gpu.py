@@ -16,7 +16,10 @@ def _bf16_round(a):
 
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(1, 1, 1, 0, 0), (70, 33, 45, 0, 0), (128, 256, 96, 1, 0),
-                                         (37, 129, 300, 0, 1), (64, 64, 32, 1, 1)])
+                                         (37, 129, 300, 0, 1), (64, 64, 32, 1, 1),
+                                         (256, 512, 264, 0, 0), (200, 264, 136, 1, 1),
+                                         (96, 1000, 72, 0, 1), (136, 384, 64, 1, 0),
+                                         (48, 1032, 864, 0, 0), (9, 2176, 192, 0, 0)])
 def test_gemm_bf16_matches_exact_product(gpu_lib, M, N, K, ta, tb):
     from maomi_amd.surrogate import gemm
     rng = np.random.default_rng(M * 7 + N)
@@ -27,12 +30,30 @@ def test_gemm_bf16_matches_exact_product(gpu_lib, M, N, K, ta, tb):
     B = torch.as_tensor(b.T.copy() if tb else b, dtype=torch.bfloat16, device="cuda")
     sam, sak = (1, M) if ta else (K, 1)
     sbk, sbn = (1, K) if tb else (N, 1)
+    # the tiled kernel takes 16-byte runs when both leading dimensions are multiples of 8
+    print(f"leading dims {(M if ta else K)}, {(K if tb else N)}", flush=True)
     C = torch.full((M, N), 7.0, dtype=torch.float32, device="cuda")
     print(f"[gemm] {M}x{N}x{K} ta={ta} tb={tb}", flush=True)
     gemm(A, B, C, M, N, K, sam, sak, sbk, sbn, N)
     assert rel_l2(C.cpu().numpy(), ref) < 1e-6
     gemm(A, B, C, M, N, K, sam, sak, sbk, sbn, N, alpha=0.5, beta=1.0)
     assert rel_l2(C.cpu().numpy(), 1.5 * ref) < 1e-6
+
+
+def test_gemm_bf16_generic_strides(gpu_lib):
+    """Operands with no unit stride go through the generic-stride kernel."""
+    from maomi_amd.surrogate import gemm
+    rng = np.random.default_rng(2)
+    M, N, K = 40, 70, 50
+    a = _bf16_round(rng.standard_normal((M, K)))
+    b = _bf16_round(rng.standard_normal((K, N)))
+    Ab = torch.zeros((M, 2 * K), dtype=torch.bfloat16, device="cuda")
+    Ab[:, ::2] = torch.as_tensor(a, dtype=torch.bfloat16, device="cuda")
+    Bb = torch.zeros((K, 3 * N), dtype=torch.bfloat16, device="cuda")
+    Bb[:, ::3] = torch.as_tensor(b, dtype=torch.bfloat16, device="cuda")
+    C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    gemm(Ab, Bb, C, M, N, K, 2 * K, 2, 3 * N, 3, N)
+    assert rel_l2(C.cpu().numpy(), a @ b) < 1e-6
 
 
 def test_layer_gradients_match_fp64_oracle(gpu_lib):
@@ -196,3 +217,25 @@ def test_trained_surrogate_beats_mean_psf_predictor(gpu_lib):
     err_mean = rel_l2_per_site(mean_psf, tv).mean().item()
     print(f"validation rel L2: trained {err_net:.4f}, mean-PSF predictor {err_mean:.4f}", flush=True)
     assert err_net < 0.85 * err_mean
+
+
+def test_surrogate_stage_audit_and_fallback(gpu_lib):
+    """Both outcomes of the audit: every audited site within tolerance keeps the network's
+    prediction everywhere; a failed audit makes every site the propagator's PSF."""
+    from maomi_amd.plain import PlainStage
+    from maomi_amd.surrogate import SurrogateNet, SurrogateStage, site_features, site_psfs
+    from maomi_amd.workloads import PlainConfig, index_volume
+    cfg = PlainConfig(fov=64, n=256, slices=16, sites_per_axis=9, zernike_std_rad=0.05)
+    vol = torch.as_tensor(index_volume(cfg), device="cuda")
+    st = PlainStage(cfg, vol).plan()
+    net = SurrogateNet(seed=1)
+    truth = site_psfs(st)
+    pred = net.predict(site_features(vol, cfg, st.offsets, cfg.focus)) * 2.0
+    sur = SurrogateStage(net, 2.0, st, audit_frac=0.1, tol=1e9, seed=5)
+    out = sur.run()
+    assert sur.report["mode"] == "surrogate" and sur.report["audited"] == 9
+    torch.testing.assert_close(out, pred, rtol=1e-6, atol=1e-7)
+    sur = SurrogateStage(net, 2.0, st, audit_frac=0.1, tol=0.0, seed=5)
+    out = sur.run()
+    assert sur.report["mode"] == "fallback_all" and sur.report["audit_fail"] == 9
+    torch.testing.assert_close(out, truth, rtol=0, atol=0)
