@@ -13,6 +13,7 @@
 // maomi/optics.py:327-340 (trapezoid over saved planes).
 #include "fft_reg.hpp"
 #include "maomi_amd.h"
+#include "screen_trig.hpp"
 
 namespace {
 
@@ -23,10 +24,42 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// 1: each pass requests the next tile's field lines once the current tile's second stage
+// is done, so that the load latency overlaps the exchange and third stage.
+// Bit 0: row pass, bit 1: column pass.
 #ifndef MAOMI_SITES_PREFETCH
-#define MAOMI_SITES_PREFETCH 0
+#define MAOMI_SITES_PREFETCH 2
 #endif
-constexpr bool kPrefetch = MAOMI_SITES_PREFETCH != 0;  // register prefetch of the next tile
+constexpr bool kPrefetchRows = (MAOMI_SITES_PREFETCH & 1) != 0;
+constexpr bool kPrefetchCols = (MAOMI_SITES_PREFETCH & 2) != 0;
+
+// 1: screens use trig::sincos_screen (|phi| < 2^17, checked on the host); 0: sincosf.
+// 1: the row pass requests a tile's phases and mask pixels before its first stage.
+#ifndef MAOMI_SITES_HOIST
+#define MAOMI_SITES_HOIST 1
+#endif
+constexpr bool kHoist = MAOMI_SITES_HOIST != 0;
+
+#ifndef MAOMI_SITES_FAST_TRIG
+#define MAOMI_SITES_FAST_TRIG 1
+#endif
+// Occupancy target in waves per SIMD for the sites kernels (0: compiler's choice).  4 caps
+// them at 128 VGPRs, so two 512-thread blocks share a CU.
+#ifndef MAOMI_SITES_WAVES
+#define MAOMI_SITES_WAVES 0
+#endif
+#if MAOMI_SITES_WAVES > 0
+#define MAOMI_SITES_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MAOMI_SITES_WAVES)))
+#else
+#define MAOMI_SITES_OCCUPANCY
+#endif
+
+__device__ __forceinline__ void screen_sincos(float phi, float* s, float* c) {
+  if (MAOMI_SITES_FAST_TRIG)
+    trig::sincos_screen(phi, s, c);
+  else
+    sincosf(phi, s, c);
+}
 
 template <int N> struct ClassIdx;
 template <> struct ClassIdx<64> { static constexpr int v = 0; };
@@ -76,7 +109,9 @@ struct Smem {
   double red[NT / 64];
 };
 
-template <int N, int NT>
+// PSF: the launch also writes the focal-window planes (a.psf_out); a compile-time switch so
+// that the plain stage carries none of its address arithmetic.
+template <int N, int NT, bool PSF>
 __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
@@ -109,25 +144,34 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
     // prefetched into registers while the current tile is transformed.
     const bool p1 = tid < L * N1;
     const int l1 = tid / N1, n1 = tid % N1;
-    float2 pre[N2];
-    if (kPrefetch && !first && p1) {
-      const float2* src = fld + (int64_t)l1 * N + n1;
-#pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
-    }
+    float2 pre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
+#pragma unroll 1
     for (int t = 0; t < N / L; ++t) {
       const int row0 = t * L;
+      // this tile's screen phases and mask pixels, requested before stage 1 so that their
+      // latency overlaps the stage-1 loads, transform and exchange
+      const bool p2 = tid < L * N2;
+      const int l = tid / N2, k2 = tid % N2;
+      const int x = row0 + l;
+      float* mrow = mask + (int64_t)x * N + k2;
+      const int64_t prow = (pslice + ox + x) * a.W + oy + k2;
+      constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
+      float phv[N1], mkv[N1];
+      if (hoist && p2) {
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+          if (!last)
+            phv[k1] = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                           : a.phase[prow + N2 * k1];
+          if (!first) mkv[k1] = mrow[N2 * k1];
+        }
+      }
       if (!first) {
         if (p1) {  // stage 1 of the inverse transform along y
           float2 v[N2];
-          if (kPrefetch) {
+          if (kPrefetchRows && t > 0) {
 #pragma unroll
             for (int n2 = 0; n2 < N2; ++n2) v[n2] = pre[n2];
-            if (t + 1 < N / L) {
-              const float2* src = fld + (int64_t)(row0 + L + l1) * N + n1;
-#pragma unroll
-              for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
-            }
           } else {
             const float2* src = fld + (int64_t)(row0 + l1) * N + n1;
 #pragma unroll
@@ -141,10 +185,7 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
         lds_barrier();
       }
       float2 v[N1];
-      const bool p2 = tid < L * N2;
-      const int l = tid / N2, k2 = tid % N2;
       if (p2) {
-        const int x = row0 + l;
         if (first) {
           const float2* src = base + (int64_t)x * N + k2;
 #pragma unroll
@@ -156,28 +197,27 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
           fftk::fft<N1, true>(v);
         }
         const float ex = sm.eg[x];
-        float* mrow = mask + (int64_t)x * N + k2;
-        const int64_t prow = (pslice + ox + x) * a.W + oy + k2;
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) {
           const int y = k2 + N2 * k1;
           if (!last) {
-            const float phi = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                                   : a.phase[prow + N2 * k1];
+            const float phi = hoist ? phv[k1]
+                              : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                                     : a.phase[prow + N2 * k1];
             float sn, cs;
-            sincosf(phi, &sn, &cs);
+            screen_sincos(phi, &sn, &cs);
             const float sg = ex * sm.eg[y];
             v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
           }
           const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
-          if (a.psf_out && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
+          if (PSF && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
             const int cx = x - a.psf_c0, cy = y - a.psf_c0;
             if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
               a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + site) * a.psf_w + cx) * a.psf_w + cy] =
                   pw * pw;
           }
           const float m = wgt * (pw * pw);
-          const float acc = first ? m : mrow[N2 * k1] + m;
+          const float acc = first ? m : (hoist ? mkv[k1] : mrow[N2 * k1]) + m;
           mrow[N2 * k1] = acc;
           if (last) esum += (double)acc;
         }
@@ -186,6 +226,11 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
 #pragma unroll
           for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
         }
+      }
+      if (kPrefetchRows && !first && p1 && t + 1 < N / L) {
+        const float2* src = fld + (int64_t)(row0 + L + l1) * N + n1;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
       }
       lds_barrier();
       if (!last) {
@@ -216,24 +261,15 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
     {
       const bool c1 = tid < L * N1;
       const int lc = tid % L, nc = tid / L;
-      float2 cpre[N2];
-      if (kPrefetch && c1) {
-        const float2* src = fld + (int64_t)nc * N + lc;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
-      }
-      for (int t = 0; t < N / L; ++t) {
+      float2 cpre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
+  #pragma unroll 1
+    for (int t = 0; t < N / L; ++t) {
         const int col0 = t * L;
         if (c1) {
           float2 v[N2];
-          if (kPrefetch) {
+          if (kPrefetchCols && t > 0) {
 #pragma unroll
             for (int n2 = 0; n2 < N2; ++n2) v[n2] = cpre[n2];
-            if (t + 1 < N / L) {
-              const float2* src = fld + (int64_t)nc * N + col0 + L + lc;
-#pragma unroll
-              for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
-            }
           } else {
             const float2* src = fld + (int64_t)nc * N + col0 + lc;
 #pragma unroll
@@ -259,6 +295,11 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
           fftk::fft<N1, true>(v);
 #pragma unroll
           for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+        }
+        if (kPrefetchCols && c1 && t + 1 < N / L) {
+          const float2* src = fld + (int64_t)nc * N + col0 + L + lc;
+#pragma unroll
+          for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
         }
         lds_barrier();
         if (p2) {
@@ -289,23 +330,23 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
 }
 
 // One field side for every site of the launch.
-template <int N, int NT>
-__global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
+template <int N, int NT, bool PSF>
+__global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_kernel(SitesArgs a) {
   __shared__ Smem<NT> sm;
-  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT>(a, a.order[i], sm);
+  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, PSF>(a, a.order[i], sm);
 }
 
 // Sites of mixed field sides in one launch (the order list puts large sites first).
 template <int NT>
-__global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
+__global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_mixed_kernel(SitesArgs a) {
   __shared__ Smem<NT> sm;
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
     const int site = a.order[i];
     switch (a.site_n[site]) {
-      case 64: run_site<64, NT>(a, site, sm); break;
-      case 128: run_site<128, NT>(a, site, sm); break;
-      case 256: run_site<256, NT>(a, site, sm); break;
-      case 512: run_site<512, NT>(a, site, sm); break;
+      case 64: run_site<64, NT, false>(a, site, sm); break;
+      case 128: run_site<128, NT, false>(a, site, sm); break;
+      case 256: run_site<256, NT, false>(a, site, sm); break;
+      case 512: run_site<512, NT, false>(a, site, sm); break;
       default: break;  // rejected on the host
     }
   }
@@ -361,7 +402,7 @@ MAOMI_API int64_t maomi_sites_default_grid(void) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256, 512>, 512, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256, 512, false>, 512, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -415,21 +456,28 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   }
   const dim3 g((unsigned)grid), b(nt);
   hipStream_t st = (hipStream_t)stream;
-#define MAOMI_LAUNCH(NTV)                                                                  \
-  switch (d->uniform_n) {                                                                  \
-    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV>), g, b, 0, st, a); break;           \
-    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV>), g, b, 0, st, a); break;         \
-    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV>), g, b, 0, st, a); break;         \
-    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV>), g, b, 0, st, a); break;         \
-    case 0: hipLaunchKernelGGL((sites_mixed_kernel<NTV>), g, b, 0, st, a); break;          \
-    default:                                                                               \
-      maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512"); \
-      return MAOMI_EUNSUPPORTED;                                                           \
+  if (a.psf_out && (d->uniform_n == 0 || nt != 512)) {
+    maomi_set_error("maomi_persistent_sites: focal-plane output needs a uniform field side and "
+                    "block_threads 512");
+    return MAOMI_EUNSUPPORTED;
   }
-  if (nt == 512) {
-    MAOMI_LAUNCH(512)
+#define MAOMI_LAUNCH(NTV, PSFV)                                                                \
+  switch (d->uniform_n) {                                                                      \
+    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV, PSFV>), g, b, 0, st, a); break;         \
+    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV, PSFV>), g, b, 0, st, a); break;       \
+    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV, PSFV>), g, b, 0, st, a); break;       \
+    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV, PSFV>), g, b, 0, st, a); break;       \
+    case 0: hipLaunchKernelGGL((sites_mixed_kernel<NTV>), g, b, 0, st, a); break;              \
+    default:                                                                                   \
+      maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");     \
+      return MAOMI_EUNSUPPORTED;                                                               \
+  }
+  if (a.psf_out) {
+    MAOMI_LAUNCH(512, true)
+  } else if (nt == 512) {
+    MAOMI_LAUNCH(512, false)
   } else {
-    MAOMI_LAUNCH(256)
+    MAOMI_LAUNCH(256, false)
   }
 #undef MAOMI_LAUNCH
   MAOMI_CHECK_LAUNCH();

@@ -102,6 +102,19 @@ def v4_masks(points, values, shape, device):
     return out.reshape(shape[0], shape[1], C)
 
 
+# Screen phases must stay below this magnitude (radians): the sites kernel's sin/cos
+# reduction is exact only there.  A physical screen is a few radians.
+SCREEN_PHASE_LIMIT = 131072.0
+
+
+def check_screen_range(slice_max):
+    """Reject phase screens outside the sites kernel's sin/cos domain (one small D2H)."""
+    m = float(slice_max.max())
+    if not m < SCREEN_PHASE_LIMIT:
+        raise ValueError(f"phase screen magnitude {m:.6g} rad outside the supported range "
+                         f"(< {SCREEN_PHASE_LIMIT:g} rad); check the index volume and n0")
+
+
 class PlainStage:
     """One plain optical-stage run on `device` (uniform N and S over all sites)."""
 
@@ -122,6 +135,7 @@ class PlainStage:
         if np.any(self.offsets < 0) or np.any(self.offsets > cfg.fov):
             raise ValueError("site windows must lie inside the padded volume")
         self.grid = int(grid)
+        self.block_threads = 0  # 0: the library's default (512)
         self.bp = dict(beam_parameters(cfg), wvl_um=cfg.wavelength_um)
         self.zern = zernike_weights(cfg)
         self.timing = {}
@@ -144,16 +158,16 @@ class PlainStage:
         H = W = cfg.padded
         self.phase = torch.empty((Z, H, W), dtype=torch.float32, device=d)
         self.phase16 = self.scale = None
-        smax = None
+        smax = torch.empty(Z, dtype=torch.float32, device=d)
         if self.precision == "mixed":
             self.phase16 = torch.empty((Z, H, W), dtype=torch.float16, device=d)
             self.scale = torch.empty(Z, dtype=torch.float32, device=d)
-            smax = torch.empty(Z, dtype=torch.float32, device=d)
         nat.check(lib.maomi_phase_prepare(nat.ptr(self.volume), H, W, Z * cfg.voxels_per_slice,
                                           cfg.voxels_per_slice, float(cfg.n0),
                                           float(self.bp["k0dz"]), nat.ptr(self.phase),
                                           nat.ptr(self.phase16), nat.ptr(self.scale),
                                           nat.ptr(smax), st), "phase prepare")
+        check_screen_range(smax)
         if self.precision == "mixed":
             self.phase = None  # only the float16 screens are read by the sites kernel
         self.tables = SiteTables([cfg.n], self.bp["dx"], self.bp["dz"],
@@ -205,7 +219,7 @@ class PlainStage:
         desc.workspace = self.workspace.data_ptr()
         desc.scratch_stride = stride
         desc.P, desc.H, desc.W, desc.Z = P, cfg.padded, cfg.padded, cfg.slices
-        desc.grid, desc.uniform_n = grid, cfg.n
+        desc.grid, desc.uniform_n, desc.block_threads = grid, cfg.n, self.block_threads
         for i in range(5):
             desc.class_edge[i] = desc.class_q[i] = desc.class_tw[i] = self.tables.offsets[i]
         if psf is not None:

@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .plain import SiteTables, _CLASSES, _SLOT, base_beam, make_pupil, v4_masks
+from .plain import SiteTables, _CLASSES, _SLOT, base_beam, check_screen_range, make_pupil, v4_masks
 from .workloads import PlainConfig, beam_parameters, zernike_weights
 
 
@@ -106,10 +106,12 @@ class UnevenStage:
         # buffers are allocated once: a captured launch keeps reading the same addresses
         if not hasattr(self, "phase"):
             self.phase = torch.empty((Z, H, H), dtype=torch.float32, device=d)
+            self.smax = torch.empty(Z, dtype=torch.float32, device=d)
         nat.check(lib.maomi_phase_prepare(nat.ptr(self.volume), H, H, Z * cfg.voxels_per_slice,
                                           cfg.voxels_per_slice, float(cfg.n0),
                                           float(self.bp["k0dz"]), nat.ptr(self.phase), None, None,
-                                          None, st), "phase prepare")
+                                          nat.ptr(self.smax), st), "phase prepare")
+        check_screen_range(self.smax)
         uniq = sorted({int(n) for n in self.sizes})
         tables = SiteTables(uniq, self.bp["dx"], self.bp["dz"],
                             float(self.bp["wvl"]) / float(self.bp["nr"]), d)
