@@ -109,12 +109,226 @@ struct Smem {
   double red[NT / 64];
 };
 
-// PSF: the launch also writes the focal-window planes (a.psf_out); a compile-time switch so
-// that the plain stage carries none of its address arithmetic.
-template <int N, int NT, bool PSF>
-__device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
+// Launch modes, compile-time so that each kernel carries only its own address arithmetic:
+// FP32 screens; FP16 screens with a per-slice scale; FP32 screens plus the focal-window
+// planes of |u|^4 (a.psf_out).
+enum Mode { kScreens32 = 0, kScreens16 = 1, kScreens32Psf = 2 };
+
+// Per-site state shared by the passes.
+struct SiteCtx {
+  int site;
+  int64_t ox, oy;
+  const float2* base;
+  float* mask;
+  float2* fld;
+};
+
+// Kinds of row pass: the first slice starts from the entry field and applies its screen,
+// middle slices transform back, apply the screen and transform forward, the last slice only
+// transforms back and accumulates; a site with no screens is first and last at once.  Each
+// kind is its own instantiation, so the per-pixel loops carry no slice-dependent branches.
+enum RowKind { kFirst = 0, kMiddle = 1, kLast = 2, kOnly = 3 };
+
+// Row pass of slice s: [IFFT_y] -> screen, |u|^4 trapezoid -> [FFT_y].
+// Barriers inside a pass only order LDS; each thread's mask pixels and the phase volume need
+// no cross-thread ordering.
+template <int N, int NT, int MODE, int KIND>
+__device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, int s,
+                                         Smem<NT>& sm, double& esum) {
+  constexpr bool half = MODE == kScreens16, PSF = MODE == kScreens32Psf;
+  constexpr bool FIRST = KIND == kFirst || KIND == kOnly, LAST = KIND == kLast || KIND == kOnly;
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
+  constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
+  const int tid = threadIdx.x;
+  const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
+  const float sscale = half && !LAST ? a.slice_scale[s] : 1.0f;
+  const int64_t pslice = (int64_t)s * a.H;
+  const bool p1 = tid < L * N1;
+  const int l1 = tid / N1, n1 = tid % N1;
+  const bool p2 = tid < L * N2;
+  const int l = tid / N2, k2 = tid % N2;
+  float2 pre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
+#pragma unroll 1
+  for (int t = 0; t < N / L; ++t) {
+    const int row0 = t * L;
+    const int x = row0 + l;
+    float* mrow = c.mask + (int64_t)x * N + k2;
+    const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
+    // this tile's screen phases and mask pixels, requested before stage 1 so that their
+    // latency overlaps the stage-1 loads, transform and exchange
+    float phv[N1], mkv[N1];
+    if (hoist && p2) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        if (!LAST)
+          phv[k1] = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                         : a.phase[prow + N2 * k1];
+        if (!FIRST) mkv[k1] = mrow[N2 * k1];
+      }
+    }
+    if (!FIRST) {
+      if (p1) {  // stage 1 of the inverse transform along y
+        float2 v[N2];
+        if (kPrefetchRows && t > 0) {
+#pragma unroll
+          for (int n2 = 0; n2 < N2; ++n2) v[n2] = pre[n2];
+        } else {
+          const float2* src = c.fld + (int64_t)(row0 + l1) * N + n1;
+#pragma unroll
+          for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
+        }
+        fftk::fft<N2, true>(v);
+        float2* dst = sm.x + l1 * LS + n1 * (N2 + 1);
+#pragma unroll
+        for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmulc(v[j], sm.tw[n1 * j]);
+      }
+      lds_barrier();
+    }
+    float2 v[N1];
+    if (p2) {
+      if (FIRST) {
+        const float2* src = c.base + (int64_t)x * N + k2;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = src[N2 * k1];
+      } else {
+        const float2* src = sm.x + l * LS + k2;
+#pragma unroll
+        for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+        fftk::fft<N1, true>(v);
+      }
+      const float ex = sm.eg[x];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const int y = k2 + N2 * k1;
+        if (!LAST) {
+          const float phi = hoist ? phv[k1]
+                            : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                                   : a.phase[prow + N2 * k1];
+          float sn, cs;
+          screen_sincos(phi, &sn, &cs);
+          const float sg = ex * sm.eg[y];
+          v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
+        }
+        const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
+        if (PSF && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
+          const int cx = x - a.psf_c0, cy = y - a.psf_c0;
+          if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
+            a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + c.site) * a.psf_w + cx) * a.psf_w + cy] =
+                pw * pw;
+        }
+        const float m = wgt * (pw * pw);
+        const float acc = FIRST ? m : (hoist ? mkv[k1] : mrow[N2 * k1]) + m;
+        mrow[N2 * k1] = acc;
+        if (LAST) esum += (double)acc;
+      }
+      if (!LAST) {  // stage 1 of the forward transform along y (factors swapped)
+        fftk::fft<N1, false>(v);
+#pragma unroll
+        for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
+      }
+    }
+    if (kPrefetchRows && !FIRST && p1 && t + 1 < N / L) {
+      const float2* src = c.fld + (int64_t)(row0 + L + l1) * N + n1;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
+    }
+    lds_barrier();
+    if (!LAST) {
+      if (p2) {
+        float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) dst[j] = v[j];
+      }
+      lds_barrier();
+      if (p1) {  // stage 2 of the forward transform along y
+        float2 w[N2];
+        const float2* src = sm.x + l1 * LS + n1;
+#pragma unroll
+        for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
+        fftk::fft<N2, false>(w);
+        float2* dst = c.fld + (int64_t)(row0 + l1) * N + n1;
+#pragma unroll
+        for (int k1 = 0; k1 < N2; ++k1) dst[N1 * k1] = w[k1];
+      }
+      lds_barrier();
+    }
+  }
+  __syncthreads();  // the row pass's stores are read by other threads in the column pass
+}
+
+// Column pass: FFT_x -> * Q (separable, 1/N^2 folded in) -> IFFT_x.
+template <int N, int NT>
+__device__ __forceinline__ void column_pass(const SiteCtx& c, Smem<NT>& sm) {
+  using T = fftk::Tile<N, NT>;
+  constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
+  const int tid = threadIdx.x;
+  const bool c1 = tid < L * N1;
+  const int lc = tid % L, nc = tid / L;
+  const bool p2 = tid < L * N2;
+  const int l = tid % L, k2 = tid / L;
+  float2 cpre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
+#pragma unroll 1
+  for (int t = 0; t < N / L; ++t) {
+    const int col0 = t * L;
+    if (c1) {
+      float2 v[N2];
+      if (kPrefetchCols && t > 0) {
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = cpre[n2];
+      } else {
+        const float2* src = c.fld + (int64_t)nc * N + col0 + lc;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[(int64_t)N1 * N * n2];
+      }
+      fftk::fft<N2, false>(v);
+      float2* dst = sm.x + lc * LS + nc * (N2 + 1);
+#pragma unroll
+      for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[nc * j]);
+    }
+    lds_barrier();
+    float2 v[N1];
+    if (p2) {
+      const float2* src = sm.x + l * LS + k2;
+#pragma unroll
+      for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+      fftk::fft<N1, false>(v);
+      const float2 qc = sm.q[col0 + l];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
+      fftk::fft<N1, true>(v);
+#pragma unroll
+      for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+    }
+    if (kPrefetchCols && c1 && t + 1 < N / L) {
+      const float2* src = c.fld + (int64_t)nc * N + col0 + L + lc;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
+    }
+    lds_barrier();
+    if (p2) {
+      float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) dst[j] = v[j];
+    }
+    lds_barrier();
+    if (c1) {
+      float2 w[N2];
+      const float2* src = sm.x + lc * LS + nc;
+#pragma unroll
+      for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
+      fftk::fft<N2, true>(w);
+      float2* dst = c.fld + (int64_t)nc * N + col0 + lc;
+#pragma unroll
+      for (int k1 = 0; k1 < N2; ++k1) dst[(int64_t)N1 * N * k1] = w[k1];
+    }
+    lds_barrier();
+  }
+  __syncthreads();  // the column pass's stores are read by other threads in the next row pass
+}
+
+template <int N, int NT, int MODE>
+__device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
   constexpr int C = ClassIdx<N>::v;
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) {
@@ -125,204 +339,19 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
   __syncthreads();
 
   const int S = a.site_s[site];
-  const int64_t ox = a.site_x[site], oy = a.site_y[site];
-  const float2* base = a.bases + a.base_off[site];
-  float* mask = a.mask + a.mask_off[site];
-  float2* fld = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
-  const bool half = a.phase16 != nullptr;
+  const SiteCtx c{site, a.site_x[site], a.site_y[site], a.bases + a.base_off[site],
+                  a.mask + a.mask_off[site], a.scratch + (int64_t)blockIdx.x * a.scratch_stride};
   double esum = 0.0;
-
-  for (int s = 0; s <= S; ++s) {
-    const bool first = (s == 0), last = (s == S);
-    const float wgt = (first || last) ? 0.5f : 1.0f;
-    const float sscale = half && !last ? a.slice_scale[s] : 1.0f;
-    const int64_t pslice = (int64_t)s * a.H;
-
-    // ---------------- row pass: [IFFT_y] -> screen, |u|^4 -> [FFT_y]
-    // Barriers inside a pass only order LDS; each thread's mask pixels and the
-    // phase volume need no cross-thread ordering, and the next tile's rows are
-    // prefetched into registers while the current tile is transformed.
-    const bool p1 = tid < L * N1;
-    const int l1 = tid / N1, n1 = tid % N1;
-    float2 pre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
-#pragma unroll 1
-    for (int t = 0; t < N / L; ++t) {
-      const int row0 = t * L;
-      // this tile's screen phases and mask pixels, requested before stage 1 so that their
-      // latency overlaps the stage-1 loads, transform and exchange
-      const bool p2 = tid < L * N2;
-      const int l = tid / N2, k2 = tid % N2;
-      const int x = row0 + l;
-      float* mrow = mask + (int64_t)x * N + k2;
-      const int64_t prow = (pslice + ox + x) * a.W + oy + k2;
-      constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
-      float phv[N1], mkv[N1];
-      if (hoist && p2) {
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-          if (!last)
-            phv[k1] = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                           : a.phase[prow + N2 * k1];
-          if (!first) mkv[k1] = mrow[N2 * k1];
-        }
-      }
-      if (!first) {
-        if (p1) {  // stage 1 of the inverse transform along y
-          float2 v[N2];
-          if (kPrefetchRows && t > 0) {
-#pragma unroll
-            for (int n2 = 0; n2 < N2; ++n2) v[n2] = pre[n2];
-          } else {
-            const float2* src = fld + (int64_t)(row0 + l1) * N + n1;
-#pragma unroll
-            for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
-          }
-          fftk::fft<N2, true>(v);
-          float2* dst = sm.x + l1 * LS + n1 * (N2 + 1);
-#pragma unroll
-          for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmulc(v[k2], sm.tw[n1 * k2]);
-        }
-        lds_barrier();
-      }
-      float2 v[N1];
-      if (p2) {
-        if (first) {
-          const float2* src = base + (int64_t)x * N + k2;
-#pragma unroll
-          for (int k1 = 0; k1 < N1; ++k1) v[k1] = src[N2 * k1];
-        } else {
-          const float2* src = sm.x + l * LS + k2;
-#pragma unroll
-          for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-          fftk::fft<N1, true>(v);
-        }
-        const float ex = sm.eg[x];
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-          const int y = k2 + N2 * k1;
-          if (!last) {
-            const float phi = hoist ? phv[k1]
-                              : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                                     : a.phase[prow + N2 * k1];
-            float sn, cs;
-            screen_sincos(phi, &sn, &cs);
-            const float sg = ex * sm.eg[y];
-            v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
-          }
-          const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
-          if (PSF && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
-            const int cx = x - a.psf_c0, cy = y - a.psf_c0;
-            if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
-              a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + site) * a.psf_w + cx) * a.psf_w + cy] =
-                  pw * pw;
-          }
-          const float m = wgt * (pw * pw);
-          const float acc = first ? m : (hoist ? mkv[k1] : mrow[N2 * k1]) + m;
-          mrow[N2 * k1] = acc;
-          if (last) esum += (double)acc;
-        }
-        if (!last) {  // stage 1 of the forward transform along y (factors swapped)
-          fftk::fft<N1, false>(v);
-#pragma unroll
-          for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
-        }
-      }
-      if (kPrefetchRows && !first && p1 && t + 1 < N / L) {
-        const float2* src = fld + (int64_t)(row0 + L + l1) * N + n1;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
-      }
-      lds_barrier();
-      if (!last) {
-        if (p2) {
-          float2* dst = sm.x + l * LS + k2 * (N1 + 1);
-#pragma unroll
-          for (int j = 0; j < N1; ++j) dst[j] = v[j];
-        }
-        lds_barrier();
-        if (tid < L * N1) {  // stage 2 of the forward transform along y
-          const int l3 = tid / N1, k2p = tid % N1;
-          float2 w[N2];
-          const float2* src = sm.x + l3 * LS + k2p;
-#pragma unroll
-          for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
-          fftk::fft<N2, false>(w);
-          float2* dst = fld + (int64_t)(row0 + l3) * N + k2p;
-#pragma unroll
-          for (int k1 = 0; k1 < N2; ++k1) dst[N1 * k1] = w[k1];
-        }
-        lds_barrier();
-      }
+  if (S < 1) {
+    row_pass<N, NT, MODE, kOnly>(a, c, 0, sm, esum);
+  } else {
+    row_pass<N, NT, MODE, kFirst>(a, c, 0, sm, esum);
+    column_pass<N, NT>(c, sm);
+    for (int s = 1; s < S; ++s) {
+      row_pass<N, NT, MODE, kMiddle>(a, c, s, sm, esum);
+      column_pass<N, NT>(c, sm);
     }
-    __syncthreads();  // the row pass's stores are read by other threads below
-    if (last) break;
-
-    // ---------------- column pass: FFT_x -> * Q -> IFFT_x
-    {
-      const bool c1 = tid < L * N1;
-      const int lc = tid % L, nc = tid / L;
-      float2 cpre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
-  #pragma unroll 1
-    for (int t = 0; t < N / L; ++t) {
-        const int col0 = t * L;
-        if (c1) {
-          float2 v[N2];
-          if (kPrefetchCols && t > 0) {
-#pragma unroll
-            for (int n2 = 0; n2 < N2; ++n2) v[n2] = cpre[n2];
-          } else {
-            const float2* src = fld + (int64_t)nc * N + col0 + lc;
-#pragma unroll
-            for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[(int64_t)N1 * N * n2];
-          }
-          fftk::fft<N2, false>(v);
-          float2* dst = sm.x + lc * LS + nc * (N2 + 1);
-#pragma unroll
-          for (int k2 = 0; k2 < N2; ++k2) dst[k2] = k2 == 0 ? v[k2] : cmul(v[k2], sm.tw[nc * k2]);
-        }
-        lds_barrier();
-        float2 v[N1];
-        const bool p2 = tid < L * N2;
-        const int l = tid % L, k2 = tid / L;
-        if (p2) {
-          const float2* src = sm.x + l * LS + k2;
-#pragma unroll
-          for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-          fftk::fft<N1, false>(v);
-          const float2 qc = sm.q[col0 + l];
-#pragma unroll
-          for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
-          fftk::fft<N1, true>(v);
-#pragma unroll
-          for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
-        }
-        if (kPrefetchCols && c1 && t + 1 < N / L) {
-          const float2* src = fld + (int64_t)nc * N + col0 + L + lc;
-#pragma unroll
-          for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
-        }
-        lds_barrier();
-        if (p2) {
-          float2* dst = sm.x + l * LS + k2 * (N1 + 1);
-#pragma unroll
-          for (int j = 0; j < N1; ++j) dst[j] = v[j];
-        }
-        lds_barrier();
-        if (tid < L * N1) {
-          const int l3 = tid % L, k2p = tid / L;
-          float2 w[N2];
-          const float2* src = sm.x + l3 * LS + k2p;
-#pragma unroll
-          for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
-          fftk::fft<N2, true>(w);
-          float2* dst = fld + (int64_t)k2p * N + col0 + l3;
-#pragma unroll
-          for (int k1 = 0; k1 < N2; ++k1) dst[(int64_t)N1 * N * k1] = w[k1];
-        }
-        lds_barrier();
-      }
-    }
-    __syncthreads();  // the column pass's stores are read by other threads in the next row pass
+    row_pass<N, NT, MODE, kLast>(a, c, S, sm, esum);
   }
   const double tot = block_sum_double(esum, sm.red);
   if (tid == 0) a.energy[site] = tot;
@@ -330,10 +359,10 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
 }
 
 // One field side for every site of the launch.
-template <int N, int NT, bool PSF>
+template <int N, int NT, int MODE>
 __global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_kernel(SitesArgs a) {
   __shared__ Smem<NT> sm;
-  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, PSF>(a, a.order[i], sm);
+  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, MODE>(a, a.order[i], sm);
 }
 
 // Sites of mixed field sides in one launch (the order list puts large sites first).
@@ -343,10 +372,10 @@ __global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_mixed_kernel(S
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
     const int site = a.order[i];
     switch (a.site_n[site]) {
-      case 64: run_site<64, NT, false>(a, site, sm); break;
-      case 128: run_site<128, NT, false>(a, site, sm); break;
-      case 256: run_site<256, NT, false>(a, site, sm); break;
-      case 512: run_site<512, NT, false>(a, site, sm); break;
+      case 64: run_site<64, NT, kScreens32>(a, site, sm); break;
+      case 128: run_site<128, NT, kScreens32>(a, site, sm); break;
+      case 256: run_site<256, NT, kScreens32>(a, site, sm); break;
+      case 512: run_site<512, NT, kScreens32>(a, site, sm); break;
       default: break;  // rejected on the host
     }
   }
@@ -402,7 +431,7 @@ MAOMI_API int64_t maomi_sites_default_grid(void) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256, 512, false>, 512, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256, 512, kScreens32>, 512, 0) !=
           hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -456,28 +485,39 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   }
   const dim3 g((unsigned)grid), b(nt);
   hipStream_t st = (hipStream_t)stream;
-  if (a.psf_out && (d->uniform_n == 0 || nt != 512)) {
-    maomi_set_error("maomi_persistent_sites: focal-plane output needs a uniform field side and "
-                    "block_threads 512");
+  if (a.psf_out && (d->uniform_n == 0 || nt != 512 || a.phase16)) {
+    maomi_set_error("maomi_persistent_sites: focal-plane output needs a uniform field side, "
+                    "FP32 screens and block_threads 512");
     return MAOMI_EUNSUPPORTED;
   }
-#define MAOMI_LAUNCH(NTV, PSFV)                                                                \
+  if (a.phase16 && d->uniform_n == 0) {
+    maomi_set_error("maomi_persistent_sites: FP16 screens need a uniform field side");
+    return MAOMI_EUNSUPPORTED;
+  }
+  const int mode = a.psf_out ? kScreens32Psf : a.phase16 ? kScreens16 : kScreens32;
+#define MAOMI_LAUNCH(NTV, MODEV)                                                               \
   switch (d->uniform_n) {                                                                      \
-    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV, PSFV>), g, b, 0, st, a); break;         \
-    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV, PSFV>), g, b, 0, st, a); break;       \
-    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV, PSFV>), g, b, 0, st, a); break;       \
-    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV, PSFV>), g, b, 0, st, a); break;       \
+    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV, MODEV>), g, b, 0, st, a); break;        \
+    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV, MODEV>), g, b, 0, st, a); break;      \
+    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV, MODEV>), g, b, 0, st, a); break;      \
+    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV, MODEV>), g, b, 0, st, a); break;      \
     case 0: hipLaunchKernelGGL((sites_mixed_kernel<NTV>), g, b, 0, st, a); break;              \
     default:                                                                                   \
       maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");     \
       return MAOMI_EUNSUPPORTED;                                                               \
   }
-  if (a.psf_out) {
-    MAOMI_LAUNCH(512, true)
+  if (mode == kScreens32Psf) {
+    MAOMI_LAUNCH(512, kScreens32Psf)
+  } else if (mode == kScreens16) {
+    if (nt == 512) {
+      MAOMI_LAUNCH(512, kScreens16)
+    } else {
+      MAOMI_LAUNCH(256, kScreens16)
+    }
   } else if (nt == 512) {
-    MAOMI_LAUNCH(512, false)
+    MAOMI_LAUNCH(512, kScreens32)
   } else {
-    MAOMI_LAUNCH(256, false)
+    MAOMI_LAUNCH(256, kScreens32)
   }
 #undef MAOMI_LAUNCH
   MAOMI_CHECK_LAUNCH();
