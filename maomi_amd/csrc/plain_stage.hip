@@ -133,14 +133,13 @@ enum RowKind { kFirst = 0, kMiddle = 1, kLast = 2, kOnly = 3 };
 // Barriers inside a pass only order LDS; each thread's mask pixels and the phase volume need
 // no cross-thread ordering.
 template <int N, int NT, int MODE, int KIND>
-__device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, int s,
+__device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, int s, int tid,
                                          Smem<NT>& sm, double& esum) {
   constexpr bool half = MODE == kScreens16, PSF = MODE == kScreens32Psf;
   constexpr bool FIRST = KIND == kFirst || KIND == kOnly, LAST = KIND == kLast || KIND == kOnly;
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
   constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
-  const int tid = threadIdx.x;
   const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
   const float sscale = half && !LAST ? a.slice_scale[s] : 1.0f;
   const int64_t pslice = (int64_t)s * a.H;
@@ -259,21 +258,21 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
 
 // Column pass: FFT_x -> * Q (separable, 1/N^2 folded in) -> IFFT_x.
 template <int N, int NT>
-__device__ __forceinline__ void column_pass(const SiteCtx& c, Smem<NT>& sm) {
+__device__ __forceinline__ void column_pass(const SiteCtx& c, int tid, Smem<NT>& sm) {
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
-  const int tid = threadIdx.x;
   const bool c1 = tid < L * N1;
   const int lc = tid % L, nc = tid / L;
   const bool p2 = tid < L * N2;
   const int l = tid % L, k2 = tid / L;
+  constexpr bool prefetch = kPrefetchCols && N <= 256;  // N = 512: 32-point lines, no room
   float2 cpre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
 #pragma unroll 1
   for (int t = 0; t < N / L; ++t) {
     const int col0 = t * L;
     if (c1) {
       float2 v[N2];
-      if (kPrefetchCols && t > 0) {
+      if (prefetch && t > 0) {
 #pragma unroll
         for (int n2 = 0; n2 < N2; ++n2) v[n2] = cpre[n2];
       } else {
@@ -300,7 +299,7 @@ __device__ __forceinline__ void column_pass(const SiteCtx& c, Smem<NT>& sm) {
 #pragma unroll
       for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
     }
-    if (kPrefetchCols && c1 && t + 1 < N / L) {
+    if (prefetch && c1 && t + 1 < N / L) {
       const float2* src = c.fld + (int64_t)nc * N + col0 + L + lc;
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
@@ -328,9 +327,13 @@ __device__ __forceinline__ void column_pass(const SiteCtx& c, Smem<NT>& sm) {
 }
 
 template <int N, int NT, int MODE>
-__device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
+__device__ __forceinline__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
   constexpr int C = ClassIdx<N>::v;
-  const int tid = threadIdx.x;
+  // The thread index is made opaque once per site: values derived from it are then computed
+  // inside this site's code path instead of being hoisted above the persistent site loop,
+  // where the paths of every field side of a mixed launch would all hold registers at once.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   for (int i = tid; i < N; i += NT) {
     sm.tw[i] = a.twid[a.ctw[C] + i];
     sm.q[i] = a.q1d[a.cq[C] + i];
@@ -343,15 +346,15 @@ __device__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
                   a.mask + a.mask_off[site], a.scratch + (int64_t)blockIdx.x * a.scratch_stride};
   double esum = 0.0;
   if (S < 1) {
-    row_pass<N, NT, MODE, kOnly>(a, c, 0, sm, esum);
+    row_pass<N, NT, MODE, kOnly>(a, c, 0, tid, sm, esum);
   } else {
-    row_pass<N, NT, MODE, kFirst>(a, c, 0, sm, esum);
-    column_pass<N, NT>(c, sm);
+    row_pass<N, NT, MODE, kFirst>(a, c, 0, tid, sm, esum);
+    column_pass<N, NT>(c, tid, sm);
     for (int s = 1; s < S; ++s) {
-      row_pass<N, NT, MODE, kMiddle>(a, c, s, sm, esum);
-      column_pass<N, NT>(c, sm);
+      row_pass<N, NT, MODE, kMiddle>(a, c, s, tid, sm, esum);
+      column_pass<N, NT>(c, tid, sm);
     }
-    row_pass<N, NT, MODE, kLast>(a, c, S, sm, esum);
+    row_pass<N, NT, MODE, kLast>(a, c, S, tid, sm, esum);
   }
   const double tot = block_sum_double(esum, sm.red);
   if (tid == 0) a.energy[site] = tot;
