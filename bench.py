@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "mixed"))
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run three device copies of GIB gibibytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -107,6 +109,19 @@ def main():
     import torch
     from maomi_amd.plain import PlainStage
     from maomi_amd.workloads import CONFIG1, CONFIG2, index_volume, site_offsets, zernike_weights
+
+    if args.copy_probe > 0:
+        n = int(args.copy_probe * (1 << 30)) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for i in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            print(f"[bench] copy {i}: {4 * n} bytes in {time.perf_counter() - t0:.6f} s",
+                  file=sys.stderr, flush=True)
+        return
 
     if args.config == 3:
         return bench_uneven(args)

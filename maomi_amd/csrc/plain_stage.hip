@@ -385,21 +385,49 @@ __global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_mixed_kernel(S
 }
 
 // ---------------------------------------------------------------- phase prep
-__global__ void phase_prepare_kernel(const float* __restrict__ index, int64_t X, int64_t Y,
-                                     int64_t Zv, int64_t vps, double n0, double k0dz,
-                                     float* __restrict__ phase32, float* __restrict__ slice_max) {
-  const int64_t Z = Zv / vps;
-  const int64_t tot = X * Y;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const float* col = index + p * Zv;
-    for (int64_t z = 0; z < Z; ++z) {
-      double acc = 0.0;
-      for (int64_t v = 0; v < vps; ++v) acc += (double)col[z * vps + v] - n0;
-      const float ph = (float)(k0dz * acc);
-      phase32[z * tot + p] = ph;
-      if (slice_max) atomicMax((int*)&slice_max[z], __float_as_int(fabsf(ph)));
+// Screens from the index volume, FP64 accumulation over the vps voxels of each slab.
+// A block owns `cols` consecutive (x, y) columns; their voxels are one contiguous range of
+// cols * Zv floats, staged through LDS with coalesced loads (stride Zv + 1 against bank
+// conflicts).  Each thread then sums its column slab by slab, and the block reduces every
+// slice's max |phi| before a single atomic per slice.
+constexpr int kPrepThreads = 256;
+constexpr int kPrepLds = 16384;  // floats of staged voxels per block (64 KiB)
+
+__global__ __launch_bounds__(kPrepThreads) void phase_prepare_kernel(
+    const float* __restrict__ index, int64_t tot, int Zv, int vps, int cols, double n0,
+    double k0dz, float* __restrict__ phase32, float* __restrict__ slice_max) {
+  extern __shared__ float stage[];
+  __shared__ float wmax[kPrepThreads / 64];
+  const int Z = Zv / vps;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int64_t p0 = (int64_t)blockIdx.x * cols; p0 < tot; p0 += (int64_t)gridDim.x * cols) {
+    const int nc = (int)(tot - p0 < cols ? tot - p0 : cols);
+    const float* src = index + p0 * Zv;
+    for (int e = tid; e < nc * Zv; e += kPrepThreads) stage[(e / Zv) * (Zv + 1) + e % Zv] = src[e];
+    __syncthreads();
+    for (int z = 0; z < Z; ++z) {
+      float mag = 0.f;
+      if (tid < nc) {
+        double acc = 0.0;
+        const float* col = stage + tid * (Zv + 1) + z * vps;
+        for (int v = 0; v < vps; ++v) acc += (double)col[v] - n0;
+        const float ph = (float)(k0dz * acc);
+        phase32[(int64_t)z * tot + p0 + tid] = ph;
+        mag = fabsf(ph);
+      }
+      if (slice_max) {  // uniform branch
+        for (int off = 32; off > 0; off >>= 1) mag = fmaxf(mag, __shfl_down(mag, off, 64));
+        if (lane == 0) wmax[wave] = mag;
+        __syncthreads();
+        if (tid == 0) {
+          float m = wmax[0];
+          for (int w = 1; w < kPrepThreads / 64; ++w) m = fmaxf(m, wmax[w]);
+          atomicMax((int*)&slice_max[z], __float_as_int(m));
+        }
+        __syncthreads();
+      }
     }
+    __syncthreads();  // the stage buffer is refilled for the next column block
   }
 }
 
@@ -546,9 +574,17 @@ MAOMI_API int maomi_phase_prepare(const float* index, int64_t X, int64_t Y, int6
     MAOMI_CHECK_LAUNCH();
   }
   const int64_t tot = X * Y;
-  const unsigned blocks = (unsigned)((tot + 255) / 256 < 65535 ? (tot + 255) / 256 : 65535);
-  hipLaunchKernelGGL(phase_prepare_kernel, dim3(blocks), dim3(256), 0, st, index, X, Y, Zv, vps,
-                     n0, k0dz, phase32, slice_max);
+  if (Zv > kPrepLds - 1) {
+    maomi_set_error("maomi_phase_prepare: more than 16383 voxels per column");
+    return MAOMI_EUNSUPPORTED;
+  }
+  // columns per block: as many as the 64 KiB stage holds, at most one per thread
+  const int cols = (int)(kPrepLds / (Zv + 1) < kPrepThreads ? kPrepLds / (Zv + 1) : kPrepThreads);
+  const int64_t nblk = (tot + cols - 1) / cols;
+  const unsigned blocks = (unsigned)(nblk < 65535 ? nblk : 65535);
+  hipLaunchKernelGGL(phase_prepare_kernel, dim3(blocks), dim3(kPrepThreads),
+                     (size_t)cols * (Zv + 1) * sizeof(float), st, index, tot, (int)Zv, (int)vps,
+                     cols, n0, k0dz, phase32, slice_max);
   MAOMI_CHECK_LAUNCH();
   if (phase16) {
     const int64_t n = tot * Z;

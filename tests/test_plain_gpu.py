@@ -75,3 +75,27 @@ def test_mixed_precision_against_fp64_oracle(gpu_lib):
     # FP16 screens carry |phi| <= max|phi_z| * 2^-12 per sample; after 16 slices and the
     # fourth power this stays below 2e-3.
     assert err < 2e-3
+
+
+@pytest.mark.parametrize("X,Y,Zv,vps", [(37, 29, 12, 3), (64, 70, 60, 1), (9, 11, 300, 2)])
+def test_phase_prepare_matches_numpy(gpu_lib, X, Y, Zv, vps):
+    """Screens phi[z] = float32(k0dz * sum_v (n - n0)) with the sum in FP64, and each slice's
+    max |phi|; the shapes cover several voxels per slab, more columns than one block stages
+    and ragged last blocks."""
+    import torch
+    from maomi_amd import _native as nat
+    rng = np.random.default_rng(X + Zv)
+    vol = (1.35 + 0.02 * rng.standard_normal((X, Y, Zv))).astype(np.float32)
+    n0, k0dz = 1.35, 2 * np.pi / 0.92 * 2.0
+    Z = Zv // vps
+    want = np.float32(k0dz * (vol.astype(np.float64) - n0).reshape(X, Y, Z, vps).sum(-1))
+    want = np.moveaxis(want, -1, 0)
+    d = torch.as_tensor(vol, device="cuda")
+    ph = torch.empty((Z, X, Y), dtype=torch.float32, device="cuda")
+    smax = torch.empty(Z, dtype=torch.float32, device="cuda")
+    nat.check(nat.lib().maomi_phase_prepare(nat.ptr(d), X, Y, Zv, vps, n0, k0dz, nat.ptr(ph),
+                                            None, None, nat.ptr(smax), nat.stream_handle(d.device)),
+              "phase prepare")
+    got = ph.cpu().numpy()
+    np.testing.assert_allclose(got, want, rtol=2e-7, atol=1e-7)
+    np.testing.assert_array_equal(smax.cpu().numpy(), np.abs(got).reshape(Z, -1).max(1))
