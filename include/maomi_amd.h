@@ -270,6 +270,33 @@ int maomi_patch_features(const float* index, int64_t X, int64_t Y, int64_t Zv, c
                          int64_t pool, int64_t zbins, double n0, double inv_std, float* out,
                          maomi_stream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Local-PSF FFT plan (replaces the array operations of LocalFFTPlan.__call__,
+ * maomi/rana/_fft.py:76-95, and the torch backend it runs on, maomi/rana/_backends.py).
+ * FP64 real arrays, complex128 spectra; the FFTs are the caller's.
+ * ------------------------------------------------------------------------- */
+
+/* per tile: source window start x, y, size nx, ny, weights offset (-1: unweighted) */
+#define MAOMI_LFFT_GATHER_DESC 5
+/* per tile: target start x, y, size tx, ty, response start x, y, z, weights offset (-1) */
+#define MAOMI_LFFT_SCATTER_DESC 8
+
+/* tiles[b] (Lx x Ly x Lz, zero padded) = source[X][Y][Z] window of tile b times its
+ * nx x ny weights (rana/_fft.py:84-86 plus the zero padding rfftn would add) */
+int maomi_lfft_gather(const double* source, int64_t X, int64_t Y, int64_t Z, const int32_t* desc,
+                      const double* weights, double* tiles, int64_t B, int64_t Lx, int64_t Ly,
+                      int64_t Lz, maomi_stream_t stream);
+/* out == NULL: spec[i] *= kspec[i] over n complex values; otherwise
+ * out[i] = sum_z spec[i][z] kspec[i][z] with z innermost of length nz (plane projection,
+ * rana/_fft.py:88-90) */
+int maomi_lfft_product(void* spec, const void* kspec, void* out, int64_t n, int64_t nz,
+                       maomi_stream_t stream);
+/* out[target of b] += full[b][response of b] times the tile's tx x ty weights, atomically;
+ * out is X x Y x Zo (Zo = 1 for a plane), full is B x Lx x Ly x Lz (rana/_fft.py:92-94) */
+int maomi_lfft_scatter(const double* full, int64_t B, int64_t Lx, int64_t Ly, int64_t Lz,
+                       const int32_t* desc, const double* weights, double* out, int64_t X,
+                       int64_t Y, int64_t Zo, int64_t Tx, int64_t Ty, maomi_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@ SOURCES = {
     "v4.hip": ["-ffp-contract=off"],
     "segmented_stage.hip": [],
     "surrogate.hip": [],
+    "local_fft.hip": [],
 }
 
 
