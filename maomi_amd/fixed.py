@@ -13,6 +13,8 @@ import time
 import numpy as np
 import torch
 
+from . import _native as nat
+
 from .optics import collection_mask, prepare_fields, pupil, _shapes
 
 
@@ -100,16 +102,45 @@ def assemble(config, out, shape, *, mask_mode="matlab_v4", z_positions=None, cut
 
 
 def zstack(volume, psf, *, mask=None, collection_mask=None, device="cuda"):
-    """MATLAB convn(volume * masks, psf, 'same') by FFT with the even-kernel crop (scanning.py:8-53)."""
+    """MATLAB convn(volume * mask * collection_mask, psf, 'same') with zero extension and the
+    MATLAB crop floor(k/2) on every axis (scanning.py:8-53, mode matlab_convn), in FP64.
+
+    One tile covering the volume through the local-FFT kernels: maomi_lfft_gather applies the
+    masks while writing the zero-padded volume, maomi_lfft_product multiplies the spectra and
+    maomi_lfft_scatter writes the 'same' window; the rFFTs are rocFFT's.  Returns a float32
+    tensor on `device`, as the reference does for float32 input."""
     from scipy.fft import next_fast_len
-    v = torch.as_tensor(volume, dtype=torch.float32, device=device)
-    p = torch.as_tensor(psf, dtype=torch.float32, device=device)
+    dev = torch.device(device)
+    v = torch.as_tensor(np.asarray(volume, np.float64), device=dev).contiguous()
+    p = np.asarray(psf, np.float64)
+    if v.ndim != 3 or p.ndim != 3:
+        raise ValueError("Expected X,Y,Z arrays")
+    X, Y, Z = v.shape
+    w = np.ones((X, Y))
     for m in (mask, collection_mask):
         if m is not None:
-            v = v * torch.as_tensor(m, dtype=v.dtype, device=device)[..., None]
+            m = np.asarray(m, np.float64)
+            if m.shape != (X, Y):
+                raise ValueError("Mask must match volume XY")
+            w = w * m
     n = tuple(next_fast_len(a + b - 1) for a, b in zip(v.shape, p.shape))
-    full = torch.fft.irfftn(torch.fft.rfftn(v, s=n) * torch.fft.rfftn(p, s=n), s=n)
-    return full[tuple(slice(b // 2, b // 2 + a) for a, b in zip(v.shape, p.shape))].contiguous()
+    lib, st = nat.lib(), nat.stream_handle(dev)
+    wd = torch.as_tensor(w.ravel(), device=dev)
+    gd = torch.as_tensor(np.array([[0, 0, X, Y, 0]], np.int32), device=dev)
+    tiles = torch.empty((1,) + n, dtype=torch.float64, device=dev)
+    nat.check(lib.maomi_lfft_gather(nat.ptr(v), X, Y, Z, nat.ptr(gd), nat.ptr(wd), nat.ptr(tiles),
+                                    1, n[0], n[1], n[2], st), "zstack gather")
+    spec = torch.fft.rfftn(tiles, dim=(1, 2, 3)).contiguous()
+    kspec = torch.fft.rfftn(torch.as_tensor(p, device=dev)[None], s=n, dim=(1, 2, 3)).contiguous()
+    nat.check(lib.maomi_lfft_product(nat.ptr(spec), nat.ptr(kspec), None, spec.numel(), 0, st),
+              "zstack product")
+    full = torch.fft.irfftn(spec, s=n, dim=(1, 2, 3)).contiguous()
+    out = torch.zeros((X, Y, Z), dtype=torch.float64, device=dev)
+    c = [b // 2 for b in p.shape]
+    sd = torch.as_tensor(np.array([[0, 0, X, Y, c[0], c[1], c[2], -1]], np.int32), device=dev)
+    nat.check(lib.maomi_lfft_scatter(nat.ptr(full), 1, n[0], n[1], n[2], nat.ptr(sd), None,
+                                     nat.ptr(out), X, Y, Z, X, Y, st), "zstack crop")
+    return out.float()
 
 
 def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, custom_phase=None,

@@ -28,7 +28,7 @@ __all__ = [
     "zrf", "zidx", "zernike", "generate_ba", "pupil", "cubic", "resize_xy",
     "translate_xy", "fresnel", "propagate", "v4_interpolate", "assemble",
     "fixed_locations", "locations", "field_shapes", "base_field", "plain_stage_masks",
-    "resample_matrix", "set_workers", "plain_phase", "plain_beam", "plain_stage",
+    "resample_matrix", "set_workers", "plain_phase", "plain_beam", "plain_stage", "zstack",
 ]
 
 _WORKERS = 1
@@ -783,3 +783,16 @@ def plain_stage(cfg, index_volume, offsets, zernike_weights=(), dtype=np.complex
         m = v4_interpolate(offs.astype(np.float64), energies, q)
         out.update(mask_raw=m, mask_mean=float(m.mean()), mask=m / m.mean())
     return out
+
+
+def zstack(volume, psf, mask=None, collection_mask=None):
+    """scanning.py:8-53 (matlab_convn): convn(volume * masks, psf, 'same') with the MATLAB crop
+    floor(k/2), zero extension, FP64.  Pinned by a direct sum in tests/test_oracle.py."""
+    from scipy.signal import fftconvolve
+    v = np.asarray(volume, np.float64)
+    for m in (mask, collection_mask):
+        if m is not None:
+            v = v * np.asarray(m, np.float64)[..., None]
+    p = np.asarray(psf, np.float64)
+    full = fftconvolve(v, p, mode="full")
+    return full[tuple(slice(b // 2, b // 2 + a) for a, b in zip(v.shape, p.shape))]

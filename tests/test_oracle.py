@@ -301,3 +301,21 @@ def test_fp64_oracle_close_to_fp32_oracle(golden):
     # at float32 resolution; that alone costs ~1e-4 in psf relative L2 against FP64.
     assert rel_l2(a["psf_3d"], b["psf_3d"]) < 3e-4
     assert rel_l2(a["energies"], b["energies"]) < 3e-5
+
+
+def test_zstack_equals_direct_convolution_with_matlab_crop():
+    rng = np.random.default_rng(12)
+    v = rng.random((7, 6, 5))
+    for kshape in ((3, 3, 3), (4, 2, 5), (1, 4, 2)):
+        p = rng.random(kshape)
+        m, c = rng.random((7, 6)), rng.random((7, 6))
+        got = O.zstack(v, p, m, c)
+        src = v * (m * c)[..., None]
+        want = np.zeros_like(v)
+        cx, cy, cz = (k // 2 for k in kshape)
+        for x, y, z in np.ndindex(*v.shape):
+            for a, b, d in np.ndindex(*kshape):
+                qx, qy, qz = x + cx - a, y + cy - b, z + cz - d
+                if 0 <= qx < 7 and 0 <= qy < 6 and 0 <= qz < 5:
+                    want[x, y, z] += src[qx, qy, qz] * p[a, b, d]
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)

@@ -50,3 +50,39 @@ def test_even_kernel_offsets_and_single_node_support():
         a = L.local_fft(src.shape, kernels, w, src, plane=False, detection=det, correlate=corr)
         b = L.local_direct(src.shape, kernels, w, src, plane=False, detection=det, correlate=corr)
         np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-11)
+
+
+def _uniform(op, kshape=(5, 3, 5), shape=(12, 10, 5), seed=30, depths=None, native=False):
+    from oracle.rana_np import Operator
+    rng = np.random.default_rng(seed)
+    k = rng.random(kshape)
+    kern = np.broadcast_to(k, (3, 2) + kshape).copy()
+    kl = [kern] if depths is None else [kern, kern.copy()]
+    o = Operator(kl, [0.0, 3.0, 6.0], [0.0, 5.0], (0.5, 0.5, 1.0), op, depths_um=depths,
+                 native=native)
+    return o, k, rng.random(shape)
+
+
+@pytest.mark.parametrize("op,native", [("detection", False), ("excitation", False),
+                                       ("excitation", True)])
+def test_operator_with_one_kernel_is_matlab_convn(op, native):
+    import oracle as O
+    o, k, vol = _uniform(op, native=native)
+    kk = np.flip(k) if (op == "excitation" and not native) else k
+    np.testing.assert_allclose(o.static(vol), np.maximum(O.zstack(vol, kk), 0), rtol=1e-11,
+                               atol=1e-12)
+    # a plane projection pairs kernel z with source z: detection reverses the kernel depth
+    # first (a convolution in z), excitation correlates, native pairs them directly, which is
+    # the centre plane of a convolution with the kernel flipped in z
+    c = k.shape[2] // 2
+    ks = k[:, :, ::-1] if native else kk
+    np.testing.assert_allclose(o.scan_slab(vol), np.maximum(O.zstack(vol, ks)[:, :, c], 0),
+                               rtol=1e-11, atol=1e-12)
+
+
+def test_depth_bank_of_equal_fields_is_the_field():
+    o1, _, vol = _uniform("excitation")
+    o2, _, _ = _uniform("excitation", depths=[0.0, 10.0])
+    np.testing.assert_allclose(o2.static(vol), o1.static(vol), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(o2.scan_slab(vol, z_start=1), o1.scan_slab(vol, z_start=1),
+                               rtol=1e-12, atol=1e-13)

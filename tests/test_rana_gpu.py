@@ -1,0 +1,77 @@
+"""GPU acquisition operator and adapter against the CPU restatement (pinned by closed forms)."""
+
+import numpy as np
+import pytest
+
+from oracle.rana_np import Operator
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(op, seed, depths=None, kshape=(5, 4, 5)):
+    from maomi_amd.fields import DepthPSFBank, FullFieldPSF
+    rng = np.random.default_rng(seed)
+    xs, ys = [0.0, 4.0, 9.0], [-1.0, 3.0, 6.0]
+    ks = [rng.random((3, 3) + kshape) + 0.05 for _ in range(1 if depths is None else len(depths))]
+    fs = [FullFieldPSF(k, xs, ys, (0.5, 0.5, 1.0), 920, op, {"n": i}) for i, k in enumerate(ks)]
+    field = fs[0] if depths is None else DepthPSFBank(depths, tuple(fs))
+    return field, Operator(ks, xs, ys, (0.5, 0.5, 1.0), op, depths_um=depths)
+
+
+@pytest.mark.parametrize("op,depths", [("excitation", None), ("detection", None),
+                                       ("excitation", [0.0, 2.0, 6.0]),
+                                       ("detection", [0.0, 6.0])])
+def test_static_and_scans_match_oracle(gpu_lib, op, depths):
+    from maomi_amd.rana import RanaOperator
+    field, ref = _field(op, 3, depths)
+    rng = np.random.default_rng(4)
+    vol = rng.random((16, 12, 5))
+    print(f"[rana] {op} depths={depths}", flush=True)
+    gpu = RanaOperator(field, device="cuda")
+    assert rel_l2(gpu.static(vol), ref.static(vol)) < 1e-12
+    for z0, avg in ((0, 1), (1, 2), (0, 3)):
+        assert rel_l2(gpu.scan_slab(vol, z_start=z0, scan_avg=avg),
+                      ref.scan_slab(vol, z_start=z0, scan_avg=avg)) < 1e-12
+    info = gpu.cache_info
+    assert info["plans"] >= 1 and info["kernel_fft_builds"] > 0
+    gpu.clear_cache()
+    assert gpu.cache_info["plans"] == 0
+
+
+def test_operator_rejections(gpu_lib):
+    from maomi_amd.rana import RanaOperator
+    field, _ = _field("excitation", 5)
+    gpu = RanaOperator(field, device="cuda")
+    with pytest.raises(ValueError, match="cover"):
+        gpu.static(np.ones((30, 12, 5)))
+    with pytest.raises(ValueError, match="nonnegative"):
+        gpu.static(-np.ones((16, 12, 5)))
+    with pytest.raises(ValueError, match="slab depth"):
+        gpu.scan_slab(np.ones((16, 12, 4)))
+    with pytest.raises(ValueError, match="scan_avg"):
+        gpu.scan_slab(np.ones((16, 12, 5)), scan_avg=0)
+    with pytest.raises(ValueError, match="origins"):
+        RanaOperator(field, origin_um=(np.nan, 0.0), device="cuda")
+    bank, _ = _field("excitation", 6, depths=[1.0, 3.0])
+    with pytest.raises(ValueError, match="depth bank"):
+        RanaOperator(bank, device="cuda").static(np.ones((16, 12, 5)))
+
+
+def test_attach_native_field_renders_image(gpu_lib):
+    from maomi_amd.rana import RanaAddon, attach_rana
+    rng = np.random.default_rng(8)
+    pts = np.array([[x, y] for x in (0.0, 4.0, 9.0) for y in (-1.0, 3.0, 6.0)])
+    psf3 = rng.random((9, 5, 4, 5)) + 0.05
+    mask, col = rng.random((16, 12)) + 0.5, rng.random((16, 12)) + 0.5
+    optics = {"points_um": pts, "psf_3d": psf3, "psf": psf3.mean(0), "mask": mask,
+              "colmask": col, "settings": {}}
+    vol = rng.random((16, 12, 5))
+    out = attach_rana(optics, RanaAddon(voxel_size_um=(0.5, 0.5, 1.0)), volume=vol, device="cuda")
+    assert out["rana"].settings["operator"] == "native_scan"
+    # native kernels scaled to the mean PSF energy, rendered as a native (convolution) operator
+    k = psf3 / psf3.sum(axis=(1, 2, 3))[:, None, None, None] * psf3.mean(0).sum()
+    ref = Operator([k.reshape(3, 3, 5, 4, 5)], [0.0, 4.0, 9.0], [-1.0, 3.0, 6.0],
+                   (0.5, 0.5, 1.0), "excitation", native=True)
+    assert rel_l2(out["image"], ref.static(vol * (mask * col)[..., None])) < 1e-12
+    assert attach_rana(optics, False) is optics
