@@ -150,7 +150,24 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
     """MATLAB v4 masks and arithmetic PSF averaging on a count x count lattice (fixed.py:193-411)."""
     from .batched import OpticalPlan
     if rana is not False and rana is not None:
-        raise NotImplementedError("RANA acquisition is outside this package's optical stage")
+        # native run without an image, then the acquisition operator (fixed.py:219-232); the
+        # reference's status window has no counterpart here
+        from .rana import attach_rana
+        overall = time.perf_counter()
+        native = run_fixed(config, volume, vessels=vessels, raw_fields=raw_fields,
+                           scatter=scatter, custom_phase=custom_phase, phase_units=phase_units,
+                           phase_aperture=phase_aperture, input_pupil=input_pupil,
+                           beam_weights=beam_weights, tails=tails, batch_size=batch_size,
+                           count_per_axis=count_per_axis, mask_mode=mask_mode, device=device,
+                           rana=False, render_image=False)
+        start = time.perf_counter()
+        result = attach_rana(native, rana, config=config,
+                             volume=volume if render_image else None, device=device)
+        result["timing_seconds"] = dict(native.get("timing_seconds", {}),
+                                        rana=time.perf_counter() - start)
+        result["native_total_seconds"] = native["total_seconds"]
+        result["total_seconds"] = time.perf_counter() - overall
+        return result
     if mask_mode != "matlab_v4":
         raise ValueError("run_fixed requires matlab_v4; use maomi.enhanced.CubicFullFieldRenderer "
                          "with a complete custom FullFieldPSF for cubic imaging")
