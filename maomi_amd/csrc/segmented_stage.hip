@@ -12,7 +12,11 @@
 // (maomi/optics.py:318-340).  Field sides are arbitrary even sizes: every line
 // transform is a two-level direct DFT (N = N1 N2) run from LDS.
 #include "common.hpp"
+#include "fft_reg.hpp"
 #include "maomi_amd.h"
+
+// Dynamic LDS of the segmented kernel; the line transforms address it by offset.
+extern __shared__ float2 seg_lds[];
 
 namespace {
 
@@ -89,6 +93,125 @@ __device__ void line_dft(float2* A, float2* B, int nl, int N, int N1, int N2, co
   __syncthreads();
 }
 
+// Fast line transform for N = R1 * R2 with R2 the power-of-two part of N (at most 16) and
+// R1 <= 64: stage 1 is an R2-point register FFT per (line, n1) job followed by the twiddle
+// W_N^(n1 k2); stage 2 an R1-point direct DFT per (line, k2) job whose R1 twiddles are held
+// in registers (R1 <= 16) or per output from LDS (wide_stage2).  X[k2 + R2 k1] lands in
+// natural order.  Lines live at LDS offsets a_off
+// (in and out) and b_off (scratch); the twiddle table W_N^j at t_off.
+template <int R2, bool INV>
+__device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl, int N) {
+  const int R1 = N / R2;
+  const float2* T = seg_lds + t_off;
+  for (int job = threadIdx.x; job < nl * R1; job += NT) {
+    const int l = job / R1, n1 = job - l * R1;
+    const float2* a = seg_lds + a_off + l * N + n1;
+    float2 v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = a[R1 * m];
+    fftk::fft<R2, INV>(v);
+    float2* b = seg_lds + b_off + l * N + n1 * R2;
+    b[0] = v[0];
+    int idx = 0;  // n1 k2 mod N
+#pragma unroll
+    for (int k2 = 1; k2 < R2; ++k2) {
+      idx += n1;
+      if (idx >= N) idx -= N;
+      b[k2] = cmul(v[k2], twid(T, idx, INV));
+    }
+  }
+}
+
+template <int R1, bool INV>
+__device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl, int N) {
+  const int R2 = N / R1;
+  float2 w[R1];  // W_R1^c = W_N^(c R2)
+#pragma unroll
+  for (int c = 0; c < R1; ++c) w[c] = twid(seg_lds + t_off, c * R2, INV);
+  for (int job = threadIdx.x; job < nl * R2; job += NT) {
+    const int l = job / R2, k2 = job - l * R2;
+    const float2* b = seg_lds + b_off + l * N + k2;
+    float2 x[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) x[m] = b[m * R2];
+    float2* a = seg_lds + a_off + l * N + k2;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      float2 acc = x[0];
+#pragma unroll
+      for (int m = 1; m < R1; ++m) acc = cadd(acc, cmul(x[m], w[(m * k1) % R1]));
+      a[k1 * R2] = acc;
+    }
+  }
+}
+
+// Stage 2 for R1 > 16: one output X[k2 + R2 k1] per work item, inputs and twiddles W_R1^(m k1)
+// read from LDS with the exponent advanced incrementally.
+template <bool INV>
+__device__ __noinline__ void wide_stage2(int a_off, int b_off, int t_off, int nl, int N, int R1) {
+  const int R2 = N / R1;
+  const float2* T = seg_lds + t_off;
+  for (int it = threadIdx.x; it < nl * N; it += NT) {
+    const int l = it / N, k = it - l * N, k1 = k / R2, k2 = k - k1 * R2;
+    const float2* b = seg_lds + b_off + l * N + k2;
+    float2 acc = b[0];
+    int e = 0;  // m k1 mod R1
+    for (int m = 1; m < R1; ++m) {
+      e += k1;
+      if (e >= R1) e -= R1;
+      acc = cadd(acc, cmul(b[m * R2], twid(T, e * R2, INV)));
+    }
+    seg_lds[a_off + l * N + k] = acc;
+  }
+}
+
+#ifndef MAOMI_SEG_FAST
+#define MAOMI_SEG_FAST 1
+#endif
+
+// R2 of the fast split of N (0: no fast split, the generic two-level DFT runs).
+__device__ __forceinline__ int fast_r2(int N) {
+  if (!MAOMI_SEG_FAST) return 0;
+  int r2 = 1;
+  while (r2 < 16 && N % (2 * r2) == 0) r2 *= 2;  // 16 points keep the kernel at 2 waves/SIMD
+  return (r2 >= 2 && N / r2 <= 64) ? r2 : 0;
+}
+
+template <bool INV>
+__device__ void fast_line_transform(int a_off, int b_off, int t_off, int nl, int N, int r2) {
+  switch (r2) {
+    case 2: fast_stage1<2, INV>(a_off, b_off, t_off, nl, N); break;
+    case 4: fast_stage1<4, INV>(a_off, b_off, t_off, nl, N); break;
+    case 8: fast_stage1<8, INV>(a_off, b_off, t_off, nl, N); break;
+    default: fast_stage1<16, INV>(a_off, b_off, t_off, nl, N); break;
+  }
+  __syncthreads();
+  switch (N / r2) {
+#define MAOMI_R1(R) case R: fast_stage2<R, INV>(a_off, b_off, t_off, nl, N); break;
+    MAOMI_R1(1) MAOMI_R1(2) MAOMI_R1(3) MAOMI_R1(4) MAOMI_R1(5) MAOMI_R1(6) MAOMI_R1(7)
+    MAOMI_R1(8) MAOMI_R1(9) MAOMI_R1(10) MAOMI_R1(11) MAOMI_R1(12) MAOMI_R1(13) MAOMI_R1(14)
+    MAOMI_R1(15) MAOMI_R1(16)
+#undef MAOMI_R1
+    default: wide_stage2<INV>(a_off, b_off, t_off, nl, N, N / r2); break;
+  }
+  __syncthreads();
+}
+
+// Transform nl lines of A through B with the fast split when N has one, else line_dft.
+__device__ __forceinline__ void line_transform(float2* A, float2* B, int nl, int N, int N1,
+                                               int N2, const float2* T, bool inv) {
+  const int r2 = fast_r2(N);
+  if (r2) {
+    const int a_off = (int)(A - seg_lds), b_off = (int)(B - seg_lds), t_off = (int)(T - seg_lds);
+    if (inv)
+      fast_line_transform<true>(a_off, b_off, t_off, nl, N, r2);
+    else
+      fast_line_transform<false>(a_off, b_off, t_off, nl, N, r2);
+  } else {
+    line_dft(A, B, nl, N, N1, N2, T, inv);
+  }
+}
+
 // out[x][y] = scale * sum_i mx[x][i] sum_j my[y][j] in[i][j]  (in: R x C, tmp: X x C)
 __device__ void resample2d(const float* in, int R, int C, const float* mx, int X, const float* my,
                            int Y, float* tmp, float* out, int64_t ostride, float scale) {
@@ -153,7 +276,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
       } else {
         for (int it = threadIdx.x; it < nl * N; it += NT) A[it] = fld[(int64_t)r0 * N + it];
         __syncthreads();
-        line_dft(A, B, nl, N, N1, N2, T, true);
+        line_transform(A, B, nl, N, N1, N2, T, true);
       }
       for (int it = threadIdx.x; it < nl * N; it += NT) {
         const int l = it / N, y = it % N, x = r0 + l;
@@ -183,7 +306,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
       }
       __syncthreads();
       if (!last) {
-        line_dft(A, B, nl, N, N1, N2, T, false);
+        line_transform(A, B, nl, N, N1, N2, T, false);
         for (int it = threadIdx.x; it < nl * N; it += NT) fld[(int64_t)r0 * N + it] = A[it];
         __syncthreads();
       }
@@ -208,13 +331,13 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         A[l * N + i] = fld[(int64_t)i * N + c0 + l];
       }
       __syncthreads();
-      line_dft(A, B, nl, N, N1, N2, T, false);
+      line_transform(A, B, nl, N, N1, N2, T, false);
       for (int it = threadIdx.x; it < nl * N; it += NT) {
         const int l = it / N, kx = it % N;
         A[it] = cmul(A[it], cmul(Qt[kx], Qt[c0 + l]));
       }
       __syncthreads();
-      line_dft(A, B, nl, N, N1, N2, T, true);
+      line_transform(A, B, nl, N, N1, N2, T, true);
       for (int it = threadIdx.x; it < nl * N; it += NT) {
         const int i = it / nl, l = it % nl;
         fld[(int64_t)i * N + c0 + l] = A[l * N + i];
@@ -230,10 +353,10 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
   }
 }
 
-__global__ __launch_bounds__(NT) void segmented_sites_kernel(SegArgs a) {
-  extern __shared__ float2 sh[];
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void
+segmented_sites_kernel(SegArgs a) {
   const int nmax = a.n;
-  float2* A = sh;
+  float2* A = seg_lds;
   float2* B = A + a.L * nmax;
   float2* T = B + a.L * nmax;
   float2* Qt = T + nmax;
