@@ -148,6 +148,11 @@ typedef struct maomi_segmented_desc {
 } maomi_segmented_desc;
 
 int maomi_segmented_sites(const maomi_segmented_desc* desc, maomi_stream_t stream);
+/* Statistics of the per-site focal PSFs psf[P][K] for mask assembly (fixed.py:152-189, in FP64):
+ * energy[p] = sum_k psf[p][k]; sum_abs[k], sum[k], sum_neg[k] = sums over sites of |x|, x and
+ * min(x, 0) (the mean PSFs and the negative resampling mass). */
+int maomi_psf_stats(const float* psf, int64_t P, int64_t K, double* energy, double* sum_abs,
+                    double* sum, double* sum_neg, maomi_stream_t stream);
 /* complex64 elements of workspace per persistent block; TT is the tail output side. */
 int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy, int64_t X,
                                         int64_t TT);

@@ -31,6 +31,22 @@ def _split(n):
     return best, n // best
 
 
+def psf_stats(psf):
+    """FP64 per-site energies and per-voxel sums over sites of |x|, x, min(x, 0) of device PSFs
+    [P][...] (one pass on the GPU; assemble uses them instead of host reductions)."""
+    P = psf.shape[0]
+    K = psf[0].numel()
+    d = psf.device
+    energy = torch.empty(P, dtype=torch.float64, device=d)
+    sums = torch.empty((3, K), dtype=torch.float64, device=d)
+    flat = psf.contiguous()
+    nat.check(nat.lib().maomi_psf_stats(nat.ptr(flat), P, K, nat.ptr(energy), nat.ptr(sums[0]),
+                                        nat.ptr(sums[1]), nat.ptr(sums[2]),
+                                        nat.stream_handle(d)), "psf stats")
+    s = sums.cpu().numpy()
+    return {"energies": energy.cpu().numpy(), "sum_abs": s[0], "sum": s[1], "sum_neg": s[2]}
+
+
 class OpticalPlan:
     def __init__(self, config, raw_fields, *, uin=None, device="cuda", native=True):
         if not str(device).startswith("cuda"):
@@ -235,6 +251,8 @@ class OpticalPlan:
             self._launch(ox, oy, order, P, tails, outs)
         self.paths_evaluated += P
         out = {"psfs": outs["psf"].cpu().numpy(), "points_um": pts, "tails_computed": tails}
+        if P:
+            out["psf_stats"] = psf_stats(outs["psf"])
         if tails:
             for name, pre in (("top", "top_"), ("bottom", "bot_")):
                 out[name] = {"convmask": outs[pre + "conv"].cpu().numpy(),

@@ -384,6 +384,37 @@ segmented_sites_kernel(SegArgs a) {
   }
 }
 
+// Per-site FP64 sums of psf[P][K] (one block per site) ...
+__global__ void psf_site_sum_kernel(const float* __restrict__ psf, int64_t K,
+                                    double* __restrict__ energy) {
+  __shared__ double red[NT / 64];
+  const float* row = psf + (int64_t)blockIdx.x * K;
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < K; k += NT) acc += (double)row[k];
+  const double t = block_sum_double(acc, red);
+  if (threadIdx.x == 0) energy[blockIdx.x] = t;
+}
+
+// ... and per-voxel FP64 sums over sites of |x|, x and min(x, 0) (consecutive threads read
+// consecutive voxels of one site).
+__global__ void psf_voxel_sums_kernel(const float* __restrict__ psf, int64_t P, int64_t K,
+                                      double* __restrict__ sabs, double* __restrict__ ssum,
+                                      double* __restrict__ sneg) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    double a = 0.0, v = 0.0, n = 0.0;
+    for (int64_t p = 0; p < P; ++p) {
+      const double x = psf[p * K + k];
+      a += fabs(x);
+      v += x;
+      n += x < 0.0 ? x : 0.0;
+    }
+    sabs[k] = a;
+    ssum[k] = v;
+    sneg[k] = n;
+  }
+}
+
 }  // namespace
 
 MAOMI_API int64_t maomi_segmented_workspace_elems(int64_t n, int64_t n2, int64_t fx, int64_t fy,
@@ -447,6 +478,23 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
   if (grid > d->P) grid = d->P;
   hipLaunchKernelGGL(segmented_sites_kernel, dim3((unsigned)grid), dim3(NT), shm,
                      (hipStream_t)stream, a);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_psf_stats(const float* psf, int64_t P, int64_t K, double* energy,
+                              double* sum_abs, double* sum, double* sum_neg,
+                              maomi_stream_t stream) {
+  if (!psf || !energy || !sum_abs || !sum || !sum_neg || P < 1 || K < 1 || P > (1ll << 31)) {
+    maomi_set_error("maomi_psf_stats: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  hipLaunchKernelGGL(psf_site_sum_kernel, dim3((unsigned)P), dim3(NT), 0, (hipStream_t)stream,
+                     psf, K, energy);
+  MAOMI_CHECK_LAUNCH();
+  const int64_t blocks = (K + 255) / 256;
+  hipLaunchKernelGGL(psf_voxel_sums_kernel, dim3((unsigned)(blocks < 65535 ? blocks : 65535)),
+                     dim3(256), 0, (hipStream_t)stream, psf, P, K, sum_abs, sum, sum_neg);
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

@@ -54,7 +54,8 @@ def assemble(config, out, shape, *, mask_mode="matlab_v4", z_positions=None, cut
     if mask_mode != "matlab_v4":
         raise ValueError("mask_mode must be cubic or matlab_v4")
     points, psfs = out["points_um"], out["psfs"]
-    energies = psfs.sum((1, 2, 3), dtype=np.float64)
+    stats = out.get("psf_stats")  # device-side sums from OpticalPlan.evaluate, when present
+    energies = stats["energies"] if stats is not None else psfs.sum((1, 2, 3), dtype=np.float64)
     tails = bool(out["tails_computed"])
     values = [energies] + ([out[k]["weight"] for k in ("top", "bottom")] if tails else [])
     values = np.stack(values, axis=1)
@@ -70,17 +71,25 @@ def assemble(config, out, shape, *, mask_mode="matlab_v4", z_positions=None, cut
     mean = float(mask.mean())
     if not np.isfinite(mean) or mean <= 0:
         raise ValueError("Nonpositive or invalid mean excitation energy")
-    flat = psfs.reshape(len(points), -1)
+    if stats is not None:
+        P = len(points)
+        psf_mean = (stats["sum_abs"] / P).reshape(psfs.shape[1:])
+        psf_raw = (stats["sum"] / P).reshape(psfs.shape[1:])
+        neg = float(-stats["sum_neg"].sum() / max(stats["sum_abs"].sum(), 1e-30))
+    else:
+        flat = psfs.reshape(len(points), -1)
+        psf_mean = (weights @ np.abs(flat)).reshape(psfs.shape[1:])
+        psf_raw = (weights @ flat).reshape(psfs.shape[1:])
+        neg = float(-np.minimum(psfs, 0).sum(dtype=np.float64)
+                    / max(np.abs(psfs).sum(dtype=np.float64), 1e-30))
     result = {
-        "psf": (weights @ np.abs(flat)).reshape(psfs.shape[1:]),
-        "psf_raw": (weights @ flat).reshape(psfs.shape[1:]),
+        "psf": psf_mean,
+        "psf_raw": psf_raw,
         "psf_3d": psfs, "points_um": points, "energies": energies,
         "mask_raw": mask, "mask": mask / mean, "mask_mean": mean, "mask_mode": mask_mode,
         "geometry_cache_hit": False, "tails_computed": tails,
         "unique_optical_paths": len(points),
-        "negative_resampling_mass_fraction": float(
-            -np.minimum(psfs, 0).sum(dtype=np.float64)
-            / max(np.abs(psfs).sum(dtype=np.float64), 1e-30)),
+        "negative_resampling_mass_fraction": neg,
     }
     if tails:
         for j, (native, name) in enumerate([("top", "psfT"), ("bottom", "psfB")]):
