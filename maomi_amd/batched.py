@@ -108,6 +108,8 @@ class OpticalPlan:
                                              i1.shape[0], i1.shape[1], float(np.float32(scale)),
                                              nat.ptr(tmp), nat.ptr(ph), st), "resize phase")
             self.phases.append(ph)
+        from .plain import check_screen_range
+        check_screen_range(torch.stack([ph.abs().amax() for ph in self.phases]))
         a, b, c = [f.shape[-1] for f in raw_fields]
         self.z_positions = g.z
         self.cut = g.cut

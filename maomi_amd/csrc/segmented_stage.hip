@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "fft_reg.hpp"
 #include "maomi_amd.h"
+#include "screen_trig.hpp"
 
 // Dynamic LDS of the segmented kernel; the line transforms address it by offset.
 extern __shared__ float2 seg_lds[];
@@ -284,7 +285,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         if (!last) {
           const float phi = g.phase[((int64_t)s * g.H + ox + x) * g.W + oy + y];
           float sn, cs;
-          sincosf(phi, &sn, &cs);
+          trig::sincos_screen(phi, &sn, &cs);  // |phi| < 2^17, checked by the host
           const float sg = E[x] * E[y];
           v = cmul(v, make_float2(cs * sg, sn * sg));
         }
