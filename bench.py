@@ -24,6 +24,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of sites_kernel<256> on the 4225-site FP32 workload, from two rocprofv3
+# --pmc runs of this script: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy,
+# --copy-probe); the runs and their commands are in profiles/sites_kernel_256_config2_pmc.md.
+TRAFFIC_CONFIG2_FP32 = (2.0 * 1.658e8 + 1.0 * 3.254e8) * 1024
 METRIC = "sites/s on 4225-site optical stage; surrogate vs propagator at ≤1% rel L2"
 
 
@@ -205,7 +209,8 @@ def main():
         "stage_seconds": {k: round(v, 6) for k, v in st.timing.items()},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": TRAFFIC_CONFIG2_FP32 if (cfg is CONFIG2 and not half) else None,
             "kernel": "sites_kernel<%d>" % cfg.n, "kernel_ms": kms,
             "algorithmic_bytes_per_launch": bytes_launch,
         },
