@@ -302,6 +302,43 @@ int maomi_lfft_scatter(const double* full, int64_t B, int64_t Lx, int64_t Ly, in
                        const int32_t* desc, const double* weights, double* out, int64_t X,
                        int64_t Y, int64_t Zo, int64_t Tx, int64_t Ty, maomi_stream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Input generation: spectral Gaussian-process index fields and the collection
+ * disk (maomi/_optical_math.py:20-67, maomi/optics.py:120-197 and 384-415).
+ * ------------------------------------------------------------------------- */
+
+/* spec (complex64, X x Y x Z) at the ifftshifted index of voxel j =
+ * (noise_re[j] + i noise_im[j]) * sqrt(sum_i coef[2i] * (kx*ky*kz)^2 * coef[2i+1]);
+ * kernels holds per scale i the 1-D factors [kx (X) | ky (Y) | kz (Z)] and coef = (prod l, w^2)
+ * (_optical_math.py:26-57, the envelope and noise product before the inverse FFT) */
+int maomi_grf_spectrum(const float* noise_re, const float* noise_im, int64_t X, int64_t Y,
+                       int64_t Z, const float* kernels, const float* coef, int32_t K, void* spec,
+                       maomi_stream_t stream);
+/* out = scale * Re(field at the ifftshifted index) + mu (field NULL: zeros), times first_z on
+ * z = 0 and last_z on z = Z - 1, then occupancy + out * (1 - occupancy / occupancy_div) when
+ * occupancy is given (_optical_math.py:59-64; optics.py:163-166 and 191-192) */
+int maomi_grf_finish(const void* field, int64_t X, int64_t Y, int64_t Z, float scale, float mu,
+                     float first_z, float last_z, const float* occupancy, float occupancy_div,
+                     float* out, maomi_stream_t stream);
+/* out[(x * Y + y) * Zo + zo] = sum over z < L of src[x][y][z] (vessel slab occupancy,
+ * optics.py:184-190) */
+int maomi_zsum(const float* src, int64_t X, int64_t Y, int64_t L, float* out, int64_t Zo,
+               int64_t zo, maomi_stream_t stream);
+/* prefix[x][m] (X x (Y + 1), FP64) = sum over y < m of float32(sum over z < L of slab[x][y][z])
+ * (the slab projection of optics.py:409, prefix-summed along y) */
+int maomi_slab_prefix(const float* slab, int64_t X, int64_t Y, int64_t L, double* prefix,
+                      maomi_stream_t stream);
+/* total (Xo x Yo, FP64) += weight * (slab projection convolved with a disk, scipy 'same'
+ * placement, at rows ox.., columns oy..); runs[2k], runs[2k+1] = first and last column of disk
+ * row k, c0 = (K0 - 1) / 2, c1 = (K1 - 1) / 2 (optics.py:393-409) */
+int maomi_disk_accumulate(const double* prefix, int64_t X, int64_t Y, const int32_t* runs,
+                          int32_t K0, int32_t c0, int32_t c1, double weight, int64_t ox,
+                          int64_t oy, int64_t Xo, int64_t Yo, double* total,
+                          maomi_stream_t stream);
+/* out = 10 ** (-total / vres * hemoabs) in float32 (optics.py:410-413) */
+int maomi_hemo_mask(const double* total, int64_t n, float vres, float hemoabs, float* out,
+                    maomi_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

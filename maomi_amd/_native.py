@@ -6,7 +6,7 @@ operation of the package raises if the HIP library is missing or a call fails.
 
 import ctypes
 import os
-from ctypes import c_char_p, c_double, c_float, c_int, c_int64, c_void_p
+from ctypes import c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_void_p
 from pathlib import Path
 
 _PATH = Path(os.environ.get("MAOMI_AMD_LIBRARY") or
@@ -108,6 +108,17 @@ _SIGS = {
     "maomi_lfft_scatter": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
                                    c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64,
                                    c_int64, c_void_p]),
+    "maomi_grf_spectrum": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p,
+                                   c_void_p, c_int32, c_void_p, c_void_p]),
+    "maomi_grf_finish": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_float, c_float, c_float,
+                                 c_float, c_void_p, c_float, c_void_p, c_void_p]),
+    "maomi_zsum": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
+                           c_void_p]),
+    "maomi_slab_prefix": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
+    "maomi_disk_accumulate": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int32, c_int32,
+                                      c_int32, c_double, c_int64, c_int64, c_int64, c_int64,
+                                      c_void_p, c_void_p]),
+    "maomi_hemo_mask": (c_int, [c_void_p, c_int64, c_float, c_float, c_void_p, c_void_p]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

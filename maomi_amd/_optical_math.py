@@ -56,25 +56,10 @@ class PsfParams:
 def masked_3DGP_test(grid_sz, l_scale, p_scale, mu=0, bin_mask=1, l_weights=None, device="cpu"):
     """Spectral Gaussian-process field: a weighted sum of Gaussian spectral envelopes
     times complex white noise, inverse-FFT'd, real part scaled by 2 p_scale sqrt(N)
-    (_optical_math.py:20-67).  Input generation; uses torch's global RNG."""
-    if l_weights is None:
-        l_weights = torch.ones(l_scale.shape[0], device=device)
-    dims = [int(s) for s in grid_sz]
-    half_pi = np.pi / 2
-    sq = [torch.linspace(-half_pi, half_pi, d, device=device, dtype=torch.float32).pow(2)
-          for d in dims]
-    gx, gy, gz = sq[0].view(-1, 1, 1), sq[1].view(1, -1, 1), sq[2].view(1, 1, -1)
-    env = torch.zeros(dims, device=device, dtype=torch.float32)
-    for i in range(l_scale.shape[0]):
-        k3 = (torch.exp(-gx * l_scale[i, 0] ** 2) * torch.exp(-gy * l_scale[i, 1] ** 2)
-              * torch.exp(-gz * l_scale[i, 2] ** 2))
-        env += torch.prod(l_scale[i]) * (k3 ** 2) * l_weights[i] ** 2
-    env = torch.sqrt(env)
-    noise = (torch.randn(dims, device=device, dtype=torch.float32)
-             + 1j * torch.randn(dims, device=device, dtype=torch.float32))
-    g = torch.fft.ifftshift(noise * env, dim=(0, 1, 2))
-    g = torch.fft.ifftshift(torch.fft.ifftn(g, dim=(0, 1, 2)), dim=(0, 1, 2))
-    return 2 * p_scale * bin_mask * np.sqrt(g.numel()) * torch.real(g) + mu
+    (_optical_math.py:20-67).  Noise from torch's generator on `device`; synthesis on the
+    GPU (maomi_amd.inputs.masked_gp); the field is returned on `device`."""
+    from .inputs import masked_gp
+    return masked_gp(grid_sz, l_scale, p_scale, mu, bin_mask, l_weights, device=device)
 
 
 def groupzproject(vol, chunk_size, mode="sum"):

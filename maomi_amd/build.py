@@ -25,6 +25,7 @@ SOURCES = {
     "segmented_stage.hip": [],
     "surrogate.hip": [],
     "local_fft.hip": [],
+    "inputs.hip": ["-ffp-contract=off"],
 }
 
 

@@ -192,7 +192,8 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
                 or sum(weights.values()) <= 0):
             raise ValueError("Nonnegative detected-intensity weight per beam required")
         start = time.perf_counter()
-        raw = prepare_fields(config, vessels, scatter=scatter) if raw_fields is None else raw_fields
+        raw = (prepare_fields(config, vessels, scatter=scatter, out_device=device)
+               if raw_fields is None else raw_fields)
         channels = {name: run_fixed(config, volume, vessels=vessels, raw_fields=raw,
                                     input_pupil=f, tails=tails, batch_size=batch_size,
                                     count_per_axis=count_per_axis, mask_mode=mask_mode,
@@ -229,7 +230,8 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
     timings = {}
     supplied = raw_fields is not None
     t = time.perf_counter()
-    raw = prepare_fields(config, vessels, scatter=scatter) if raw_fields is None else raw_fields
+    raw = (prepare_fields(config, vessels, scatter=scatter, out_device=device)
+           if raw_fields is None else raw_fields)
     exp_shapes = field_shapes(config)
     if len(raw) != 3 or any(tuple(a.shape) != tuple(s) for a, s in zip(raw, exp_shapes)):
         raise ValueError(f"Raw A/B/C fields must have shapes {exp_shapes}")

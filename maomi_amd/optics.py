@@ -133,51 +133,15 @@ def _shapes(config):
     return n, n2, za, zb, pc, nc, xy
 
 
-def prepare_fields(config, vessels=None, *, device="cpu", scatter=True):
-    """Seeded raw A/B/C index fields (optics.py:120-197); input generation, untimed."""
-    n, n2, za, zb, pc, nc, xy = _shapes(config)
-    r = config.vres
-    shapes = [(xy[0] + n, xy[1] + n, int(np.ceil(za / pc))), (xy[0] + n2, xy[1] + n2, zb),
-              (xy[0] + n, xy[1] + n, int(np.ceil(nc / pc)))]
-    raw = []
-    devs = [torch.device(device).index or 0] if str(device).startswith("cuda") else []
-    with torch.random.fork_rng(devices=devs):
-        torch.manual_seed(config.seed)
-        scales = torch.tensor(config.scatter_scales_um, device=device,
-                              dtype=torch.float32)[:, None].repeat(1, 3) * r
-        wts = torch.tensor(config.scatter_weights, device=device, dtype=torch.float32)
-        for i, shape in enumerate(shapes):
-            ls = scales.clone()
-            slab = 1 if i == 1 else pc
-            ls[:, 2] /= slab
-            raw.append(legacy.masked_3DGP_test(shape, ls, slab, l_weights=wts, device=device)
-                       if scatter else torch.zeros(shape, device=device))
-    if za % pc:
-        raw[0][..., 0] *= (za % pc) / pc
-    if nc % pc:
-        raw[2][..., -1] *= (nc % pc) / pc
-    if vessels is not None:
-        vessels = vessels if hasattr(vessels, "shape") else np.asarray(vessels)
-        need = (shapes[0][0], shapes[0][1], za + zb + nc)
-        if any(a < b for a, b in zip(vessels.shape, need)):
-            raise ValueError(f"Padded vessels need at least {need}, got {tuple(vessels.shape)}")
+def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_device=None):
+    """Seeded raw A/B/C index fields (optics.py:120-197); input generation, untimed.
 
-        def read(xs, ys, zs):
-            a = vessels[xs, ys, zs]
-            if isinstance(a, torch.Tensor):
-                return a.to(device=device, dtype=torch.float32)
-            return torch.as_tensor(np.asarray(a), device=device, dtype=torch.float32)
-
-        X, Y = slice(0, need[0]), slice(0, need[1])
-        bounds = [0] + list(range(za % pc or pc, za, pc)) + [za]
-        occA = torch.stack([read(X, Y, slice(s, e)).sum(-1) for s, e in zip(bounds[:-1], bounds[1:])], -1)
-        off = (n - n2) // 2
-        occB = read(slice(off, off + shapes[1][0]), slice(off, off + shapes[1][1]), slice(za, za + zb))
-        occC = torch.stack([read(X, Y, slice(z, min(z + pc, za + zb + nc))).sum(-1)
-                            for z in range(za + zb, za + zb + nc, pc)], -1)
-        for i, occ in enumerate((occA, occB, occC)):
-            raw[i] = occ + raw[i] * (1 - occ / (1 if i == 1 else pc))
-    return raw
+    The noise is drawn with torch's generator on `device`, as the reference does; synthesis
+    runs on the GPU (maomi_amd.inputs).  Fields are returned on `out_device` (default
+    `device`)."""
+    from .inputs import prepare_fields as gpu_fields
+    return gpu_fields(config, vessels, device=device, scatter=scatter, out_device=out_device,
+                      shapes=_shapes(config))
 
 
 def propagate(config, raw_fields, points_um, *, uin=None, device="cuda", save_full=False,
@@ -212,33 +176,10 @@ def propagate(config, raw_fields, points_um, *, uin=None, device="cuda", save_fu
 
 
 def collection_mask(vessels, config):
-    """Depth-dependent collection disk and hemoglobin absorption (optics.py:384-415); input generation."""
-    from scipy.signal import fftconvolve
-    a = vessels if hasattr(vessels, "shape") else np.asarray(vessels)
-    r = config.vres
-    pc = int(round(config.prop_um * r))
-    total = np.zeros(a.shape[:2], np.float32)
-    for start in range(0, a.shape[-1], pc):
-        depth = (start / pc + 0.5) * config.prop_um
-        radius = r * np.tan(np.arcsin(config.objNA / config.n)) * (
-            config.depth_um + config.volume_um[2] / 2 - depth)
-        if radius <= 0:
-            continue
-        x = np.arange(-a.shape[0] / 2, a.shape[0] / 2)[:, None]
-        y = np.arange(-a.shape[1] / 2, a.shape[1] / 2)[None, :]
-        disk = (x * x + y * y <= radius * radius).astype(np.float32)
-        nz = np.argwhere(disk)
-        if not len(nz):
-            continue
-        lo, hi = nz.min(0), nz.max(0) + 1
-        disk = disk[lo[0]:hi[0], lo[1]:hi[1]]
-        disk /= disk.sum()
-        total += fftconvolve(np.asarray(a[..., start:start + pc], np.float32).sum(-1), disk,
-                             mode="same")
-    shape = np.round(np.array(config.volume_um[:2]) * r).astype(int)
-    o = (np.array(total.shape) - shape) // 2
-    total = total[o[0]:o[0] + shape[0], o[1]:o[1] + shape[1]]
-    return 10 ** (-total / r * config.hemoabs)
+    """Depth-dependent collection disk and hemoglobin absorption (optics.py:384-415); input
+    generation, on the GPU (maomi_amd.inputs.collection_mask)."""
+    from .inputs import collection_mask as gpu_mask
+    return gpu_mask(vessels, config)
 
 
 def run_optics(config, points_um, *, vessels=None, scatter=True, custom_phase=None,

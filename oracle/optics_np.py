@@ -62,6 +62,8 @@ class OracleConfig:
     n_diff: float = 0.02
     hemoabs: float = 0.0155194235
     propcrop: bool = True
+    scatter_scales_um: tuple = (0.51, 1.56, 4.52, 14.78)
+    scatter_weights: tuple = (0.57, 0.29, 0.19, 0.15)
     seed: int = 20260927
     scaling: str = "two-photon"
     temporal_length_um: float = 10.0
