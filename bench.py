@@ -92,9 +92,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="2", choices=("1", "2", "3", "4", "5", "ref"),
+    ap.add_argument("--config", default="2", choices=("1", "2", "3", "4", "5", "ref", "inputs"),
                     help="1-5: the plain-stage workloads and the surrogate; ref: the reference's "
-                         "own A/B/C geometry (demo scene) through run_fixed")
+                         "own A/B/C geometry (demo scene) through run_fixed; inputs: raw index "
+                         "fields and collection mask of a 256 x 256 x 50 um volume")
     ap.add_argument("--train-volumes", type=int, default=5,
                     help="surrogate (config 5): training volumes of 4225 sites each")
     ap.add_argument("--epochs", type=int, default=6, help="surrogate (config 5) epochs")
@@ -131,6 +132,8 @@ def main():
 
     if args.config == "ref":
         return bench_reference(args)
+    if args.config == "inputs":
+        return bench_inputs(args)
     args.config = int(args.config)
     if args.config == 3:
         return bench_uneven(args)
@@ -353,6 +356,84 @@ def bench_reference(args):
                      "frac": ach / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "segmented_sites_kernel (host-timed propagation stage)",
                      "kernel_ms": prop * 1e3, "algorithmic_bytes_per_launch": per_site * P},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+INPUTS_SCENE = dict(volume_um=(256.0, 256.0, 50.0), depth_um=100.0, vres=1.0, tail_um=20.0)
+
+
+def bench_inputs(args):
+    """Input generation (maomi/optics.py:120-197, 384-415): the seeded raw A/B/C index fields
+    with vessel occupancy and the collection mask for a 256 x 256 x 50 um volume 100 um deep at
+    1 voxel/um (fields 532^2 x 24, 292^2 x 10, 532^2 x 4; vessels 532^2 x 150, 3% occupied,
+    resident in HBM).  Noise is drawn on the GPU.  One step = prepare_fields + collection_mask;
+    the unit is raw-field voxels per second."""
+    import torch
+    from maomi_amd.optics import OpticalConfig, collection_mask, prepare_fields
+    cfg = OpticalConfig(**INPUTS_SCENE)
+    shape = np.rint(np.array(cfg.parameters()[0].vasc_sz) * cfg.vres).astype(int)
+    shape[2] = max(shape[2], 150)
+    vessels_np = np.random.default_rng(7).random(shape) < 0.03
+    vessels = torch.as_tensor(vessels_np, device="cuda:0")
+    for _ in range(args.warmup):
+        f = prepare_fields(cfg, vessels, device="cuda", out_device="cuda")
+        collection_mask(vessels, cfg)
+    voxels = sum(int(np.prod(a.shape)) for a in f)
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f = prepare_fields(cfg, vessels, device="cuda", out_device="cuda", events=events)
+        m = collection_mask(vessels, cfg, events=events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # spectrum kernel: reads the two float32 draws, writes complex64; finish kernel: reads the
+    # complex64 field, writes float32 (+ float32 occupancy read); the collection-mask launch
+    # reports its own byte count (vessel slabs, FP64 prefix and partial tables, output)
+    per_voxel = {"grf_spectrum": 16, "grf_finish": 12, "grf_finish_occupancy": 16,
+                 "collection_mask": 1}
+    kern = {}
+    for name, dims, (a, b) in events:
+        k = kern.setdefault(name, [0.0, 0])
+        k[0] += a.elapsed_time(b) * 1e-3
+        k[1] += per_voxel[name] * int(np.prod(dims))
+    top = max(kern, key=lambda k: kern[k][0])
+    ach = kern[top][1] / kern[top][0] / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import OracleConfig
+        from oracle import inputs_np as I
+        ocfg = OracleConfig(**INPUTS_SCENE)
+        t = time.perf_counter()
+        with torch.random.fork_rng(devices=[]):
+            torch.manual_seed(cfg.seed)
+            noises = [(torch.randn(s).numpy(), torch.randn(s).numpy())
+                      for s in I.field_shapes(ocfg)]
+        I.prepare_fields(ocfg, noises, vessels=vessels_np)
+        I.collection_mask(vessels_np, ocfg)
+        tc = time.perf_counter() - t
+        cpu = dict(value=voxels / tc, unit="voxels/s", cores=1, kind="port",
+                   sample=(f"the whole workload once: torch CPU normal draws, NumPy FP64 "
+                           f"envelope and inverse FFT, occupancy mix, scipy fftconvolve "
+                           f"collection disks ({tc:.2f} s)"))
+    line = {
+        "metric": "input generation voxels/s (raw A/B/C index fields + collection mask)",
+        "value": voxels * args.steps / elapsed, "unit": "voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 disk sums)",
+        "data": "synthetic (seeded normal draws and random vessel occupancy; no downloads)",
+        "config": {"workload": ("raw A/B/C fields and collection mask, 256 x 256 x 50 um "
+                                "volume 100 um deep, 1 voxel/um, vessels 3% occupied"),
+                   "fields": [list(a.shape) for a in f], "vessels": [int(v) for v in shape],
+                   "mask": list(m.shape)},
+        "kernels_ms_per_step": {k: v[0] / args.steps * 1e3 for k, v in kern.items()},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": top,
+                     "kernel_ms": kern[top][0] / args.steps * 1e3,
+                     "algorithmic_bytes_per_step": kern[top][1] / args.steps},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -320,24 +320,29 @@ int maomi_grf_spectrum(const float* noise_re, const float* noise_im, int64_t X, 
 int maomi_grf_finish(const void* field, int64_t X, int64_t Y, int64_t Z, float scale, float mu,
                      float first_z, float last_z, const float* occupancy, float occupancy_div,
                      float* out, maomi_stream_t stream);
-/* out[(x * Y + y) * Zo + zo] = sum over z < L of src[x][y][z] (vessel slab occupancy,
- * optics.py:184-190) */
-int maomi_zsum(const float* src, int64_t X, int64_t Y, int64_t L, float* out, int64_t Zo,
-               int64_t zo, maomi_stream_t stream);
-/* prefix[x][m] (X x (Y + 1), FP64) = sum over y < m of float32(sum over z < L of slab[x][y][z])
- * (the slab projection of optics.py:409, prefix-summed along y) */
-int maomi_slab_prefix(const float* slab, int64_t X, int64_t Y, int64_t L, double* prefix,
-                      maomi_stream_t stream);
-/* total (Xo x Yo, FP64) += weight * (slab projection convolved with a disk, scipy 'same'
- * placement, at rows ox.., columns oy..); runs[2k], runs[2k+1] = first and last column of disk
- * row k, c0 = (K0 - 1) / 2, c1 = (K1 - 1) / 2 (optics.py:393-409) */
-int maomi_disk_accumulate(const double* prefix, int64_t X, int64_t Y, const int32_t* runs,
-                          int32_t K0, int32_t c0, int32_t c1, double weight, int64_t ox,
-                          int64_t oy, int64_t Xo, int64_t Yo, double* total,
-                          maomi_stream_t stream);
-/* out = 10 ** (-total / vres * hemoabs) in float32 (optics.py:410-413) */
-int maomi_hemo_mask(const double* total, int64_t n, float vres, float hemoabs, float* out,
-                    maomi_stream_t stream);
+/* For every run r (z0, z1) = (runs[2r], runs[2r + 1]), clamped to [0, Z):
+ * out[r * out_stride_run + (x * Y + y) * out_stride_xy] = sum over z0 <= z < z1 of
+ * src[x * stride_x + y * stride_y + z] (vessel slab occupancy, optics.py:184-190); src is
+ * X x Y x Z with unit z stride, elements float32 (elem_bytes 4) or 0/1 bytes (elem_bytes 1);
+ * runs is a device array; the source is read once for all runs */
+int maomi_zsum(const void* src, int32_t elem_bytes, int64_t X, int64_t Y, int64_t Z,
+               int64_t stride_x, int64_t stride_y, const int32_t* runs, int32_t nruns, float* out,
+               int64_t out_stride_run, int64_t out_stride_xy, maomi_stream_t stream);
+/* FP64 workspace elements of maomi_collection_mask: ns * ((X * Y + 1) / 2 + X * (Y + 1) + Xo * Yo) */
+int64_t maomi_collection_mask_workspace(int64_t X, int64_t Y, int64_t Xo, int64_t Yo,
+                                        int32_t ns);
+/* Collection mask (optics.py:384-415): for each of ns slabs, slabs[6s..6s+5] = (z0, z1, runs
+ * offset, K0, c0 = (K0 - 1) / 2, c1 = (K1 - 1) / 2), the projection of vessels[:, :, z0:z1]
+ * (X x Y x Z, element strides stride_x, stride_y, 1; elements as maomi_zsum) is convolved with
+ * the slab's disk, given as K0 row runs runs[off + 2k], runs[off + 2k + 1] (first and last
+ * column) and the weight weights[s] = 1 / cells, with scipy 'same' placement, on the output
+ * window rows ox.., columns oy.. (Xo x Yo); out (float32) = 10 ** (-total / vres * hemoabs) of
+ * the slab sum.  slabs, weights and runs are device arrays. */
+int maomi_collection_mask(const void* vessels, int32_t elem_bytes, int64_t X, int64_t Y,
+                          int64_t Z, int64_t stride_x, int64_t stride_y, const int32_t* slabs,
+                          const double* weights, const int32_t* runs, int32_t ns, int64_t ox,
+                          int64_t oy, int64_t Xo, int64_t Yo, float vres, float hemoabs,
+                          double* workspace, float* out, maomi_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -112,13 +112,13 @@ _SIGS = {
                                    c_void_p, c_int32, c_void_p, c_void_p]),
     "maomi_grf_finish": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_float, c_float, c_float,
                                  c_float, c_void_p, c_float, c_void_p, c_void_p]),
-    "maomi_zsum": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_int64, c_int64,
-                           c_void_p]),
-    "maomi_slab_prefix": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p]),
-    "maomi_disk_accumulate": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int32, c_int32,
-                                      c_int32, c_double, c_int64, c_int64, c_int64, c_int64,
+    "maomi_zsum": (c_int, [c_void_p, c_int32, c_int64, c_int64, c_int64, c_int64, c_int64,
+                           c_void_p, c_int32, c_void_p, c_int64, c_int64, c_void_p]),
+    "maomi_collection_mask_workspace": (c_int64, [c_int64, c_int64, c_int64, c_int64, c_int32]),
+    "maomi_collection_mask": (c_int, [c_void_p, c_int32, c_int64, c_int64, c_int64, c_int64,
+                                      c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_int64,
+                                      c_int64, c_int64, c_int64, c_float, c_float, c_void_p,
                                       c_void_p, c_void_p]),
-    "maomi_hemo_mask": (c_int, [c_void_p, c_int64, c_float, c_float, c_void_p, c_void_p]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

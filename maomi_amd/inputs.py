@@ -46,7 +46,7 @@ def _envelope_tables(dims, l_scale, l_weights, device):
     return torch.stack(rows), torch.stack(coef)
 
 
-def _spectrum(re, im, l_scale, l_weights, table_device):
+def _spectrum(re, im, l_scale, l_weights, table_device, events=None):
     dev = _gpu()
     dims = tuple(int(s) for s in re.shape)
     if len(dims) != 3 or tuple(im.shape) != dims:
@@ -57,20 +57,37 @@ def _spectrum(re, im, l_scale, l_weights, table_device):
     re = torch.as_tensor(re, dtype=torch.float32).to(dev).contiguous()
     im = torch.as_tensor(im, dtype=torch.float32).to(dev).contiguous()
     spec = torch.empty(dims, dtype=torch.complex64, device=dev)
+    ev = _record(events, "grf_spectrum", dims)
     nat.check(nat.lib().maomi_grf_spectrum(nat.ptr(re), nat.ptr(im), *dims, nat.ptr(ker),
                                            nat.ptr(coef), int(ker.shape[0]), nat.ptr(spec),
                                            nat.stream_handle()), "maomi_grf_spectrum")
+    if ev is not None:
+        ev[1].record()
     return torch.fft.ifftn(spec, dim=(0, 1, 2)).contiguous()
 
 
-def _finish(g, dims, scale, mu=0.0, first=1.0, last=1.0, occupancy=None, occ_div=1.0):
+def _record(events, name, dims):
+    """Start a timing event pair on the current stream when `events` (a list) is given."""
+    if events is None:
+        return None
+    pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    events.append((name, tuple(dims), pair))
+    pair[0].record()
+    return pair
+
+
+def _finish(g, dims, scale, mu=0.0, first=1.0, last=1.0, occupancy=None, occ_div=1.0,
+            events=None):
     dev = _gpu()
     out = torch.empty(dims, dtype=torch.float32, device=dev)
     occ = None if occupancy is None else occupancy.to(dev, torch.float32).contiguous()
+    ev = _record(events, "grf_finish" + ("_occupancy" if occ is not None else ""), dims)
     nat.check(nat.lib().maomi_grf_finish(nat.ptr(g), *dims, float(scale), float(mu),
                                          float(first), float(last), nat.ptr(occ),
                                          float(occ_div), nat.ptr(out), nat.stream_handle()),
               "maomi_grf_finish")
+    if ev is not None:
+        ev[1].record()
     return out
 
 
@@ -102,47 +119,79 @@ def masked_gp(grid_sz, l_scale, p_scale, mu=0, bin_mask=1, l_weights=None, devic
                           table_device=device).to(device)
 
 
-def _reader(vessels, dev):
+def _reader(vessels, dev, raw=False):
+    """read(xs, ys, zs) -> contiguous device tensor of that vessel block: float32, or with
+    raw=True boolean blocks as their 0/1 bytes (uint8) and anything else as float32."""
     def read(xs, ys, zs):
         a = vessels[xs, ys, zs]
-        if isinstance(a, torch.Tensor):
-            return a.to(device=dev, dtype=torch.float32).contiguous()
-        return torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(torch.float32).contiguous()
+        if not isinstance(a, torch.Tensor):
+            a = np.ascontiguousarray(a)
+            a = torch.from_numpy(a.view(np.uint8) if a.dtype == np.bool_ else a)
+        if raw and a.dtype in (torch.bool, torch.uint8):
+            return a.to(dev).contiguous().view(torch.uint8)
+        return a.to(device=dev, dtype=torch.float32).contiguous()
     return read
 
 
-def _zsum_into(slab, out, zo):
-    X, Y, L = (int(s) for s in slab.shape)
-    nat.check(nat.lib().maomi_zsum(nat.ptr(slab), X, Y, L, nat.ptr(out), int(out.shape[2]),
-                                   int(zo), nat.stream_handle()), "maomi_zsum")
+def _elem_bytes(t):
+    return 1 if t.dtype == torch.uint8 else 4
+
+
+def _device_source(vessels):
+    """The vessels themselves, as (tensor, element bytes), when they are a contiguous 3-D GPU
+    tensor of bool, uint8 or float32 that the kernels can read in place; None otherwise."""
+    if (isinstance(vessels, torch.Tensor) and vessels.is_cuda and vessels.dim() == 3
+            and vessels.is_contiguous()):
+        if vessels.dtype in (torch.bool, torch.uint8):
+            return vessels.view(torch.uint8), 1
+        if vessels.dtype == torch.float32:
+            return vessels, 4
+    return None
+
+
+def _zsum_runs(vessels, X, Y, runs, out, stride_run, stride_xy, dev):
+    """Sums of the z runs [(z0, z1), ...] of vessels[:X, :Y] into out (one launch; GPU-resident
+    vessels are read in place, others through indexing of the z range the runs cover)."""
+    if any(z0 > z1 for z0, z1 in runs) or any(a[1] > b[0] for a, b in zip(runs, runs[1:])):
+        raise ValueError("z runs must be sorted and must not overlap")
+    src = _device_source(vessels)
+    if src is not None:
+        block, eb, zoff = src[0][:X, :Y], src[1], 0
+    else:
+        z_lo, z_hi = min(r[0] for r in runs), max(r[1] for r in runs)
+        block = _reader(vessels, dev, raw=True)(slice(0, X), slice(0, Y), slice(z_lo, z_hi))
+        eb, zoff = _elem_bytes(block), z_lo
+    rd = torch.tensor([[z0 - zoff, z1 - zoff] for z0, z1 in runs], dtype=torch.int32).to(dev)
+    nat.check(nat.lib().maomi_zsum(nat.ptr(block), eb, X, Y, int(block.shape[2]),
+                                   block.stride(0), block.stride(1), nat.ptr(rd), len(runs),
+                                   nat.ptr(out), stride_run, stride_xy, nat.stream_handle()),
+              "maomi_zsum")
 
 
 def _occupancy(vessels, shapes, n, n2, za, zb, pc, nc, dev):
     """Vessel occupancy of the A/B/C fields (optics.py:167-190): slab sums for A and C, the
     centred crop for B."""
-    read = _reader(vessels, dev)
-    need = (shapes[0][0], shapes[0][1], za + zb + nc)
-    X, Y = slice(0, need[0]), slice(0, need[1])
+    X, Y = shapes[0][0], shapes[0][1]
     occA = torch.empty(shapes[0], dtype=torch.float32, device=dev)
     bounds = [0] + list(range(za % pc or pc, za, pc)) + [za]
-    for k, (s, e) in enumerate(zip(bounds[:-1], bounds[1:])):
-        _zsum_into(read(X, Y, slice(s, e)), occA, k)
+    _zsum_runs(vessels, X, Y, list(zip(bounds[:-1], bounds[1:])), occA, 1, shapes[0][2], dev)
     off = (n - n2) // 2
-    occB = read(slice(off, off + shapes[1][0]), slice(off, off + shapes[1][1]),
-                slice(za, za + zb))
+    occB = _reader(vessels, dev)(slice(off, off + shapes[1][0]), slice(off, off + shapes[1][1]),
+                                 slice(za, za + zb))
     occC = torch.empty(shapes[2], dtype=torch.float32, device=dev)
-    for k, z in enumerate(range(za + zb, za + zb + nc, pc)):
-        _zsum_into(read(X, Y, slice(z, min(z + pc, za + zb + nc))), occC, k)
+    runs = [(z, min(z + pc, za + zb + nc)) for z in range(za + zb, za + zb + nc, pc)]
+    _zsum_runs(vessels, X, Y, runs, occC, 1, shapes[2][2], dev)
     return occA, occB, occC
 
 
 def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_device=None,
-                   shapes=None):
+                   shapes=None, events=None):
     """Seeded raw A/B/C index fields (optics.py:120-197) synthesised on the GPU.
 
     `device` is where the noise is drawn (the reference's `device`); the fields are returned on
     `out_device` (default: `device`).  `shapes` is (n, n2, za, zb, pc, nc, xy) of
-    `maomi_amd.optics._shapes`."""
+    `maomi_amd.optics._shapes`.  With `events` (a list), a (name, dims, (start, end)) HIP event
+    pair is appended for every field-kernel launch."""
     dev = _gpu()
     n, n2, za, zb, pc, nc, xy = shapes
     fshapes = [(int(xy[0] + n), int(xy[1] + n), int(np.ceil(za / pc))),
@@ -173,14 +222,14 @@ def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_devi
                 ls[:, 2] /= 1 if i == 1 else pc
                 re = torch.randn(shape, device=device, dtype=torch.float32)
                 im = torch.randn(shape, device=device, dtype=torch.float32)
-                spectra[i] = (_spectrum(re, im, ls, wts, device), ls)
+                spectra[i] = (_spectrum(re, im, ls, wts, device, events), ls)
     out = []
     for i, shape in enumerate(fshapes):
         N = int(np.prod(shape))
         p_scale = 1 if i == 1 else pc
         g = spectra[i][0] if spectra[i] is not None else None
         scale = np.float32(2 * p_scale * 1 * np.sqrt(N))
-        out.append(_finish(g, shape, scale, 0.0, first[i], last[i], occ[i], divs[i]))
+        out.append(_finish(g, shape, scale, 0.0, first[i], last[i], occ[i], divs[i], events))
     target = device if out_device is None else out_device
     return [a.to(target) for a in out]
 
@@ -188,10 +237,18 @@ def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_devi
 def _disk_runs(X, Y, radius):
     """The reference's full-grid disk x^2 + y^2 <= radius^2 on arange(-X/2, X/2) by
     arange(-Y/2, Y/2), cropped to its bounding box (optics.py:396-405); returns the first and
-    last column of each row, the cropped size and the cell count, or None when empty."""
-    x = np.arange(-X / 2, X / 2)[:, None]
-    y = np.arange(-Y / 2, Y / 2)[None, :]
-    disk = x * x + y * y <= radius * radius
+    last column of each row, the cropped size and the cell count, or None when empty.
+
+    The same predicate is evaluated only where |x| and |y| are within radius + 1 of the centre:
+    beyond that x^2 alone exceeds radius^2 by far more than a rounding step."""
+    x = np.arange(-X / 2, X / 2)
+    y = np.arange(-Y / 2, Y / 2)
+    xs = np.flatnonzero(np.abs(x) <= radius + 1)
+    ys = np.flatnonzero(np.abs(y) <= radius + 1)
+    if not len(xs) or not len(ys):
+        return None
+    xx, yy = x[xs][:, None], y[ys][None, :]
+    disk = xx * xx + yy * yy <= radius * radius
     rows, cols = np.flatnonzero(disk.any(1)), np.flatnonzero(disk.any(0))
     if not len(rows):
         return None
@@ -206,13 +263,17 @@ def _disk_runs(X, Y, radius):
     return runs, K0, K1, int(count.sum())
 
 
-def collection_mask(vessels, config):
+def collection_mask(vessels, config, events=None):
     """Depth-dependent collection disk and hemoglobin absorption (optics.py:384-415).
 
     For every slab of prop_um, the vessel projection is convolved with a normalised disk whose
     radius follows the collection cone, 'same' placement as scipy.signal.fftconvolve; the sum
     over slabs, cropped to the volume, gives 10 ** (-total / vres * hemoabs).  The convolution
-    is an exact FP64 sum of row runs over prefix sums on the GPU.  Returns float32 NumPy."""
+    is an exact FP64 sum of row runs over prefix sums, every slab in one launch.  GPU-resident
+    vessels are read in place; others are read slab by slab through indexing.  Returns float32
+    NumPy.  With `events` (a list), a ("collection_mask", (bytes,), (start, end)) HIP event pair
+    is appended for the launch; bytes counts the vessel slabs read, the FP64 prefix and partial
+    tables written and read once, and the output."""
     dev = _gpu()
     a = vessels if hasattr(vessels, "shape") else np.asarray(vessels)
     X, Y, Zv = (int(s) for s in (a.shape[0], a.shape[1], a.shape[-1]))
@@ -223,11 +284,9 @@ def collection_mask(vessels, config):
     Xo, Yo = int(shape[0]), int(shape[1])
     if ox < 0 or oy < 0:
         raise ValueError(f"Vessels of {X} x {Y} do not cover the {Xo} x {Yo} volume")
-    read = _reader(a, dev)
-    total = torch.zeros((Xo, Yo), dtype=torch.float64, device=dev)
-    pre = torch.empty((X, Y + 1), dtype=torch.float64, device=dev)
-    lib, stream = nat.lib(), nat.stream_handle()
     cone = np.tan(np.arcsin(config.objNA / config.n))
+    used, runs_all, weights = [], [], []
+    offset = 0
     for start in range(0, Zv, pc):
         depth = (start / pc + 0.5) * config.prop_um
         radius = r * cone * (config.depth_um + config.volume_um[2] / 2 - depth)
@@ -237,18 +296,48 @@ def collection_mask(vessels, config):
         if disk is None:
             continue
         runs, K0, K1, count = disk
-        weight = float(np.float32(1.0) / np.float32(count))
-        slab = read(slice(None), slice(None), slice(start, start + pc))
-        runs_d = torch.from_numpy(runs).to(dev)
-        nat.check(lib.maomi_slab_prefix(nat.ptr(slab), X, Y, int(slab.shape[2]), nat.ptr(pre),
-                                        stream), "maomi_slab_prefix")
-        nat.check(lib.maomi_disk_accumulate(nat.ptr(pre), X, Y, nat.ptr(runs_d), K0,
-                                            (K0 - 1) // 2, (K1 - 1) // 2, weight, ox, oy, Xo,
-                                            Yo, nat.ptr(total), stream),
-                  "maomi_disk_accumulate")
-        del slab, runs_d
+        used.append([start, min(start + pc, Zv), offset, K0, (K0 - 1) // 2, (K1 - 1) // 2])
+        runs_all.append(runs)
+        weights.append(float(np.float32(1.0) / np.float32(count)))
+        offset += len(runs)
+    src = _device_source(a)
+    if src is not None:
+        vol, eb = src
+    elif used:
+        # stage the slabs that contribute, read through indexing, contiguously on the device
+        read = _reader(a, dev, raw=True)
+        parts = [read(slice(None), slice(None), slice(z0, z1)) for z0, z1, *_ in used]
+        if len({t.dtype for t in parts}) > 1:
+            parts = [t.to(torch.float32) for t in parts]
+        vol = torch.cat(parts, dim=2)
+        eb = _elem_bytes(vol)
+        z = 0
+        for u, t in zip(used, parts):
+            u[0], u[1] = z, z + int(t.shape[2])
+            z += int(t.shape[2])
+    ns = len(used)
+    lib, stream = nat.lib(), nat.stream_handle()
     out = torch.empty((Xo, Yo), dtype=torch.float32, device=dev)
-    nat.check(lib.maomi_hemo_mask(nat.ptr(total), Xo * Yo, float(np.float32(r)),
-                                  float(np.float32(config.hemoabs)), nat.ptr(out), stream),
-              "maomi_hemo_mask")
+    if ns:
+        slabs = torch.from_numpy(np.asarray(used, np.int32).ravel()).to(dev)
+        wts = torch.tensor(weights, dtype=torch.float64).to(dev)
+        runs = torch.from_numpy(np.concatenate(runs_all).astype(np.int32)).to(dev)
+        ws = torch.empty(int(lib.maomi_collection_mask_workspace(X, Y, Xo, Yo, ns)),
+                         dtype=torch.float64, device=dev)
+        nbytes = (X * Y * sum(u[1] - u[0] for u in used) * eb
+                  + ns * (8 * X * Y + 16 * X * (Y + 1) + 16 * Xo * Yo) + 4 * Xo * Yo)
+        ev = _record(events, "collection_mask", (nbytes,))
+        nat.check(lib.maomi_collection_mask(nat.ptr(vol), eb, X, Y, int(vol.shape[2]),
+                                            vol.stride(0), vol.stride(1), nat.ptr(slabs),
+                                            nat.ptr(wts), nat.ptr(runs), ns, ox, oy, Xo, Yo,
+                                            float(np.float32(r)),
+                                            float(np.float32(config.hemoabs)), nat.ptr(ws),
+                                            nat.ptr(out), stream), "maomi_collection_mask")
+        if ev is not None:
+            ev[1].record()
+    else:
+        nat.check(lib.maomi_collection_mask(None, 1, X, Y, 1, Y, 1, None, None, None, 0, ox, oy,
+                                            Xo, Yo, float(np.float32(r)),
+                                            float(np.float32(config.hemoabs)), None,
+                                            nat.ptr(out), stream), "maomi_collection_mask")
     return out.cpu().numpy()

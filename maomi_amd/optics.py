@@ -133,7 +133,8 @@ def _shapes(config):
     return n, n2, za, zb, pc, nc, xy
 
 
-def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_device=None):
+def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_device=None,
+                   events=None):
     """Seeded raw A/B/C index fields (optics.py:120-197); input generation, untimed.
 
     The noise is drawn with torch's generator on `device`, as the reference does; synthesis
@@ -141,7 +142,7 @@ def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_devi
     `device`)."""
     from .inputs import prepare_fields as gpu_fields
     return gpu_fields(config, vessels, device=device, scatter=scatter, out_device=out_device,
-                      shapes=_shapes(config))
+                      shapes=_shapes(config), events=events)
 
 
 def propagate(config, raw_fields, points_um, *, uin=None, device="cuda", save_full=False,
@@ -175,11 +176,11 @@ def propagate(config, raw_fields, points_um, *, uin=None, device="cuda", save_fu
     return res
 
 
-def collection_mask(vessels, config):
+def collection_mask(vessels, config, events=None):
     """Depth-dependent collection disk and hemoglobin absorption (optics.py:384-415); input
     generation, on the GPU (maomi_amd.inputs.collection_mask)."""
     from .inputs import collection_mask as gpu_mask
-    return gpu_mask(vessels, config)
+    return gpu_mask(vessels, config, events=events)
 
 
 def run_optics(config, points_um, *, vessels=None, scatter=True, custom_phase=None,
