@@ -75,6 +75,8 @@ typedef struct maomi_sites_desc {
   float* psf_out;             /* optional [psf_nz][P][psf_w][psf_w]: |u|^4 of planes      */
   int64_t psf_z0, psf_nz;     /*   psf_z0 .. psf_z0+psf_nz-1, crop [psf_c0, psf_c0+psf_w)^2 */
   int64_t psf_c0, psf_w;      /*   of each site window (training targets of the surrogate) */
+  const void* screen;         /* [Z][H][W] complex64 exp(i phase) (maomi_screen_exp), or NULL;
+                                 exactly one of phase / phase16 / screen is given            */
 } maomi_sites_desc;
 
 int maomi_persistent_sites(const maomi_sites_desc* desc, maomi_stream_t stream);
@@ -97,6 +99,10 @@ int64_t maomi_sites_default_grid(void);
 int maomi_phase_prepare(const float* index, int64_t X, int64_t Y, int64_t Zv, int64_t vps,
                         double n0, double k0dz, float* phase32, void* phase16,
                         float* slice_scale, float* slice_max, maomi_stream_t stream);
+/* screen[i] = (cos phase[i], sin phase[i]) for n phases (|phase| < 2^17), with the sin/cos of
+ * the sites kernels: the complex screens of maomi_sites_desc.screen, computed once per volume
+ * voxel instead of once per site and pixel (the screen product of numerics.py:200-205) */
+int maomi_screen_exp(const float* phase, int64_t n, void* screen, maomi_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Reference-geometry stage: segments A / B / C per site in one persistent launch.

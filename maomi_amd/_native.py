@@ -26,7 +26,7 @@ class SitesDesc(ctypes.Structure):
         ("grid", c_int64), ("uniform_n", c_int64), ("block_threads", c_int64),
         ("class_edge", c_int64 * 5), ("class_q", c_int64 * 5), ("class_tw", c_int64 * 5),
         ("psf_out", c_void_p), ("psf_z0", c_int64), ("psf_nz", c_int64), ("psf_c0", c_int64),
-        ("psf_w", c_int64),
+        ("psf_w", c_int64), ("screen", c_void_p),
     ]
 
 
@@ -119,6 +119,7 @@ _SIGS = {
                                       c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_int64,
                                       c_int64, c_int64, c_int64, c_float, c_float, c_void_p,
                                       c_void_p, c_void_p]),
+    "maomi_screen_exp": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

@@ -31,9 +31,7 @@ def _split(n):
     return best, n // best
 
 
-def psf_stats(psf):
-    """FP64 per-site energies and per-voxel sums over sites of |x|, x, min(x, 0) of device PSFs
-    [P][...] (one pass on the GPU; assemble uses them instead of host reductions)."""
+def _psf_stats_device(psf):
     P = psf.shape[0]
     K = psf[0].numel()
     d = psf.device
@@ -43,8 +41,29 @@ def psf_stats(psf):
     nat.check(nat.lib().maomi_psf_stats(nat.ptr(flat), P, K, nat.ptr(energy), nat.ptr(sums[0]),
                                         nat.ptr(sums[1]), nat.ptr(sums[2]),
                                         nat.stream_handle(d)), "psf stats")
-    s = sums.cpu().numpy()
-    return {"energies": energy.cpu().numpy(), "sum_abs": s[0], "sum": s[1], "sum_neg": s[2]}
+    return energy, sums
+
+
+def psf_stats(psf):
+    """FP64 per-site energies and per-voxel sums over sites of |x|, x, min(x, 0) of device PSFs
+    [P][...] (one pass on the GPU; assemble uses them instead of host reductions)."""
+    energy, sums = _psf_stats_device(psf)
+    energy, s = to_host([energy, sums])
+    return {"energies": energy, "sum_abs": s[0], "sum": s[1], "sum_neg": s[2]}
+
+
+def to_host(tensors):
+    """Copy device tensors into pinned host memory on the current stream with one
+    synchronisation; returns NumPy arrays (pageable copies of large results are several times
+    slower)."""
+    hosts = []
+    for t in tensors:
+        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        hosts.append(h)
+    if tensors:
+        torch.cuda.current_stream(tensors[0].device).synchronize()
+    return [h.numpy() for h in hosts]
 
 
 class OpticalPlan:
@@ -236,12 +255,28 @@ class OpticalPlan:
 
     def _launch(self, off_x, off_y, order, P, tails, outs):
         desc = self._desc(off_x, off_y, order, P, tails, outs, self._grid(P))
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record()
         nat.check(nat.lib().maomi_segmented_sites(desc, nat.stream_handle(self.device)),
                   "segmented sites")
+        ev[1].record()
+        self.launch_events = ev
+
+    def last_launch_seconds(self):
+        """Duration of the last persistent launch (HIP events on its stream), once complete."""
+        ev = getattr(self, "launch_events", None)
+        return None if ev is None else ev[0].elapsed_time(ev[1]) * 1e-3
 
     # ------------------------------------------------------------------ API
     @torch.inference_mode()
-    def evaluate(self, points_um, *, batch_size=16, tails=False):
+    def evaluate(self, points_um, *, batch_size=16, tails=False, tail_convmask="sites"):
+        """PSFs (and tails) of every point in one persistent launch (batched.py:170-233).
+
+        tail_convmask="mean" returns each tail's convmask averaged over the points (FP64 on
+        the device) as "convmask_mean" instead of the per-point stack, which is all `assemble`
+        uses; "sites" keeps the reference's per-point "convmask"."""
+        if tail_convmask not in ("sites", "mean"):
+            raise ValueError("tail_convmask must be 'sites' or 'mean'")
         pts, off = self._offsets(points_um)
         P = len(pts)
         d = self.device
@@ -252,14 +287,36 @@ class OpticalPlan:
         if P:
             self._launch(ox, oy, order, P, tails, outs)
         self.paths_evaluated += P
-        out = {"psfs": outs["psf"].cpu().numpy(), "points_um": pts, "tails_computed": tails}
+        names, dev = ["psfs"], [outs["psf"]]
         if P:
-            out["psf_stats"] = psf_stats(outs["psf"])
+            energy, sums = _psf_stats_device(outs["psf"])
+            names += ["energy", "sums"]
+            dev += [energy, sums]
+        if tails:
+            for pre in ("top_", "bot_"):
+                if tail_convmask == "mean" and P:
+                    names.append(pre + "mean")
+                    dev.append(_psf_stats_device(outs[pre + "conv"])[1][1])
+                else:
+                    names.append(pre + "conv")
+                    dev.append(outs[pre + "conv"])
+                names += [pre + "w", pre + "prof"]
+                dev += [outs[pre + "w"], outs[pre + "prof"]]
+        host = dict(zip(names, to_host(dev)))
+        out = {"psfs": host["psfs"], "points_um": pts, "tails_computed": tails}
+        if P:
+            s = host["sums"]
+            out["psf_stats"] = {"energies": host["energy"], "sum_abs": s[0], "sum": s[1],
+                                "sum_neg": s[2]}
         if tails:
             for name, pre in (("top", "top_"), ("bottom", "bot_")):
-                out[name] = {"convmask": outs[pre + "conv"].cpu().numpy(),
-                             "weight": outs[pre + "w"].cpu().numpy(),
-                             "profile": outs[pre + "prof"].cpu().numpy()}
+                t = {"weight": host[pre + "w"], "profile": host[pre + "prof"]}
+                if pre + "mean" in host:
+                    TT = outs[pre + "conv"].shape[1:]
+                    t["convmask_mean"] = (host[pre + "mean"] / P).reshape(TT).astype(np.float32)
+                else:
+                    t["convmask"] = host[pre + "conv"]
+                out[name] = t
         return out
 
     def prepare_graph(self, batch_size=32):

@@ -80,6 +80,7 @@ struct SitesArgs {
   const int64_t* base_off;
   const float* phase;
   const __half* phase16;
+  const float2* screen;
   const float* slice_scale;
   const float* edge;
   const float2* q1d;
@@ -111,8 +112,10 @@ struct Smem {
 
 // Launch modes, compile-time so that each kernel carries only its own address arithmetic:
 // FP32 screens; FP16 screens with a per-slice scale; FP32 screens plus the focal-window
-// planes of |u|^4 (a.psf_out).
-enum Mode { kScreens32 = 0, kScreens16 = 1, kScreens32Psf = 2 };
+// planes of |u|^4 (a.psf_out); complex64 screens exp(i phi) computed once per volume voxel
+// (the same sin/cos as the per-site path, so results are identical; the screen read doubles
+// but the per-pixel sin/cos, shared by every site whose window covers the pixel, is gone).
+enum Mode { kScreens32 = 0, kScreens16 = 1, kScreens32Psf = 2, kScreensC64 = 3 };
 
 // Per-site state shared by the passes.
 struct SiteCtx {
@@ -136,6 +139,7 @@ template <int N, int NT, int MODE, int KIND>
 __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, int s, int tid,
                                          Smem<NT>& sm, double& esum) {
   constexpr bool half = MODE == kScreens16, PSF = MODE == kScreens32Psf;
+  constexpr bool cplx = MODE == kScreensC64;
   constexpr bool FIRST = KIND == kFirst || KIND == kOnly, LAST = KIND == kLast || KIND == kOnly;
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
@@ -156,13 +160,18 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
     const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
     // this tile's screen phases and mask pixels, requested before stage 1 so that their
     // latency overlaps the stage-1 loads, transform and exchange
-    float phv[N1], mkv[N1];
+    float phv[cplx ? 1 : N1], mkv[N1];
+    float2 scv[cplx ? N1 : 1];
     if (hoist && p2) {
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
-        if (!LAST)
-          phv[k1] = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                         : a.phase[prow + N2 * k1];
+        if (!LAST) {
+          if (cplx)
+            scv[k1] = a.screen[prow + N2 * k1];
+          else
+            phv[k1] = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                           : a.phase[prow + N2 * k1];
+        }
         if (!FIRST) mkv[k1] = mrow[N2 * k1];
       }
     }
@@ -201,11 +210,17 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
       for (int k1 = 0; k1 < N1; ++k1) {
         const int y = k2 + N2 * k1;
         if (!LAST) {
-          const float phi = hoist ? phv[k1]
-                            : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                                   : a.phase[prow + N2 * k1];
           float sn, cs;
-          screen_sincos(phi, &sn, &cs);
+          if (cplx) {
+            const float2 e = hoist ? scv[k1] : a.screen[prow + N2 * k1];
+            cs = e.x;
+            sn = e.y;
+          } else {
+            const float phi = hoist ? phv[k1]
+                              : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                                     : a.phase[prow + N2 * k1];
+            screen_sincos(phi, &sn, &cs);
+          }
           const float sg = ex * sm.eg[y];
           v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
         }
@@ -445,6 +460,17 @@ __global__ void phase_to_half_kernel(const float* __restrict__ phase32, int64_t 
   }
 }
 
+// screen[i] = (cos phase[i], sin phase[i]) with the sites kernels' own sin/cos
+__global__ void screen_exp_kernel(const float* __restrict__ phase, int64_t n,
+                                  float2* __restrict__ screen) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float sn, cs;
+    screen_sincos(phase[i], &sn, &cs);
+    screen[i] = make_float2(cs, sn);
+  }
+}
+
 __global__ void fill_float_kernel(float* p, int64_t n, float v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -478,8 +504,8 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
     maomi_set_error("maomi_persistent_sites: null pointer argument");
     return MAOMI_EINVAL;
   }
-  if ((d->phase == nullptr) == (d->phase16 == nullptr)) {
-    maomi_set_error("maomi_persistent_sites: give exactly one of phase / phase16");
+  if ((d->phase != nullptr) + (d->phase16 != nullptr) + (d->screen != nullptr) != 1) {
+    maomi_set_error("maomi_persistent_sites: give exactly one of phase / phase16 / screen");
     return MAOMI_EINVAL;
   }
   if (d->phase16 && !d->slice_scale) {
@@ -493,6 +519,7 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   SitesArgs a;
   a.bases = (const float2*)d->bases; a.base_off = d->base_off;
   a.phase = d->phase; a.phase16 = (const __half*)d->phase16; a.slice_scale = d->slice_scale;
+  a.screen = (const float2*)d->screen;
   a.edge = d->edge; a.q1d = (const float2*)d->q1d; a.twid = (const float2*)d->twiddle;
   a.site_x = d->site_x; a.site_y = d->site_y; a.site_n = d->site_n; a.site_s = d->site_s;
   a.order = d->order; a.mask = d->mask; a.mask_off = d->mask_off; a.energy = d->energy;
@@ -516,16 +543,24 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   }
   const dim3 g((unsigned)grid), b(nt);
   hipStream_t st = (hipStream_t)stream;
-  if (a.psf_out && (d->uniform_n == 0 || nt != 512 || a.phase16)) {
+  if (a.psf_out && (d->uniform_n == 0 || nt != 512 || !a.phase)) {
     maomi_set_error("maomi_persistent_sites: focal-plane output needs a uniform field side, "
-                    "FP32 screens and block_threads 512");
+                    "FP32 phase screens and block_threads 512");
+    return MAOMI_EUNSUPPORTED;
+  }
+  if (a.screen && (d->uniform_n == 0 || nt != 512)) {
+    maomi_set_error("maomi_persistent_sites: complex screens need a uniform field side and "
+                    "block_threads 512");
     return MAOMI_EUNSUPPORTED;
   }
   if (a.phase16 && d->uniform_n == 0) {
     maomi_set_error("maomi_persistent_sites: FP16 screens need a uniform field side");
     return MAOMI_EUNSUPPORTED;
   }
-  const int mode = a.psf_out ? kScreens32Psf : a.phase16 ? kScreens16 : kScreens32;
+  const int mode = a.psf_out ? kScreens32Psf
+                   : a.phase16 ? kScreens16
+                   : a.screen  ? kScreensC64
+                               : kScreens32;
 #define MAOMI_LAUNCH(NTV, MODEV)                                                               \
   switch (d->uniform_n) {                                                                      \
     case 64: hipLaunchKernelGGL((sites_kernel<64, NTV, MODEV>), g, b, 0, st, a); break;        \
@@ -539,6 +574,8 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   }
   if (mode == kScreens32Psf) {
     MAOMI_LAUNCH(512, kScreens32Psf)
+  } else if (mode == kScreensC64) {
+    MAOMI_LAUNCH(512, kScreensC64)
   } else if (mode == kScreens16) {
     if (nt == 512) {
       MAOMI_LAUNCH(512, kScreens16)
@@ -593,5 +630,18 @@ MAOMI_API int maomi_phase_prepare(const float* index, int64_t X, int64_t Y, int6
                        slice_max, (__half*)phase16, slice_scale);
     MAOMI_CHECK_LAUNCH();
   }
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_screen_exp(const float* phase, int64_t n, void* screen,
+                               maomi_stream_t stream) {
+  if (!phase || !screen || n < 1) {
+    maomi_set_error("maomi_screen_exp: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  const unsigned b = (unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535);
+  hipLaunchKernelGGL(screen_exp_kernel, dim3(b), dim3(256), 0, (hipStream_t)stream, phase, n,
+                     (float2*)screen);
+  MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

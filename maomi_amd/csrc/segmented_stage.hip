@@ -23,6 +23,14 @@ namespace {
 
 constexpr int NT = 256;
 
+// Workgroup barrier that orders LDS only: global loads and stores stay in flight.  Full
+// __syncthreads() is kept where one thread reads global workspace another thread wrote.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct Seg {
   const float* phase;    // [Z][H][W] phase in radians
   int64_t H, W, Z;       // volume dims; Z = screens
@@ -65,6 +73,7 @@ struct SegArgs {
   float2* work;              // per block: 2 n^2 complex + n2^2 + fx fy + X max(fy,n2) floats
   int64_t work_stride;       // float2 elements per block
   int L;                     // lines per LDS tile
+  int nb;                    // line stride of the scratch buffer B (>= n, see fast_stage1)
 };
 
 __device__ __forceinline__ float2 twid(const float2* T, int j, bool inv) {
@@ -83,7 +92,7 @@ __device__ void line_dft(float2* A, float2* B, int nl, int N, int N1, int N2, co
     for (int m = 0; m < N2; ++m) acc = cadd(acc, cmul(a[N1 * m], twid(T, ((m * k2) % N2) * s1, inv)));
     B[l * N + n1 * N2 + k2] = cmul(acc, twid(T, (n1 * k2) % N, inv));
   }
-  __syncthreads();
+  lds_barrier();
   for (int it = threadIdx.x; it < nl * N; it += NT) {
     const int l = it / N, k = it % N, k2 = k % N2, k1 = k / N2;
     const float2* b = B + l * N + k2;
@@ -91,18 +100,22 @@ __device__ void line_dft(float2* A, float2* B, int nl, int N, int N1, int N2, co
     for (int m = 0; m < N1; ++m) acc = cadd(acc, cmul(b[m * N2], twid(T, ((m * k1) % N1) * s2, inv)));
     A[l * N + k] = acc;
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Fast line transform for N = R1 * R2 with R2 the power-of-two part of N (at most 16) and
 // R1 <= 64: stage 1 is an R2-point register FFT per (line, n1) job followed by the twiddle
 // W_N^(n1 k2); stage 2 an R1-point direct DFT per (line, k2) job whose R1 twiddles are held
 // in registers (R1 <= 16) or per output from LDS (wide_stage2).  X[k2 + R2 k1] lands in
-// natural order.  Lines live at LDS offsets a_off
-// (in and out) and b_off (scratch); the twiddle table W_N^j at t_off.
+// natural order.  Lines live at LDS offsets a_off (in and out, line stride N) and b_off
+// (scratch); the twiddle table W_N^j at t_off.  In the scratch buffer, element (n1, k2) of a
+// line sits at n1 (R2 + 1) + k2 and lines are pad_stride(N) apart: the padding keeps the
+// stage-1 stores of neighbouring jobs (R2 apart otherwise) off the same LDS banks.
+__device__ __forceinline__ int pad_stride(int N, int R2) { return N + N / R2; }
+
 template <int R2, bool INV>
 __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl, int N) {
-  const int R1 = N / R2;
+  const int R1 = N / R2, NB = pad_stride(N, R2);
   const float2* T = seg_lds + t_off;
   for (int job = threadIdx.x; job < nl * R1; job += NT) {
     const int l = job / R1, n1 = job - l * R1;
@@ -111,7 +124,7 @@ __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl
 #pragma unroll
     for (int m = 0; m < R2; ++m) v[m] = a[R1 * m];
     fftk::fft<R2, INV>(v);
-    float2* b = seg_lds + b_off + l * N + n1 * R2;
+    float2* b = seg_lds + b_off + l * NB + n1 * (R2 + 1);
     b[0] = v[0];
     int idx = 0;  // n1 k2 mod N
 #pragma unroll
@@ -125,16 +138,16 @@ __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl
 
 template <int R1, bool INV>
 __device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl, int N) {
-  const int R2 = N / R1;
+  const int R2 = N / R1, NB = pad_stride(N, R2);
   float2 w[R1];  // W_R1^c = W_N^(c R2)
 #pragma unroll
   for (int c = 0; c < R1; ++c) w[c] = twid(seg_lds + t_off, c * R2, INV);
   for (int job = threadIdx.x; job < nl * R2; job += NT) {
     const int l = job / R2, k2 = job - l * R2;
-    const float2* b = seg_lds + b_off + l * N + k2;
+    const float2* b = seg_lds + b_off + l * NB + k2;
     float2 x[R1];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) x[m] = b[m * R2];
+    for (int m = 0; m < R1; ++m) x[m] = b[m * (R2 + 1)];
     float2* a = seg_lds + a_off + l * N + k2;
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
@@ -150,17 +163,17 @@ __device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl
 // read from LDS with the exponent advanced incrementally.
 template <bool INV>
 __device__ __noinline__ void wide_stage2(int a_off, int b_off, int t_off, int nl, int N, int R1) {
-  const int R2 = N / R1;
+  const int R2 = N / R1, NB = pad_stride(N, R2);
   const float2* T = seg_lds + t_off;
   for (int it = threadIdx.x; it < nl * N; it += NT) {
     const int l = it / N, k = it - l * N, k1 = k / R2, k2 = k - k1 * R2;
-    const float2* b = seg_lds + b_off + l * N + k2;
+    const float2* b = seg_lds + b_off + l * NB + k2;
     float2 acc = b[0];
     int e = 0;  // m k1 mod R1
     for (int m = 1; m < R1; ++m) {
       e += k1;
       if (e >= R1) e -= R1;
-      acc = cadd(acc, cmul(b[m * R2], twid(T, e * R2, INV)));
+      acc = cadd(acc, cmul(b[m * (R2 + 1)], twid(T, e * R2, INV)));
     }
     seg_lds[a_off + l * N + k] = acc;
   }
@@ -186,7 +199,7 @@ __device__ void fast_line_transform(int a_off, int b_off, int t_off, int nl, int
     case 8: fast_stage1<8, INV>(a_off, b_off, t_off, nl, N); break;
     default: fast_stage1<16, INV>(a_off, b_off, t_off, nl, N); break;
   }
-  __syncthreads();
+  lds_barrier();
   switch (N / r2) {
 #define MAOMI_R1(R) case R: fast_stage2<R, INV>(a_off, b_off, t_off, nl, N); break;
     MAOMI_R1(1) MAOMI_R1(2) MAOMI_R1(3) MAOMI_R1(4) MAOMI_R1(5) MAOMI_R1(6) MAOMI_R1(7)
@@ -195,7 +208,7 @@ __device__ void fast_line_transform(int a_off, int b_off, int t_off, int nl, int
 #undef MAOMI_R1
     default: wide_stage2<INV>(a_off, b_off, t_off, nl, N, N / r2); break;
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Transform nl lines of A through B with the fast split when N has one, else line_dft.
@@ -250,8 +263,13 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
   const int first_saved = top ? a.cut : 0;
   const int nsaved = top ? S + 1 - a.cut : S + 1;
   double wsum = 0.0;
-  if ((top || bot) && a.tails)
+  if ((top || bot) && a.tails) {
     for (int i = threadIdx.x; i < a.n2 * a.n2; i += NT) tail[i] = 0.f;
+    __syncthreads();  // other threads accumulate into these pixels
+  }
+  // element it = l * N + y of a tile, stepped by NT without divisions
+  const int tl0 = threadIdx.x / N, ty0 = threadIdx.x - tl0 * N;
+  const int dl = NT / N, dy = NT - dl * N;
   for (int s = 0; s <= S; ++s) {
     const bool first = s == 0, last = s == S;
     const bool save_tail = a.tails && (top || bot) && s >= first_saved;
@@ -273,14 +291,19 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
           A[l * N + y] = (xs >= 0 && xs < src_n && ys >= 0 && ys < src_n)
                              ? src[(int64_t)xs * src_n + ys] : make_float2(0.f, 0.f);
         }
-        __syncthreads();
+        lds_barrier();
       } else {
         for (int it = threadIdx.x; it < nl * N; it += NT) A[it] = fld[(int64_t)r0 * N + it];
-        __syncthreads();
+        lds_barrier();
         line_transform(A, B, nl, N, N1, N2, T, true);
       }
-      for (int it = threadIdx.x; it < nl * N; it += NT) {
-        const int l = it / N, y = it % N, x = r0 + l;
+      int l = tl0, y = ty0;
+      for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, y += dy) {
+        if (y >= N) {
+          y -= N;
+          ++l;
+        }
+        const int x = r0 + l;
         float2 v = A[it];
         if (!last) {
           const float phi = g.phase[((int64_t)s * g.H + ox + x) * g.W + oy + y];
@@ -305,11 +328,11 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         if (last && dst) dst[(int64_t)x * N + y] = v;
         A[it] = v;
       }
-      __syncthreads();
+      lds_barrier();
       if (!last) {
         line_transform(A, B, nl, N, N1, N2, T, false);
         for (int it = threadIdx.x; it < nl * N; it += NT) fld[(int64_t)r0 * N + it] = A[it];
-        __syncthreads();
+        lds_barrier();  // A is refilled by the next tile
       }
     }
     if (save_tail) {
@@ -324,27 +347,49 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
                  a.psf + (int64_t)site * a.X * a.Y * b + s, b, a.focal_scale * ax);
     }
     if (last) break;
+    __syncthreads();  // the row pass's field stores are read by other threads below
     // ---------------- column pass
     for (int c0 = 0; c0 < N; c0 += L) {
       const int nl = min(L, N - c0);
-      for (int it = threadIdx.x; it < nl * N; it += NT) {
-        const int i = it / nl, l = it % nl;
-        A[l * N + i] = fld[(int64_t)i * N + c0 + l];
+      const int ti0 = threadIdx.x / nl, tc0 = threadIdx.x - ti0 * nl;
+      const int di = NT / nl, dc = NT - di * nl;
+      {
+        int i = ti0, l = tc0;
+        for (int it = threadIdx.x; it < nl * N; it += NT, i += di, l += dc) {
+          if (l >= nl) {
+            l -= nl;
+            ++i;
+          }
+          A[l * N + i] = fld[(int64_t)i * N + c0 + l];
+        }
       }
-      __syncthreads();
+      lds_barrier();
       line_transform(A, B, nl, N, N1, N2, T, false);
-      for (int it = threadIdx.x; it < nl * N; it += NT) {
-        const int l = it / N, kx = it % N;
-        A[it] = cmul(A[it], cmul(Qt[kx], Qt[c0 + l]));
+      {
+        int l = tl0, kx = ty0;
+        for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, kx += dy) {
+          if (kx >= N) {
+            kx -= N;
+            ++l;
+          }
+          A[it] = cmul(A[it], cmul(Qt[kx], Qt[c0 + l]));
+        }
       }
-      __syncthreads();
+      lds_barrier();
       line_transform(A, B, nl, N, N1, N2, T, true);
-      for (int it = threadIdx.x; it < nl * N; it += NT) {
-        const int i = it / nl, l = it % nl;
-        fld[(int64_t)i * N + c0 + l] = A[l * N + i];
+      {
+        int i = ti0, l = tc0;
+        for (int it = threadIdx.x; it < nl * N; it += NT, i += di, l += dc) {
+          if (l >= nl) {
+            l -= nl;
+            ++i;
+          }
+          fld[(int64_t)i * N + c0 + l] = A[l * N + i];
+        }
       }
-      __syncthreads();
+      lds_barrier();  // A is refilled by the next tile
     }
+    __syncthreads();  // the column pass's field stores are read by other threads next
   }
   if ((top || bot) && a.tails) {
     __syncthreads();
@@ -359,7 +404,7 @@ segmented_sites_kernel(SegArgs a) {
   const int nmax = a.n;
   float2* A = seg_lds;
   float2* B = A + a.L * nmax;
-  float2* T = B + a.L * nmax;
+  float2* T = B + a.L * a.nb;
   float2* Qt = T + nmax;
   float* E = (float*)(Qt + nmax);
   double* red = (double*)(E + nmax + (nmax & 1));
@@ -471,9 +516,18 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
     return MAOMI_EINVAL;
   }
   // two line buffers of L x n complex plus tables, kept under 60 KiB of LDS
-  a.L = (int)((61440 - 16 * n - 4 * n - 96) / (16 * n));
+  // scratch line stride: the padded layout of the fast split (fast_r2 on the device)
+  a.nb = (int)n;
+  for (const int64_t side : {n, n2}) {
+    int64_t r2 = 1;
+    while (r2 < 16 && side % (2 * r2) == 0) r2 *= 2;
+    if (MAOMI_SEG_FAST && r2 >= 2 && side / r2 <= 64 && side + side / r2 > a.nb)
+      a.nb = (int)(side + side / r2);
+  }
+  // line buffers A (L x n) and B (L x nb) complex plus tables, kept under 60 KiB of LDS
+  a.L = (int)((61440 - 16 * n - 4 * n - 96) / (8 * n + 8 * a.nb));
   if (a.L < 1) a.L = 1;
-  const size_t shm = ((size_t)2 * a.L * n + 2 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
+  const size_t shm = ((size_t)a.L * (n + a.nb) + 2 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
                      (NT / 64) * sizeof(double) + 16;
   int64_t grid = d->grid > 0 ? d->grid : d->P;
   if (grid > d->P) grid = d->P;

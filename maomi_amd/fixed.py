@@ -102,7 +102,9 @@ def assemble(config, out, shape, *, mask_mode="matlab_v4", z_positions=None, cut
             tm = masks[..., j + 1]
             tmean = float(tm.mean())
             result[name] = {
-                "convmask": data["convmask"].mean(0), "weight": float(data["weight"].mean()),
+                "convmask": (data["convmask_mean"] if "convmask_mean" in data
+                             else data["convmask"].mean(0)),
+                "weight": float(data["weight"].mean()),
                 "psfZ": interp / interp.mean() if interp.mean() > 0 else interp,
                 "mask_raw": tm, "mask": tm / tmean if tmean > 0 else tm, "mask_mean": tmean,
                 "weights_per_point": data["weight"], "profiles_per_point": data["profile"],
@@ -255,8 +257,10 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
         torch.cuda.synchronize()
         timings["plan"] = time.perf_counter() - t
         t = time.perf_counter()
-        outputs = plan.evaluate(points, batch_size=batch_size, tails=tails)
+        outputs = plan.evaluate(points, batch_size=batch_size, tails=tails, tail_convmask="mean")
         timings["propagation"] = time.perf_counter() - t
+        if plan.last_launch_seconds() is not None:
+            timings["propagation_kernel"] = plan.last_launch_seconds()
         t = time.perf_counter()
         result = assemble(config, outputs, expected[:2], mask_mode=mask_mode,
                           z_positions=plan.z_positions, cut=plan.cut, device=device)

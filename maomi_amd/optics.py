@@ -166,7 +166,7 @@ def propagate(config, raw_fields, points_um, *, uin=None, device="cuda", save_fu
     pts = np.asarray(points_um, float)
     if np.any(pts < 0) or np.any(pts > np.array(config.volume_um[:2])):
         raise ValueError("PSF coordinates outside FOV")
-    data = plan.evaluate(pts, tails=True)
+    data = plan.evaluate(pts, tails=True, tail_convmask="mean")
     shape = tuple(np.round(np.array(config.volume_um[:2]) * config.vres).astype(int))
     res = assemble(config, data, shape, mask_mode="matlab_v4", z_positions=plan.z_positions,
                    cut=plan.cut)

@@ -119,12 +119,18 @@ class PlainStage:
     """One plain optical-stage run on `device` (uniform N and S over all sites)."""
 
     def __init__(self, cfg: PlainConfig, index_volume, *, device="cuda", precision="fp32",
-                 grid=0, sites=None):
+                 grid=0, sites=None, screens="complex"):
+        """screens: "complex" precomputes exp(i phase) once per volume voxel for FP32 runs
+        (the sites kernel then reads 8-byte screens instead of evaluating sin/cos per site
+        and pixel; results are identical); "phase" reads the FP32 phases."""
         if not torch.cuda.is_available():
             raise RuntimeError("PlainStage needs a ROCm GPU (torch.cuda.is_available() is False)")
         _check_sizes(cfg.n)
         if precision not in ("fp32", "mixed"):
             raise ValueError("precision must be fp32 or mixed")
+        if screens not in ("complex", "phase"):
+            raise ValueError("screens must be complex or phase")
+        self.screens = screens
         self.cfg, self.device, self.precision = cfg, torch.device(device), precision
         vol = torch.as_tensor(index_volume, dtype=torch.float32, device=self.device)
         exp = (cfg.padded, cfg.padded, cfg.slices * cfg.voxels_per_slice)
@@ -168,8 +174,13 @@ class PlainStage:
                                           nat.ptr(self.phase16), nat.ptr(self.scale),
                                           nat.ptr(smax), st), "phase prepare")
         check_screen_range(smax)
+        self.screen = None
         if self.precision == "mixed":
             self.phase = None  # only the float16 screens are read by the sites kernel
+        elif self.screens == "complex":
+            self.screen = torch.empty((Z, H, W), dtype=torch.complex64, device=d)
+            nat.check(lib.maomi_screen_exp(nat.ptr(self.phase), Z * H * W, nat.ptr(self.screen),
+                                           st), "screen exp")
         self.tables = SiteTables([cfg.n], self.bp["dx"], self.bp["dz"],
                                  float(self.bp["wvl"]) / float(self.bp["nr"]), d)
         c = _CLASSES[cfg.n]
@@ -205,7 +216,11 @@ class PlainStage:
         desc = nat.SitesDesc()
         desc.bases = self.base.data_ptr()
         desc.base_off = self.base_off.data_ptr()
-        desc.phase = self.phase.data_ptr() if self.phase is not None else None
+        # complex screens unless the launch needs phases (focal-plane output, 256 threads)
+        cplx = (getattr(self, "screen", None) is not None and psf is None
+                and self.block_threads in (0, 512))
+        desc.phase = (self.phase.data_ptr() if self.phase is not None and not cplx else None)
+        desc.screen = self.screen.data_ptr() if cplx else None
         desc.phase16 = self.phase16.data_ptr() if self.phase16 is not None else None
         desc.slice_scale = self.scale.data_ptr() if self.scale is not None else None
         desc.edge = self.tables.edge.data_ptr()

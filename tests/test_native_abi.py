@@ -44,5 +44,7 @@ def test_error_reporting_without_gpu_work():
 
 
 def test_sites_descriptor_layout_matches_header():
-    # 18 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding
-    assert ctypes.sizeof(_native.SitesDesc) == 8 * (18 + 12 + 15)
+    # 19 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding; the complex
+    # screen pointer is the last member
+    assert ctypes.sizeof(_native.SitesDesc) == 8 * (19 + 12 + 15)
+    assert _native.SitesDesc.screen.offset == 8 * (18 + 12 + 15)

@@ -99,3 +99,24 @@ def test_phase_prepare_matches_numpy(gpu_lib, X, Y, Zv, vps):
     got = ph.cpu().numpy()
     np.testing.assert_allclose(got, want, rtol=2e-7, atol=1e-7)
     np.testing.assert_array_equal(smax.cpu().numpy(), np.abs(got).reshape(Z, -1).max(1))
+
+
+@pytest.mark.parametrize("n,slices,fov", [(128, 16, 64), (256, 8, 64), (512, 4, 32)])
+def test_complex_screens_equal_phase_screens(gpu_lib, n, slices, fov):
+    """exp(i phase) precomputed once per voxel gives the per-site sin/cos path's masks."""
+    import torch
+    from maomi_amd.plain import PlainStage
+    from maomi_amd.workloads import PlainConfig, index_volume, site_offsets
+    cfg = PlainConfig(fov=fov, n=n, slices=slices, sites_per_axis=2, corr_vox=2.0,
+                      zernike_std_rad=0.05)
+    vol, offs = index_volume(cfg), site_offsets(cfg)
+    print(f"[complex screens] N={n} S={slices} sites={len(offs)}", flush=True)
+    out = []
+    for screens in ("phase", "complex"):
+        st = PlainStage(cfg, vol, sites=offs, screens=screens).plan()
+        assert (st.screen is not None) == (screens == "complex")
+        st.propagate()
+        torch.cuda.synchronize()
+        out.append((st.masks.cpu().numpy(), st.energies.cpu().numpy()))
+    assert rel_l2(out[1][0], out[0][0]) < 1e-7
+    assert rel_l2(out[1][1], out[0][1]) < 1e-7
