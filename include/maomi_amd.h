@@ -151,6 +151,9 @@ typedef struct maomi_segmented_desc {
   void* work;                     /* grid * work_stride complex64 elements            */
   int64_t work_stride;            /* >= maomi_segmented_workspace_elems(...)          */
   int64_t grid;                   /* persistent blocks (0 = one per site)             */
+  const int32_t* mfx_band;        /* [X][2] first nonzero column and span of each row of */
+  const int32_t* mfy_band;        /*   mfx, mfy, mt (banded resamples: only the span is  */
+  const int32_t* mt_band;         /*   summed), or NULL to sum whole rows                */
 } maomi_segmented_desc;
 
 int maomi_segmented_sites(const maomi_segmented_desc* desc, maomi_stream_t stream);

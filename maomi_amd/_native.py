@@ -52,6 +52,7 @@ class SegmentedDesc(ctypes.Structure):
         ("psf", c_void_p), ("top_conv", c_void_p), ("top_prof", c_void_p), ("top_w", c_void_p),
         ("bot_conv", c_void_p), ("bot_prof", c_void_p), ("bot_w", c_void_p),
         ("work", c_void_p), ("work_stride", c_int64), ("grid", c_int64),
+        ("mfx_band", c_void_p), ("mfy_band", c_void_p), ("mt_band", c_void_p),
     ]
 
 

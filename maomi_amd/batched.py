@@ -44,6 +44,18 @@ def _psf_stats_device(psf):
     return energy, sums
 
 
+def _row_bands(m):
+    """[rows][2] int32 (first nonzero column, span to the last nonzero) of a device matrix,
+    (0, 0) for an all-zero row: the resample kernels sum only the span."""
+    a = m.detach().cpu().numpy()
+    out = np.zeros((a.shape[0], 2), np.int32)
+    for i, row in enumerate(a):
+        nz = np.flatnonzero(row)
+        if len(nz):
+            out[i] = nz[0], nz[-1] - nz[0] + 1
+    return torch.as_tensor(out, device=m.device)
+
+
 def psf_stats(psf):
     """FP64 per-site energies and per-voxel sums over sites of |x|, x, min(x, 0) of device PSFs
     [P][...] (one pass on the GPU; assemble uses them instead of host reductions)."""
@@ -158,6 +170,7 @@ class OpticalPlan:
         self.mfx = torch.as_tensor(resample_matrix(fx, ss), dtype=torch.float32, device=dev)
         self.mfy = torch.as_tensor(resample_matrix(fy, ss), dtype=torch.float32, device=dev)
         self.mt = torch.as_tensor(resample_matrix(n2, ss), dtype=torch.float32, device=dev)
+        self.bands = [_row_bands(m) for m in (self.mfx, self.mfy, self.mt)]
         self.scale = ss ** 4 * (r * (1e6 * g.wvl) ** 1.5) / (np.pi * g.nr)
         self.cache = {}
         self.paths_evaluated = 0
@@ -221,6 +234,7 @@ class OpticalPlan:
         desc.fx, desc.fy = self.fshape
         desc.X, desc.Y, desc.TT = self.mfx.shape[0], self.mfy.shape[0], self.mt.shape[0]
         desc.mfx, desc.mfy, desc.mt = self.mfx.data_ptr(), self.mfy.data_ptr(), self.mt.data_ptr()
+        desc.mfx_band, desc.mfy_band, desc.mt_band = (b.data_ptr() for b in self.bands)
         desc.ax_focus = self.axial["focus"].data_ptr()
         desc.ax_top = self.axial["top"].data_ptr()
         desc.ax_bot = self.axial["bottom"].data_ptr()

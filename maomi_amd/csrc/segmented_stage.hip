@@ -56,6 +56,9 @@ struct SegArgs {
   const float* mfx;          // [X][fx]
   const float* mfy;          // [Y][fy]
   const float* mt;           // [TT][n2]
+  const int32_t* bfx;        // [X][2] band (first column, span) of mfx rows, or null
+  const int32_t* bfy;        // [Y][2]
+  const int32_t* bt;         // [TT][2]
   const float* ax_focus;     // [b]
   const float* ax_top;       // [a + 1 - cut]
   const float* ax_bot;       // [c + 1]
@@ -226,20 +229,26 @@ __device__ __forceinline__ void line_transform(float2* A, float2* B, int nl, int
   }
 }
 
-// out[x][y] = scale * sum_i mx[x][i] sum_j my[y][j] in[i][j]  (in: R x C, tmp: X x C)
-__device__ void resample2d(const float* in, int R, int C, const float* mx, int X, const float* my,
-                           int Y, float* tmp, float* out, int64_t ostride, float scale) {
+// out[x][y] = scale * sum_i mx[x][i] sum_j my[y][j] in[i][j]  (in: R x C, tmp: X x C).
+// With bands (bx[x] = first nonzero column of row x of mx and the span holding all its
+// nonzeros; likewise by), only the span is summed: the same nonzero terms in the same order,
+// so the result equals the full-row sum.
+__device__ void resample2d(const float* in, int R, int C, const float* mx, const int32_t* bx,
+                           int X, const float* my, const int32_t* by, int Y, float* tmp,
+                           float* out, int64_t ostride, float scale) {
   for (int it = threadIdx.x; it < X * C; it += NT) {
     const int x = it / C, j = it % C;
+    const int i0 = bx ? bx[2 * x] : 0, i1 = bx ? i0 + bx[2 * x + 1] : R;
     float acc = 0.f;
-    for (int i = 0; i < R; ++i) acc += mx[x * R + i] * in[i * C + j];
+    for (int i = i0; i < i1; ++i) acc += mx[x * R + i] * in[i * C + j];
     tmp[it] = acc;
   }
   __syncthreads();
   for (int it = threadIdx.x; it < X * Y; it += NT) {
     const int x = it / Y, y = it % Y;
+    const int j0 = by ? by[2 * y] : 0, j1 = by ? j0 + by[2 * y + 1] : C;
     float acc = 0.f;
-    for (int j = 0; j < C; ++j) acc += my[y * C + j] * tmp[x * C + j];
+    for (int j = j0; j < j1; ++j) acc += my[y * C + j] * tmp[x * C + j];
     out[(int64_t)it * ostride] = scale * acc;
   }
   __syncthreads();
@@ -343,7 +352,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
     if (save_foc) {
       __syncthreads();
       const int b = a.seg[1].steps;
-      resample2d(stage, a.fx, a.fy, a.mfx, a.X, a.mfy, a.Y, tmp,
+      resample2d(stage, a.fx, a.fy, a.mfx, a.bfx, a.X, a.mfy, a.bfy, a.Y, tmp,
                  a.psf + (int64_t)site * a.X * a.Y * b + s, b, a.focal_scale * ax);
     }
     if (last) break;
@@ -394,7 +403,8 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
   if ((top || bot) && a.tails) {
     __syncthreads();
     float* conv = (top ? a.top_conv : a.bot_conv) + (int64_t)site * a.TT * a.TT;
-    resample2d(tail, a.n2, a.n2, a.mt, a.TT, a.mt, a.TT, tmp, conv, 1, a.tail_scale);
+    resample2d(tail, a.n2, a.n2, a.mt, a.bt, a.TT, a.mt, a.bt, a.TT, tmp, conv, 1,
+               a.tail_scale);
     if (threadIdx.x == 0) (top ? a.top_w : a.bot_w)[site] = wsum * a.prop_um;
   }
 }
@@ -506,6 +516,7 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
   a.fx = (int)d->fx; a.fy = (int)d->fy; a.foff_x = (int)((n2 - d->fx) / 2);
   a.foff_y = (int)((n2 - d->fy) / 2); a.X = (int)d->X; a.Y = (int)d->Y; a.TT = (int)d->TT;
   a.mfx = d->mfx; a.mfy = d->mfy; a.mt = d->mt; a.ax_focus = d->ax_focus; a.ax_top = d->ax_top;
+  a.bfx = d->mfx_band; a.bfy = d->mfy_band; a.bt = d->mt_band;
   a.ax_bot = d->ax_bot; a.ss2 = (float)d->ss2; a.focal_scale = (float)d->focal_scale;
   a.tail_scale = (float)d->tail_scale; a.prop_um = d->prop_um;
   a.psf = d->psf; a.top_conv = d->top_conv; a.top_prof = d->top_prof; a.top_w = d->top_w;
