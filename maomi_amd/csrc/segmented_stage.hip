@@ -79,6 +79,15 @@ struct SegArgs {
   int nb;                    // line stride of the scratch buffer B (>= n, see fast_stage1)
 };
 
+// acc + x * w as four FMAs (a multiply-add chain per output of the direct DFTs)
+__device__ __forceinline__ float2 cmac(float2 acc, float2 x, float2 w) {
+  acc.x = fmaf(x.x, w.x, acc.x);
+  acc.x = fmaf(-x.y, w.y, acc.x);
+  acc.y = fmaf(x.x, w.y, acc.y);
+  acc.y = fmaf(x.y, w.x, acc.y);
+  return acc;
+}
+
 __device__ __forceinline__ float2 twid(const float2* T, int j, bool inv) {
   const float2 w = T[j];
   return inv ? make_float2(w.x, -w.y) : w;
@@ -123,18 +132,65 @@ __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl
   for (int job = threadIdx.x; job < nl * R1; job += NT) {
     const int l = job / R1, n1 = job - l * R1;
     const float2* a = seg_lds + a_off + l * N + n1;
-    float2 v[R2];
+    float2 v[R2], w[R2];
 #pragma unroll
     for (int m = 0; m < R2; ++m) v[m] = a[R1 * m];
+    // W_N^(n1 k2): n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction; read before any store
+#pragma unroll
+    for (int k2 = 1; k2 < R2; ++k2) w[k2] = twid(T, n1 * k2, INV);
     fftk::fft<R2, INV>(v);
     float2* b = seg_lds + b_off + l * NB + n1 * (R2 + 1);
     b[0] = v[0];
-    int idx = 0;  // n1 k2 mod N
 #pragma unroll
-    for (int k2 = 1; k2 < R2; ++k2) {
-      idx += n1;
-      if (idx >= N) idx -= N;
-      b[k2] = cmul(v[k2], twid(T, idx, INV));
+    for (int k2 = 1; k2 < R2; ++k2) b[k2] = cmul(v[k2], w[k2]);
+  }
+}
+
+__host__ __device__ constexpr int smallest_factor(int n, int f = 2) {
+  return f * f > n ? n : (n % f == 0 ? f : smallest_factor(n, f + 1));
+}
+
+// X[k] = sum_m x[m] W_R^(m k) with w[c] = W_R^c.  A composite R = P Q (P its smallest prime
+// factor) runs as P-point DFTs over m = Q n1 + n2, the twiddle W_R^(n2 k1), then Q-point DFTs,
+// X[k1 + P k2] (each level direct; P = 2 as add/subtract); a prime R is one direct DFT.
+template <int R>
+__device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w)[R],
+                                          float2 (&X)[R]) {
+  constexpr int P = smallest_factor(R), Q = R / P;
+  if constexpr (P == R) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      float2 acc = x[0];
+#pragma unroll
+      for (int m = 1; m < R; ++m) acc = cmac(acc, x[m], w[(m * k) % R]);
+      X[k] = acc;
+    }
+  } else {
+    float2 y[R];  // y[n2 P + k1]
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        float2 acc;
+        if constexpr (P == 2) {
+          acc = k1 == 0 ? cadd(x[n2], x[Q + n2]) : csub(x[n2], x[Q + n2]);
+        } else {
+          acc = x[n2];
+#pragma unroll
+          for (int n1 = 1; n1 < P; ++n1) acc = cmac(acc, x[Q * n1 + n2], w[((n1 * k1) % P) * Q]);
+        }
+        y[n2 * P + k1] = (n2 * k1) % R == 0 ? acc : cmul(acc, w[(n2 * k1) % R]);
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) {
+        float2 acc = y[k1];
+#pragma unroll
+        for (int n2 = 1; n2 < Q; ++n2) acc = cmac(acc, y[n2 * P + k1], w[((n2 * k2) % Q) * P]);
+        X[k1 + P * k2] = acc;
+      }
     }
   }
 }
@@ -152,13 +208,10 @@ __device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl
 #pragma unroll
     for (int m = 0; m < R1; ++m) x[m] = b[m * (R2 + 1)];
     float2* a = seg_lds + a_off + l * N + k2;
+    float2 X[R1];
+    small_dft<R1>(x, w, X);
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      float2 acc = x[0];
-#pragma unroll
-      for (int m = 1; m < R1; ++m) acc = cadd(acc, cmul(x[m], w[(m * k1) % R1]));
-      a[k1 * R2] = acc;
-    }
+    for (int k1 = 0; k1 < R1; ++k1) a[k1 * R2] = X[k1];
   }
 }
 
@@ -176,7 +229,7 @@ __device__ __noinline__ void wide_stage2(int a_off, int b_off, int t_off, int nl
     for (int m = 1; m < R1; ++m) {
       e += k1;
       if (e >= R1) e -= R1;
-      acc = cadd(acc, cmul(b[m * (R2 + 1)], twid(T, e * R2, INV)));
+      acc = cmac(acc, b[m * (R2 + 1)], twid(T, e * R2, INV));
     }
     seg_lds[a_off + l * N + k] = acc;
   }
