@@ -22,6 +22,10 @@ extern __shared__ float2 seg_lds[];
 namespace {
 
 constexpr int NT = 256;
+// Tile loads are issued in batches of kBatch elements per thread, each batch loaded into
+// registers before any of it is stored, so a batch waits out one memory latency instead of
+// one per element (larger batches run the kernel out of registers).
+constexpr int kBatch = 8;
 
 // Workgroup barrier that orders LDS only: global loads and stores stay in flight.  Full
 // __syncthreads() is kept where one thread reads global workspace another thread wrote.
@@ -355,9 +359,41 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         }
         lds_barrier();
       } else {
-        for (int it = threadIdx.x; it < nl * N; it += NT) A[it] = fld[(int64_t)r0 * N + it];
+        const float2* f = fld + (int64_t)r0 * N;
+        for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+          float2 ld[kBatch];
+#pragma unroll
+          for (int k = 0; k < kBatch; ++k)
+            if (b0 + k * NT < nl * N) ld[k] = f[b0 + k * NT];
+#pragma unroll
+          for (int k = 0; k < kBatch; ++k)
+            if (b0 + k * NT < nl * N) A[b0 + k * NT] = ld[k];
+        }
         lds_barrier();
         line_transform(A, B, nl, N, N1, N2, T, true);
+      }
+      // the tile's phases, batched the same way, wait in B (free between the transforms) at
+      // the indices of their elements; each thread reads back only its own
+      float* PH = (float*)B;
+      if (!last) {
+        int l = tl0, y = ty0;
+        for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+          float ph[kBatch];
+#pragma unroll
+          for (int k = 0; k < kBatch; ++k) {
+            if (b0 + k * NT < nl * N)
+              ph[k] = g.phase[((int64_t)s * g.H + ox + r0 + l) * g.W + oy + y];
+            l += dl;
+            y += dy;
+            if (y >= N) {
+              y -= N;
+              ++l;
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < kBatch; ++k)
+            if (b0 + k * NT < nl * N) PH[b0 + k * NT] = ph[k];
+        }
       }
       int l = tl0, y = ty0;
       for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, y += dy) {
@@ -368,7 +404,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         const int x = r0 + l;
         float2 v = A[it];
         if (!last) {
-          const float phi = g.phase[((int64_t)s * g.H + ox + x) * g.W + oy + y];
+          const float phi = PH[it];
           float sn, cs;
           trig::sincos_screen(phi, &sn, &cs);  // |phi| < 2^17, checked by the host
           const float sg = E[x] * E[y];
@@ -417,12 +453,25 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
       const int di = NT / nl, dc = NT - di * nl;
       {
         int i = ti0, l = tc0;
-        for (int it = threadIdx.x; it < nl * N; it += NT, i += di, l += dc) {
-          if (l >= nl) {
-            l -= nl;
-            ++i;
+        for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+          float2 ld[kBatch];
+          int dst[kBatch];
+#pragma unroll
+          for (int k = 0; k < kBatch; ++k) {
+            if (b0 + k * NT < nl * N) {
+              ld[k] = fld[(int64_t)i * N + c0 + l];
+              dst[k] = l * N + i;
+            }
+            i += di;
+            l += dc;
+            if (l >= nl) {
+              l -= nl;
+              ++i;
+            }
           }
-          A[l * N + i] = fld[(int64_t)i * N + c0 + l];
+#pragma unroll
+          for (int k = 0; k < kBatch; ++k)
+            if (b0 + k * NT < nl * N) A[dst[k]] = ld[k];
         }
       }
       lds_barrier();
