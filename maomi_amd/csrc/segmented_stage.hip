@@ -322,7 +322,11 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
                             float* stage, float* tmp, int site, float2* A, float2* B, float2* T,
                             float2* Qt, float* E, double* red) {
   const Seg& g = a.seg[k];
-  const int N = g.n, N1 = g.n1, N2 = g.n2f, S = g.steps, L = a.L;
+  const int N = g.n, N1 = g.n1, N2 = g.n2f, S = g.steps;
+  // lines per tile for this side: as many as the line buffers (sized for the largest side)
+  // hold, so that smaller fields keep the block's threads busy
+  const int r2s = fast_r2(N), NBs = r2s ? pad_stride(N, r2s) : N;
+  const int L = max(1, min(a.L * a.n / N, a.L * a.nb / NBs));
   for (int i = threadIdx.x; i < N; i += NT) { T[i] = g.tw[i]; E[i] = g.edge[i]; }
   __syncthreads();
   const bool top = k == 0, bot = k == 2, foc = k == 1;
