@@ -557,23 +557,43 @@ __global__ void psf_site_sum_kernel(const float* __restrict__ psf, int64_t K,
   if (threadIdx.x == 0) energy[blockIdx.x] = t;
 }
 
-// ... and per-voxel FP64 sums over sites of |x|, x and min(x, 0) (consecutive threads read
-// consecutive voxels of one site).
-__global__ void psf_voxel_sums_kernel(const float* __restrict__ psf, int64_t P, int64_t K,
-                                      double* __restrict__ sabs, double* __restrict__ ssum,
-                                      double* __restrict__ sneg) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    double a = 0.0, v = 0.0, n = 0.0;
-    for (int64_t p = 0; p < P; ++p) {
-      const double x = psf[p * K + k];
-      a += fabs(x);
-      v += x;
-      n += x < 0.0 ? x : 0.0;
+// ... and per-voxel FP64 sums over sites of |x|, x and min(x, 0).  A block takes kVox
+// consecutive voxels; its NT / kVox site lanes each sum every (NT / kVox)-th site of one
+// voxel (consecutive threads read consecutive voxels of one site), and the lanes' partial
+// sums are added in lane order, so the result does not depend on scheduling.
+constexpr int kVox = 16;
+
+__global__ __launch_bounds__(NT) void psf_voxel_sums_kernel(const float* __restrict__ psf,
+                                                            int64_t P, int64_t K,
+                                                            double* __restrict__ sabs,
+                                                            double* __restrict__ ssum,
+                                                            double* __restrict__ sneg) {
+  constexpr int LANES = NT / kVox;
+  __shared__ double part[3][LANES][kVox];
+  const int v = threadIdx.x % kVox, j = threadIdx.x / kVox;
+  for (int64_t k0 = (int64_t)blockIdx.x * kVox; k0 < K; k0 += (int64_t)gridDim.x * kVox) {
+    const int64_t k = k0 + v;
+    double a = 0.0, s = 0.0, n = 0.0;
+    if (k < K) {
+#pragma unroll 4
+      for (int64_t p = j; p < P; p += LANES) {
+        const double x = psf[p * K + k];
+        a += fabs(x);
+        s += x;
+        n += x < 0.0 ? x : 0.0;
+      }
     }
-    sabs[k] = a;
-    ssum[k] = v;
-    sneg[k] = n;
+    part[0][j][v] = a;
+    part[1][j][v] = s;
+    part[2][j][v] = n;
+    __syncthreads();
+    if (threadIdx.x < 3 * kVox) {
+      const int c = threadIdx.x / kVox, w = threadIdx.x % kVox;
+      double t = 0.0;
+      for (int l = 0; l < LANES; ++l) t += part[c][l][w];
+      if (k0 + w < K) (c == 0 ? sabs : c == 1 ? ssum : sneg)[k0 + w] = t;
+    }
+    __syncthreads();
   }
 }
 
@@ -664,9 +684,9 @@ MAOMI_API int maomi_psf_stats(const float* psf, int64_t P, int64_t K, double* en
   hipLaunchKernelGGL(psf_site_sum_kernel, dim3((unsigned)P), dim3(NT), 0, (hipStream_t)stream,
                      psf, K, energy);
   MAOMI_CHECK_LAUNCH();
-  const int64_t blocks = (K + 255) / 256;
+  const int64_t blocks = (K + kVox - 1) / kVox;
   hipLaunchKernelGGL(psf_voxel_sums_kernel, dim3((unsigned)(blocks < 65535 ? blocks : 65535)),
-                     dim3(256), 0, (hipStream_t)stream, psf, P, K, sum_abs, sum, sum_neg);
+                     dim3(NT), 0, (hipStream_t)stream, psf, P, K, sum_abs, sum, sum_neg);
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

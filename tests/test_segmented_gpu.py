@@ -124,3 +124,19 @@ def test_zstack_matches_oracle(gpu_lib):
         want = O.zstack(v, p, m, c)
         assert got.dtype == np.float32 and got.shape == v.shape
         assert rel_l2(got, want) < 1e-6
+
+
+@pytest.mark.parametrize("P,K", [(1, 7), (37, 1000), (300, 5184)])
+def test_psf_stats_match_numpy(gpu_lib, P, K):
+    """FP64 per-site energies and per-voxel sums of |x|, x and min(x, 0) over sites."""
+    import torch
+    from maomi_amd.batched import psf_stats
+    rng = np.random.default_rng(P + K)
+    a = (rng.standard_normal((P, K)) + 0.3).astype(np.float32)
+    print(f"[psf stats] P={P} K={K}", flush=True)
+    got = psf_stats(torch.as_tensor(a, device="cuda"))
+    x = a.astype(np.float64)
+    np.testing.assert_allclose(got["energies"], x.sum(1), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(got["sum_abs"], np.abs(x).sum(0), rtol=1e-12)
+    np.testing.assert_allclose(got["sum"], x.sum(0), rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(got["sum_neg"], np.minimum(x, 0).sum(0), rtol=1e-12, atol=1e-12)
