@@ -119,37 +119,6 @@ __device__ void line_dft(float2* A, float2* B, int nl, int N, int N1, int N2, co
   lds_barrier();
 }
 
-// Fast line transform for N = R1 * R2 with R2 the power-of-two part of N (at most 16) and
-// R1 <= 64: stage 1 is an R2-point register FFT per (line, n1) job followed by the twiddle
-// W_N^(n1 k2); stage 2 an R1-point direct DFT per (line, k2) job whose R1 twiddles are held
-// in registers (R1 <= 16) or per output from LDS (wide_stage2).  X[k2 + R2 k1] lands in
-// natural order.  Lines live at LDS offsets a_off (in and out, line stride N) and b_off
-// (scratch); the twiddle table W_N^j at t_off.  In the scratch buffer, element (n1, k2) of a
-// line sits at n1 (R2 + 1) + k2 and lines are pad_stride(N) apart: the padding keeps the
-// stage-1 stores of neighbouring jobs (R2 apart otherwise) off the same LDS banks.
-__device__ __forceinline__ int pad_stride(int N, int R2) { return N + N / R2; }
-
-template <int R2, bool INV>
-__device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl, int N) {
-  const int R1 = N / R2, NB = pad_stride(N, R2);
-  const float2* T = seg_lds + t_off;
-  for (int job = threadIdx.x; job < nl * R1; job += NT) {
-    const int l = job / R1, n1 = job - l * R1;
-    const float2* a = seg_lds + a_off + l * N + n1;
-    float2 v[R2], w[R2];
-#pragma unroll
-    for (int m = 0; m < R2; ++m) v[m] = a[R1 * m];
-    // W_N^(n1 k2): n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction; read before any store
-#pragma unroll
-    for (int k2 = 1; k2 < R2; ++k2) w[k2] = twid(T, n1 * k2, INV);
-    fftk::fft<R2, INV>(v);
-    float2* b = seg_lds + b_off + l * NB + n1 * (R2 + 1);
-    b[0] = v[0];
-#pragma unroll
-    for (int k2 = 1; k2 < R2; ++k2) b[k2] = cmul(v[k2], w[k2]);
-  }
-}
-
 __host__ __device__ constexpr int smallest_factor(int n, int f = 2) {
   return f * f > n ? n : (n % f == 0 ? f : smallest_factor(n, f + 1));
 }
@@ -199,6 +168,63 @@ __device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w
   }
 }
 
+// Fast line transform for N = R1 * R2 (the split of fast_split: R2 <= 32, R1 <= 64): stage 1
+// is an R2-point DFT per (line, n1) job (a register FFT when R2 is a power of two, else the
+// factorised small_dft) followed by the twiddle W_N^(n1 k2); stage 2 an R1-point DFT per
+// (line, k2) job in registers (R1 <= 32) or per output from LDS (wide_stage2).
+// X[k2 + R2 k1] lands in natural order.  Lines live at LDS offsets a_off (in and out, line
+// stride N) and b_off (scratch); the twiddle table W_N^j at t_off.  In the scratch buffer,
+// element (n1, k2) of a line sits at n1 (R2 + 1) + k2 and lines are pad_stride(N) apart: the
+// padding keeps the stage-1 stores of neighbouring jobs (R2 apart otherwise) off the same
+// LDS banks.
+__host__ __device__ inline int pad_stride(int N, int R2) { return N + N / R2; }
+
+// R2 of the fast split of N, 0 when there is none (the generic two-level DFT runs):
+// the power-of-two part (<= 16) when the rest is at most 32 points; else the largest factor
+// R2 <= 24 leaving at most 32; else the power-of-two part when the rest is at most 64.
+__host__ __device__ inline int fast_split(int N) {
+  int r2 = 1;
+  while (r2 < 16 && N % (2 * r2) == 0) r2 *= 2;
+  if (r2 >= 2 && N / r2 <= 32) return r2;
+  for (int f = 24; f >= 2; --f)
+    if (N % f == 0 && N / f <= 32) return f;
+  return (r2 >= 2 && N / r2 <= 64) ? r2 : 0;
+}
+
+template <int R2, bool INV>
+__device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl, int N) {
+  constexpr bool POW2 = (R2 & (R2 - 1)) == 0;
+  const int R1 = N / R2, NB = pad_stride(N, R2);
+  const float2* T = seg_lds + t_off;
+  float2 wr[POW2 ? 1 : R2];  // W_R2^c = W_N^(c R1) for the factorised DFT
+  if constexpr (!POW2) {
+#pragma unroll
+    for (int c = 0; c < R2; ++c) wr[c] = twid(T, c * R1, INV);
+  }
+  for (int job = threadIdx.x; job < nl * R1; job += NT) {
+    const int l = job / R1, n1 = job - l * R1;
+    const float2* a = seg_lds + a_off + l * N + n1;
+    float2 v[R2], w[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = a[R1 * m];
+    // W_N^(n1 k2): n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction; read before any store
+#pragma unroll
+    for (int k2 = 1; k2 < R2; ++k2) w[k2] = twid(T, n1 * k2, INV);
+    if constexpr (POW2) {
+      fftk::fft<R2, INV>(v);
+    } else {
+      float2 X[R2];
+      small_dft<R2>(v, wr, X);
+#pragma unroll
+      for (int k = 0; k < R2; ++k) v[k] = X[k];
+    }
+    float2* b = seg_lds + b_off + l * NB + n1 * (R2 + 1);
+    b[0] = v[0];
+#pragma unroll
+    for (int k2 = 1; k2 < R2; ++k2) b[k2] = cmul(v[k2], w[k2]);
+  }
+}
+
 template <int R1, bool INV>
 __device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl, int N) {
   const int R2 = N / R1, NB = pad_stride(N, R2);
@@ -243,44 +269,91 @@ __device__ __noinline__ void wide_stage2(int a_off, int b_off, int t_off, int nl
 #define MAOMI_SEG_FAST 1
 #endif
 
-// R2 of the fast split of N (0: no fast split, the generic two-level DFT runs).
-__device__ __forceinline__ int fast_r2(int N) {
-  if (!MAOMI_SEG_FAST) return 0;
-  int r2 = 1;
-  while (r2 < 16 && N % (2 * r2) == 0) r2 *= 2;  // 16 points keep the kernel at 2 waves/SIMD
-  return (r2 >= 2 && N / r2 <= 64) ? r2 : 0;
+__device__ __forceinline__ int fast_r2(int N) { return MAOMI_SEG_FAST ? fast_split(N) : 0; }
+
+#define MAOMI_SEG_SIZES(X)                                                                 \
+  X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)  \
+  X(18) X(19) X(20) X(21) X(22) X(23) X(24)
+#define MAOMI_SEG_SIZES_R1(X) MAOMI_SEG_SIZES(X) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
+
+// Line splits fall into three classes: 0, a power of two <= 16 times at most 16 (the common
+// field sides); 1, both factors at most 16; 2, the rest.  Each class has a kernel of its own
+// (CLS) that contains only the dispatch of classes up to CLS, so that the registers the
+// larger instantiations clobber do not weigh on the call sites of the common sizes.
+__host__ __device__ inline int split_class(int N, int r2) {
+  const int r1 = N / r2;
+  if (r2 <= 16 && (r2 & (r2 - 1)) == 0 && r1 <= 16) return 0;
+  return (r2 <= 16 && r1 <= 16) ? 1 : 2;
 }
 
-template <bool INV>
+template <bool INV, int CLS>
 __device__ void fast_line_transform(int a_off, int b_off, int t_off, int nl, int N, int r2) {
-  switch (r2) {
-    case 2: fast_stage1<2, INV>(a_off, b_off, t_off, nl, N); break;
-    case 4: fast_stage1<4, INV>(a_off, b_off, t_off, nl, N); break;
-    case 8: fast_stage1<8, INV>(a_off, b_off, t_off, nl, N); break;
-    default: fast_stage1<16, INV>(a_off, b_off, t_off, nl, N); break;
-  }
-  lds_barrier();
-  switch (N / r2) {
+  const int r1 = N / r2, c = split_class(N, r2);
+  if (CLS == 0 || c == 0) {
+    switch (r2) {
+      case 2: fast_stage1<2, INV>(a_off, b_off, t_off, nl, N); break;
+      case 4: fast_stage1<4, INV>(a_off, b_off, t_off, nl, N); break;
+      case 8: fast_stage1<8, INV>(a_off, b_off, t_off, nl, N); break;
+      default: fast_stage1<16, INV>(a_off, b_off, t_off, nl, N); break;
+    }
+    lds_barrier();
+    switch (r1) {
 #define MAOMI_R1(R) case R: fast_stage2<R, INV>(a_off, b_off, t_off, nl, N); break;
-    MAOMI_R1(1) MAOMI_R1(2) MAOMI_R1(3) MAOMI_R1(4) MAOMI_R1(5) MAOMI_R1(6) MAOMI_R1(7)
-    MAOMI_R1(8) MAOMI_R1(9) MAOMI_R1(10) MAOMI_R1(11) MAOMI_R1(12) MAOMI_R1(13) MAOMI_R1(14)
-    MAOMI_R1(15) MAOMI_R1(16)
+      MAOMI_R1(1) MAOMI_R1(2) MAOMI_R1(3) MAOMI_R1(4) MAOMI_R1(5) MAOMI_R1(6) MAOMI_R1(7)
+      MAOMI_R1(8) MAOMI_R1(9) MAOMI_R1(10) MAOMI_R1(11) MAOMI_R1(12) MAOMI_R1(13) MAOMI_R1(14)
+      MAOMI_R1(15) MAOMI_R1(16)
 #undef MAOMI_R1
-    default: wide_stage2<INV>(a_off, b_off, t_off, nl, N, N / r2); break;
+      default: break;
+    }
+  } else if (CLS == 1 || c == 1) {
+    if constexpr (CLS >= 1) {
+      switch (r2) {
+#define MAOMI_R2(R) case R: fast_stage1<R, INV>(a_off, b_off, t_off, nl, N); break;
+        MAOMI_R2(2) MAOMI_R2(3) MAOMI_R2(4) MAOMI_R2(5) MAOMI_R2(6) MAOMI_R2(7) MAOMI_R2(8)
+        MAOMI_R2(9) MAOMI_R2(10) MAOMI_R2(11) MAOMI_R2(12) MAOMI_R2(13) MAOMI_R2(14)
+        MAOMI_R2(15) MAOMI_R2(16)
+#undef MAOMI_R2
+        default: break;
+      }
+      lds_barrier();
+      switch (r1) {
+#define MAOMI_R1(R) case R: fast_stage2<R, INV>(a_off, b_off, t_off, nl, N); break;
+        MAOMI_R1(1) MAOMI_R1(2) MAOMI_R1(3) MAOMI_R1(4) MAOMI_R1(5) MAOMI_R1(6) MAOMI_R1(7)
+        MAOMI_R1(8) MAOMI_R1(9) MAOMI_R1(10) MAOMI_R1(11) MAOMI_R1(12) MAOMI_R1(13) MAOMI_R1(14)
+        MAOMI_R1(15) MAOMI_R1(16)
+#undef MAOMI_R1
+        default: break;
+      }
+    }
+  } else if constexpr (CLS == 2) {
+    switch (r2) {
+#define MAOMI_R2(R) case R: fast_stage1<R, INV>(a_off, b_off, t_off, nl, N); break;
+      MAOMI_SEG_SIZES(MAOMI_R2)
+#undef MAOMI_R2
+      default: break;  // fast_split never returns another size
+    }
+    lds_barrier();
+    switch (r1) {
+#define MAOMI_R1(R) case R: fast_stage2<R, INV>(a_off, b_off, t_off, nl, N); break;
+      MAOMI_R1(1) MAOMI_SEG_SIZES_R1(MAOMI_R1)
+#undef MAOMI_R1
+      default: wide_stage2<INV>(a_off, b_off, t_off, nl, N, r1); break;
+    }
   }
   lds_barrier();
 }
 
 // Transform nl lines of A through B with the fast split when N has one, else line_dft.
+template <int CLS>
 __device__ __forceinline__ void line_transform(float2* A, float2* B, int nl, int N, int N1,
                                                int N2, const float2* T, bool inv) {
   const int r2 = fast_r2(N);
   if (r2) {
     const int a_off = (int)(A - seg_lds), b_off = (int)(B - seg_lds), t_off = (int)(T - seg_lds);
     if (inv)
-      fast_line_transform<true>(a_off, b_off, t_off, nl, N, r2);
+      fast_line_transform<true, CLS>(a_off, b_off, t_off, nl, N, r2);
     else
-      fast_line_transform<false>(a_off, b_off, t_off, nl, N, r2);
+      fast_line_transform<false, CLS>(a_off, b_off, t_off, nl, N, r2);
   } else {
     line_dft(A, B, nl, N, N1, N2, T, inv);
   }
@@ -317,7 +390,8 @@ __device__ __forceinline__ float powk(float p, int order) {
 
 // One segment of one site.  src: first-plane source (complex), with index shift
 // (sx, sy) and side src_n; dst (may be null): receives the last plane (n x n).
-__device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, const float2* src,
+template <int CLS>
+__device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, const float2* src,
                             int src_n, int sx, int sy, float2* fld, float2* dst, float* tail,
                             float* stage, float* tmp, int site, float2* A, float2* B, float2* T,
                             float2* Qt, float* E, double* red) {
@@ -374,7 +448,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
             if (b0 + k * NT < nl * N) A[b0 + k * NT] = ld[k];
         }
         lds_barrier();
-        line_transform(A, B, nl, N, N1, N2, T, true);
+        line_transform<CLS>(A, B, nl, N, N1, N2, T, true);
       }
       // the tile's phases, batched the same way, wait in B (free between the transforms) at
       // the indices of their elements; each thread reads back only its own
@@ -432,7 +506,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
       }
       lds_barrier();
       if (!last) {
-        line_transform(A, B, nl, N, N1, N2, T, false);
+        line_transform<CLS>(A, B, nl, N, N1, N2, T, false);
         for (int it = threadIdx.x; it < nl * N; it += NT) fld[(int64_t)r0 * N + it] = A[it];
         lds_barrier();  // A is refilled by the next tile
       }
@@ -479,7 +553,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         }
       }
       lds_barrier();
-      line_transform(A, B, nl, N, N1, N2, T, false);
+      line_transform<CLS>(A, B, nl, N, N1, N2, T, false);
       {
         int l = tl0, kx = ty0;
         for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, kx += dy) {
@@ -491,7 +565,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
         }
       }
       lds_barrier();
-      line_transform(A, B, nl, N, N1, N2, T, true);
+      line_transform<CLS>(A, B, nl, N, N1, N2, T, true);
       {
         int i = ti0, l = tc0;
         for (int it = threadIdx.x; it < nl * N; it += NT, i += di, l += dc) {
@@ -515,6 +589,7 @@ __device__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, con
   }
 }
 
+template <int CLS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void
 segmented_sites_kernel(SegArgs a) {
   const int nmax = a.n;
@@ -535,13 +610,13 @@ segmented_sites_kernel(SegArgs a) {
     const int site = a.order_list[i];
     const int64_t ox = a.site_x[site], oy = a.site_y[site];
     // A: base -> planes >= cut (top tail), last plane kept whole in xfer
-    run_segment(a, 0, ox, oy, a.base, a.n, 0, 0, fld, xfer, tail, stage, tmp, site, A, B, T, Qt, E,
+    run_segment<CLS>(a, 0, ox, oy, a.base, a.n, 0, 0, fld, xfer, tail, stage, tmp, site, A, B, T, Qt, E,
                 red);
     // B: crop of A's last plane -> focal planes; last plane kept in xfer (n2 x n2)
-    run_segment(a, 1, ox, oy, xfer, a.n, a.coff, a.coff, fld, a.tails ? xfer2 : nullptr, tail,
+    run_segment<CLS>(a, 1, ox, oy, xfer, a.n, a.coff, a.coff, fld, a.tails ? xfer2 : nullptr, tail,
                 stage, tmp, site, A, B, T, Qt, E, red);
     if (a.tails)  // C: B's last plane padded back to n -> bottom tail
-      run_segment(a, 2, ox, oy, xfer2, a.n2, -a.coff, -a.coff, fld, nullptr, tail, stage, tmp, site,
+      run_segment<CLS>(a, 2, ox, oy, xfer2, a.n2, -a.coff, -a.coff, fld, nullptr, tail, stage, tmp, site,
                   A, B, T, Qt, E, red);
   }
 }
@@ -656,10 +731,8 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
   // scratch line stride: the padded layout of the fast split (fast_r2 on the device)
   a.nb = (int)n;
   for (const int64_t side : {n, n2}) {
-    int64_t r2 = 1;
-    while (r2 < 16 && side % (2 * r2) == 0) r2 *= 2;
-    if (MAOMI_SEG_FAST && r2 >= 2 && side / r2 <= 64 && side + side / r2 > a.nb)
-      a.nb = (int)(side + side / r2);
+    const int r2 = MAOMI_SEG_FAST ? fast_split((int)side) : 0;
+    if (r2 && pad_stride((int)side, r2) > a.nb) a.nb = pad_stride((int)side, r2);
   }
   // line buffers A (L x n) and B (L x nb) complex plus tables, kept under 60 KiB of LDS
   a.L = (int)((61440 - 16 * n - 4 * n - 96) / (8 * n + 8 * a.nb));
@@ -668,8 +741,22 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
                      (NT / 64) * sizeof(double) + 16;
   int64_t grid = d->grid > 0 ? d->grid : d->P;
   if (grid > d->P) grid = d->P;
-  hipLaunchKernelGGL(segmented_sites_kernel, dim3((unsigned)grid), dim3(NT), shm,
-                     (hipStream_t)stream, a);
+  // the kernel of the largest line-split class among the two field sides
+  int cls = 0;
+  for (const int64_t side : {n, n2}) {
+    const int r2 = MAOMI_SEG_FAST ? fast_split((int)side) : 0;
+    const int c = r2 ? split_class((int)side, r2) : 2;
+    if (c > cls) cls = c;
+  }
+  if (cls == 0)
+    hipLaunchKernelGGL(segmented_sites_kernel<0>, dim3((unsigned)grid), dim3(NT), shm,
+                       (hipStream_t)stream, a);
+  else if (cls == 1)
+    hipLaunchKernelGGL(segmented_sites_kernel<1>, dim3((unsigned)grid), dim3(NT), shm,
+                       (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL(segmented_sites_kernel<2>, dim3((unsigned)grid), dim3(NT), shm,
+                       (hipStream_t)stream, a);
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }
