@@ -140,3 +140,30 @@ def test_psf_stats_match_numpy(gpu_lib, P, K):
     np.testing.assert_allclose(got["sum_abs"], np.abs(x).sum(0), rtol=1e-12)
     np.testing.assert_allclose(got["sum"], x.sum(0), rtol=1e-12, atol=1e-11)
     np.testing.assert_allclose(got["sum_neg"], np.minimum(x, 0).sum(0), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("kw,sides", [
+    (dict(), (156, 72)),                                                       # 13 x 12 split
+    (dict(volume_um=(8.0, 8.0, 6.0), depth_um=30.0, tail_um=4.0, psf_um=(4.0, 4.0, 4.0)),
+     (148, 36)),                                                               # 4 x 37, LDS stage 2
+    (dict(volume_um=(8.0, 8.0, 6.0), depth_um=40.0, vres=2.0, tail_um=4.0,
+          psf_um=(4.0, 4.0, 4.0)), (384, 72)),                                 # 16 x 24
+    (dict(volume_um=(256.0, 256.0, 50.0), depth_um=100.0, tail_um=20.0), (552, 72)),  # 24 x 23
+], ids=["default", "prime37", "r1_24", "r2_24"])
+def test_line_splits_match_oracle(gpu_lib, kw, sides):
+    """Field sides whose line transforms take the factorised, generic-factor and LDS paths."""
+    from maomi_amd.optics import propagate
+    cfg, ocfg = _cfgs(kw)
+    shapes = O.field_shapes(ocfg)
+    raw = golden_raw_fields(shapes, seed=7)
+    pts = O.locations(ocfg.volume_um, 2)[:2]
+    print(f"[segmented splits] sides {sides} fields {shapes}", flush=True)
+    res = propagate(cfg, raw, pts, device="cuda")
+    from maomi_amd.batched import OpticalPlan
+    assert (OpticalPlan(cfg, raw, device="cuda").n, OpticalPlan(cfg, raw, device="cuda").n2) == sides
+    ref = O.propagate(ocfg, raw, pts)
+    for key in ("psf_3d", "psf_raw", "energies"):
+        assert rel_l2(res[key], ref[key]) < 2e-5, key
+    for t in ("psfT", "psfB"):
+        for key in ("convmask", "weight", "psfZ"):
+            assert rel_l2(res[t][key], ref[t][key]) < 2e-5, (t, key)
