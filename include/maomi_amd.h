@@ -200,6 +200,15 @@ int maomi_fft2_c64(void* u, int64_t B, int64_t N, int64_t inverse, const void* t
 
 /* u <- u / ||u||_2 with the norm accumulated in FP64 (batched.py:51).  dwork: 1 double. */
 int maomi_normalize_l2(void* u, int64_t n, double* dwork, maomi_stream_t stream);
+/* B Fresnel steps of maomi_fresnel_step that differ only in their distance dz_b: beam_params
+ * (device, B x 3 floats) = (1 / (2 dz_b), -pi lambda' dz_b, k (sc - 1) / (2 sc dz_b)), each
+ * rounded to float from double with lambda' = wavelength / nr, k = 2 pi / lambda',
+ * sc = dx1 / dx0; then, when dwork (B doubles) is given, each beam scaled to unit L2 norm
+ * (the base beams of one field side at every depth, batched.py:43-51) */
+int maomi_fresnel_steps(void* u, int64_t B, int64_t N, double dx0, double dx1,
+                        const float* beam_params, double wavelength, double nr,
+                        const float* screen0, const float* screen1, const void* tw,
+                        double* dwork, maomi_stream_t stream);
 
 /* Super-Gaussian edge window of edge_apodization (maomi/_propagation.py:59-61):
  * e[i] = exp(-a_i^16), a_i = (i - N/2)/(0.47 N); e1d (N) and/or e2d = e e^T (N*N). */

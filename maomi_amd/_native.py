@@ -121,6 +121,9 @@ _SIGS = {
                                       c_int64, c_int64, c_int64, c_float, c_float, c_void_p,
                                       c_void_p, c_void_p]),
     "maomi_screen_exp": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "maomi_fresnel_steps": (c_int, [c_void_p, c_int64, c_int64, c_double, c_double, c_void_p,
+                                    c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p]),
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),

@@ -144,8 +144,12 @@ __global__ void zernike_apply_kernel(float2* out, const float* __restrict__ zmap
 // Entry: u <- u * exp(i theta_in) * s0 with
 //   theta_in = ((k T) (1 - sc)) * (1 / (2 dz)),  T = (x dx0)^2 + (y dx0)^2,
 // the order Python evaluates 1j*k*T*(1-sc)/(2dz) on a complex64 tensor.
+// Per-beam parameters of a batch of Fresnel steps with different distances: bp[3 b + 0..2] =
+// (1 / (2 dz), -pi lambda' dz, k (sc - 1) / (2 sc dz)), each rounded to float as the scalar
+// arguments are; null bp uses the scalar arguments for every beam.
 __global__ void entry_kernel(float2* u, int B, int N, float dx0, float kf, float onemsc,
-                             float inv2dz, const float* __restrict__ s0, float pre) {
+                             float inv2dz, const float* __restrict__ s0, float pre,
+                             const float* __restrict__ bp) {
   const int64_t tot = (int64_t)N * N;
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot * B;
        p += (int64_t)gridDim.x * blockDim.x) {
@@ -156,7 +160,8 @@ __global__ void entry_kernel(float2* u, int B, int N, float dx0, float kf, float
     if (onemsc != 0.0f) {
       const float xa = x * dx0, ya = y * dx0;
       const float T = xa * xa + ya * ya;
-      const float th = ((kf * T) * onemsc) * inv2dz;
+      const float i2 = bp ? bp[3 * (p / tot)] : inv2dz;
+      const float th = ((kf * T) * onemsc) * i2;
       float s, c;
       sincosf(th, &s, &c);
       v = cmul(v, make_float2(c, s));
@@ -170,7 +175,7 @@ __global__ void entry_kernel(float2* u, int B, int N, float dx0, float kf, float
 // Multiply by the Fresnel factor in FFT order (ifftshift of the centred grid)
 // and 1/N^2: arg = (c * v) * (1/sc), v = (x/(N dx))^2 + (y/(N dx))^2.
 __global__ void fresnel_q_kernel(float2* u, int B, int N, float cq, float hd, float invsc,
-                                 int scaled, float norm) {
+                                 int scaled, float norm, const float* __restrict__ bp) {
   const int64_t tot = (int64_t)N * N;
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot * B;
        p += (int64_t)gridDim.x * blockDim.x) {
@@ -179,7 +184,7 @@ __global__ void fresnel_q_kernel(float2* u, int B, int N, float cq, float hd, fl
     const int ci = (ki + N / 2) % N, cj = (kj + N / 2) % N;  // centred index
     const float x = (float)(ci - N / 2), y = (float)(cj - N / 2);
     const float xa = x / hd, ya = y / hd;
-    float arg = cq * (xa * xa + ya * ya);
+    float arg = (bp ? bp[3 * (p / tot) + 1] : cq) * (xa * xa + ya * ya);
     if (scaled) arg = arg * invsc;
     float s, c;
     sincosf(arg, &s, &c);
@@ -190,7 +195,8 @@ __global__ void fresnel_q_kernel(float2* u, int B, int N, float cq, float hd, fl
 
 // Exit: u <- u * s1 * exp(i theta_out), theta_out = cexit * ((x dx1)^2 + (y dx1)^2).
 __global__ void exit_kernel(float2* u, int B, int N, float dx1, float cexit,
-                            const float* __restrict__ s1, int chirp) {
+                            const float* __restrict__ s1, int chirp,
+                            const float* __restrict__ bp) {
   const int64_t tot = (int64_t)N * N;
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot * B;
        p += (int64_t)gridDim.x * blockDim.x) {
@@ -200,7 +206,7 @@ __global__ void exit_kernel(float2* u, int B, int N, float dx1, float cexit,
     if (s1) v = cscale(v, s1[q]);
     if (chirp) {
       const float xa = (float)(i - N / 2) * dx1, ya = (float)(j - N / 2) * dx1;
-      const float th = cexit * (xa * xa + ya * ya);
+      const float th = (bp ? bp[3 * (p / tot) + 2] : cexit) * (xa * xa + ya * ya);
       float s, c;
       sincosf(th, &s, &c);
       v = cmul(v, make_float2(c, s));
@@ -219,6 +225,28 @@ __global__ void sumsq_kernel(const float2* __restrict__ u, int64_t n, double* ou
   }
   const double t = block_sum_double(acc, red);
   if (threadIdx.x == 0) atomicAdd(out, t);
+}
+
+// per-beam sums of squares (beam blockIdx.y) and the scaling by their inverse root
+__global__ void sumsq_beams_kernel(const float2* __restrict__ u, int64_t n, double* out) {
+  __shared__ double red[16];
+  const float2* ub = u + (int64_t)blockIdx.y * n;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = ub[i].x, b = ub[i].y;
+    acc += a * a + b * b;
+  }
+  const double t = block_sum_double(acc, red);
+  if (threadIdx.x == 0) atomicAdd(out + blockIdx.y, t);
+}
+
+__global__ void scale_beams_kernel(float2* u, int64_t n, const double* sumsq) {
+  const float inv = 1.0f / (float)sqrt(sumsq[blockIdx.y]);
+  float2* ub = u + (int64_t)blockIdx.y * n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    ub[i] = cscale(ub[i], inv);
 }
 
 __global__ void scale_by_norm_kernel(float2* u, int64_t n, const double* sumsq) {
@@ -297,9 +325,10 @@ MAOMI_API int maomi_pupil_gaussian(void* out, int64_t N, double D1, double rad, 
   return MAOMI_OK;
 }
 
-MAOMI_API int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, double dx1,
-                                 double dz, double wavelength, double nr, const float* screen0,
-                                 const float* screen1, const void* tw, maomi_stream_t stream) {
+static int fresnel_steps(void* u, int64_t B, int64_t N, double dx0, double dx1, double dz,
+                         double wavelength, double nr, const float* screen0,
+                         const float* screen1, const void* tw, const float* beam_params,
+                         maomi_stream_t stream) {
   if (!u || !tw || B < 1 || N < 2 || N % 2 || !(dz > 0) || !(dx0 > 0) || !(dx1 > 0)) {
     maomi_set_error("maomi_fresnel_step: bad arguments (even N, positive dz/dx required)");
     return MAOMI_EINVAL;
@@ -314,7 +343,8 @@ MAOMI_API int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, doub
   const float inv2dz = 1.0f / (float)(2.0 * dz);
   const float invsc0 = 1.0f / (float)sc;
   hipLaunchKernelGGL(entry_kernel, dim3(nblocks(n)), dim3(256), 0, st, f, (int)B, (int)N,
-                     (float)dx0, (float)k, onemsc, inv2dz, screen0, sc != 1.0 ? invsc0 : 1.0f);
+                     (float)dx0, (float)k, onemsc, inv2dz, screen0, sc != 1.0 ? invsc0 : 1.0f,
+                     beam_params);
   MAOMI_CHECK_LAUNCH();
   int rc = maomi_fft2_generic(f, (int)B, (int)N, 0, (const float2*)tw, st);
   if (rc) return rc;
@@ -323,13 +353,47 @@ MAOMI_API int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, doub
   const int scaled = sc != 1.0;
   const float invsc = 1.0f / (float)sc;
   hipLaunchKernelGGL(fresnel_q_kernel, dim3(nblocks(n)), dim3(256), 0, st, f, (int)B, (int)N, cq,
-                     hd, invsc, scaled, (float)(1.0 / ((double)N * (double)N)));
+                     hd, invsc, scaled, (float)(1.0 / ((double)N * (double)N)), beam_params);
   MAOMI_CHECK_LAUNCH();
   rc = maomi_fft2_generic(f, (int)B, (int)N, 1, (const float2*)tw, st);
   if (rc) return rc;
   const double cexit = k / 2.0 * (sc - 1.0) / (sc * dz);
   hipLaunchKernelGGL(exit_kernel, dim3(nblocks(n)), dim3(256), 0, st, f, (int)B, (int)N,
-                     (float)dx1, (float)cexit, screen1, scaled);
+                     (float)dx1, (float)cexit, screen1, scaled, beam_params);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_fresnel_step(void* u, int64_t B, int64_t N, double dx0, double dx1,
+                                 double dz, double wavelength, double nr, const float* screen0,
+                                 const float* screen1, const void* tw, maomi_stream_t stream) {
+  return fresnel_steps(u, B, N, dx0, dx1, dz, wavelength, nr, screen0, screen1, tw, nullptr,
+                       stream);
+}
+
+MAOMI_API int maomi_fresnel_steps(void* u, int64_t B, int64_t N, double dx0, double dx1,
+                                  const float* beam_params, double wavelength, double nr,
+                                  const float* screen0, const float* screen1, const void* tw,
+                                  double* dwork, maomi_stream_t stream) {
+  if (!beam_params || B > 65535) {
+    maomi_set_error("maomi_fresnel_steps: beam parameters required, at most 65535 beams");
+    return MAOMI_EINVAL;
+  }
+  // dz = 1 stands in for the scalar distance: every distance-dependent factor is per beam
+  int rc = fresnel_steps(u, B, N, dx0, dx1, 1.0, wavelength, nr, screen0, screen1, tw,
+                         beam_params, stream);
+  if (rc || !dwork) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = N * N;
+  for (int64_t b0 = 0; b0 < B; b0 += 64) {  // zero the per-beam sums, 64 at a time
+    hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(64), 0, st, dwork + b0,
+                       (int)(B - b0 < 64 ? B - b0 : 64));
+  }
+  const unsigned bx = (unsigned)((n + 255) / 256 < 64 ? (n + 255) / 256 : 64);
+  hipLaunchKernelGGL(sumsq_beams_kernel, dim3(bx, (unsigned)B), dim3(256), 0, st,
+                     (const float2*)u, n, dwork);
+  hipLaunchKernelGGL(scale_beams_kernel, dim3(bx, (unsigned)B), dim3(256), 0, st, (float2*)u, n,
+                     dwork);
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

@@ -78,6 +78,33 @@ def base_beam(u, bp, edge2d, tw, device, *, normalize=True):
     return b
 
 
+def base_beams(u, bp, dz_focus, edge2d, tw, out, device):
+    """base_beam of pupil u for several entry distances at once: out[b] = unit-norm Fresnel step
+    d1 -> dx over dz_focus[b] (batched.py:43-51 per depth), one batched launch sequence."""
+    lib = nat.lib()
+    st = nat.stream_handle(device)
+    n = u.shape[-1]
+    B = len(dz_focus)
+    if tuple(out.shape) != (B, n, n) or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous complex64 (B, n, n) tensor")
+    lam = float(bp["wvl"]) / float(bp["nr"])
+    k = 2.0 * np.pi / lam
+    sc = float(bp["dx"]) / float(bp["d1"])
+    par = np.empty((B, 3), np.float32)
+    for b, dz in enumerate(float(z) for z in dz_focus):
+        # the float roundings of maomi_fresnel_step's scalar arguments
+        par[b] = (np.float32(1.0) / np.float32(2.0 * dz), np.float32(-np.pi * lam * dz),
+                  np.float32(k / 2.0 * (sc - 1.0) / (sc * dz)))
+    par_d = torch.as_tensor(par, device=device)
+    dwork = torch.zeros(B, dtype=torch.float64, device=device)
+    out.copy_(u.expand(B, n, n))
+    nat.check(lib.maomi_fresnel_steps(nat.ptr(out), B, n, float(bp["d1"]), float(bp["dx"]),
+                                      nat.ptr(par_d), float(bp["wvl"]), float(bp["nr"]),
+                                      nat.ptr(edge2d), nat.ptr(edge2d), nat.ptr(tw),
+                                      nat.ptr(dwork), st), "fresnel steps")
+    return out
+
+
 def v4_masks(points, values, shape, device):
     """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153)."""
     lib = nat.lib()

@@ -18,7 +18,8 @@ import numpy as np
 import torch
 
 from . import _native as nat
-from .plain import SiteTables, _CLASSES, _SLOT, base_beam, check_screen_range, make_pupil, v4_masks
+from .plain import (SiteTables, _CLASSES, _SLOT, base_beams, check_screen_range, make_pupil,
+                    v4_masks)
 from .workloads import PlainConfig, beam_parameters, zernike_weights
 
 
@@ -120,25 +121,34 @@ class UnevenStage:
         else:
             for name in ("edge", "q", "tw"):
                 getattr(self.tables, name).copy_(getattr(tables, name))
-        # one base beam per (N, S): focus on the last plane, S dz below the entry plane
+        # one base beam per (N, S): focus on the last plane, S dz below the entry plane.  The
+        # pupil depends on N only; the beams of one N differ in their entry distance, so each
+        # N is one pupil and one batched Fresnel step over its depths.
         keys = sorted({(int(n), int(s)) for n, s in zip(self.sizes, self.slices)})
-        blocks, offs, pos = [], {}, 0
-        for n, s in keys:
-            c = _CLASSES[n]
-            sub = replace(cfg, n=n, slices=s, focus_um=s * cfg.dz_um)
-            bp = dict(beam_parameters(sub), wvl_um=cfg.wavelength_um)
-            pup, _ = make_pupil(n, bp, self.zern, d)
-            tw = self.tables.tw[c * _SLOT:c * _SLOT + n]
-            blocks.append(base_beam(pup, bp, self.tables.edge2d[n], tw, d).reshape(-1))
-            offs[(n, s)] = pos
-            pos += n * n
+        offs, pos = {}, 0
+        for key in keys:
+            offs[key] = pos
+            pos += key[0] * key[0]
         if not hasattr(self, "bases"):
             self.bases = torch.empty(pos, dtype=torch.complex64, device=d)
             boff = np.array([offs[(int(n), int(s))] for n, s in zip(self.sizes, self.slices)])
             self.base_off = torch.as_tensor(boff.astype(np.int64), device=d)
             self.masks = torch.empty(self.mask_total, dtype=torch.float32, device=d)
             self.energies = torch.empty(len(self.sizes), dtype=torch.float64, device=d)
-        torch.cat(blocks, out=self.bases)
+        for n in sorted({k[0] for k in keys}):
+            depths = [s for m, s in keys if m == n]
+            c = _CLASSES[n]
+            bps = [dict(beam_parameters(replace(cfg, n=n, slices=s, focus_um=s * cfg.dz_um)),
+                        wvl_um=cfg.wavelength_um) for s in depths]
+            if any(bp[q] != bps[0][q] for bp in bps for q in ("d1", "rad", "objrad", "k", "fl",
+                                                              "dx", "wvl", "nr")):
+                raise AssertionError("pupil parameters must not depend on the depth")
+            pup, _ = make_pupil(n, bps[0], self.zern, d)
+            tw = self.tables.tw[c * _SLOT:c * _SLOT + n]
+            B = len(depths)
+            out = self.bases[offs[(n, depths[0])]:offs[(n, depths[0])] + B * n * n]
+            base_beams(pup, bps[0], [bp["dz_focus"] for bp in bps], self.tables.edge2d[n], tw,
+                       out.view(B, n, n), d)
         return self
 
     def _desc(self, order, grid, threads, work):

@@ -56,3 +56,26 @@ def test_uneven_draw_statistics():
     frac = [np.mean(sizes == n) for n in (128, 256, 512)]
     assert abs(frac[0] - 0.5) < 0.03 and abs(frac[1] - 0.4) < 0.03 and abs(frac[2] - 0.1) < 0.02
     assert slices.min() >= 10 and slices.max() <= 60
+
+
+@pytest.mark.gpu
+def test_batched_base_beams_match_single_steps(gpu_lib):
+    """One pupil, one batched Fresnel step over several entry distances = the per-depth path."""
+    import torch
+    from dataclasses import replace
+    from maomi_amd.plain import SiteTables, _CLASSES, _SLOT, base_beam, base_beams, make_pupil
+    from maomi_amd.workloads import CONFIG2, beam_parameters, zernike_weights
+    for n in (128, 256):
+        cfg = replace(CONFIG2, n=n)
+        depths = [10, 23, 37, 60]
+        bps = [dict(beam_parameters(replace(cfg, slices=s, focus_um=s * cfg.dz_um)),
+                    wvl_um=cfg.wavelength_um) for s in depths]
+        tables = SiteTables([n], bps[0]["dx"], bps[0]["dz"],
+                            float(bps[0]["wvl"]) / float(bps[0]["nr"]), "cuda")
+        tw = tables.tw[_CLASSES[n] * _SLOT:_CLASSES[n] * _SLOT + n]
+        pup, _ = make_pupil(n, bps[0], zernike_weights(cfg), "cuda")
+        out = torch.empty((len(depths), n, n), dtype=torch.complex64, device="cuda")
+        base_beams(pup, bps[0], [bp["dz_focus"] for bp in bps], tables.edge2d[n], tw, out, "cuda")
+        for b, bp in enumerate(bps):
+            ref = base_beam(pup, bp, tables.edge2d[n], tw, "cuda")
+            assert rel_l2(out[b].cpu().numpy(), ref.cpu().numpy()) < 1e-6
