@@ -246,6 +246,72 @@ def dense_solve_f64(G, v):
         torch.backends.cuda.preferred_linalg_library(prev)
 
 
+def _householder(x):
+    """Unit u with (I - 2 u u^T) x = alpha e_0, alpha = -sign(x_0) |x|; None when x = 0."""
+    nx = torch.linalg.vector_norm(x)
+    if float(nx) == 0.0:
+        return None, 0.0
+    alpha = -nx if float(x[0]) >= 0 else nx
+    u = x.clone()
+    u[0] = u[0] - alpha
+    nu = torch.linalg.vector_norm(u)
+    return (u / nu if float(nu) > 0 else None), float(alpha)
+
+
+def v4_solve(points, G, v):
+    """Solve the v4 system G w = v (FP64, on G's device) through its positive-definite part.
+
+    G_ij = r^2 (ln r - 1) is conditionally positive definite of order 2: on vectors orthogonal
+    to the polynomials 1, x, y it is positive definite (the r^2 part vanishes there, the thin-
+    plate r^2 ln r part is positive).  With Q = H1 H2 H3 the Householder reflectors of the
+    QR of P = [1 x y], A = Q^T G Q has A22 = A[3:, 3:] positive definite: one Cholesky
+    factorisation (half the work of LU, no pivoting) and a 3 x 3 Schur complement give
+    w = Q [a; b].  Mathematically the solution of numerics.py:129's np.linalg.solve; points
+    on one line, repeated points or a failed factorisation fall back to LU.  G is modified."""
+    n = G.shape[0]
+    v = v.reshape(n, -1)
+    if n <= 3:
+        return dense_solve_f64(G, v)
+    pts = points.to(G.dtype)
+    Pm = torch.stack([torch.ones(n, dtype=G.dtype, device=G.device), pts[:, 0], pts[:, 1]], 1)
+    scale = float(Pm.abs().max())
+    us = []
+    for k in range(3):
+        u, alpha = _householder(Pm[k:, k])
+        if u is None or abs(alpha) <= 1e-10 * scale * (n ** 0.5):
+            return None  # rank < 3: the caller falls back to LU
+        full = torch.zeros(n, dtype=G.dtype, device=G.device)
+        full[k:] = u
+        Pm = Pm - 2.0 * torch.outer(full, full @ Pm)
+        us.append(full)
+    y = v.clone()
+    for u in us:  # G <- H G H (symmetric rank-2 update), y <- H y
+        g = G @ u
+        c = 4.0 * float(u @ g)
+        G.addr_(u, g, alpha=-2.0)
+        G.addr_(g, u, alpha=-2.0)
+        G.addr_(u, u, alpha=c)
+        y -= 2.0 * torch.outer(u, u @ y)
+    A11, A12, A22 = G[:3, :3], G[:3, 3:], G[3:, 3:]
+    L, info = torch.linalg.cholesky_ex(A22)
+    if int(info) != 0:
+        return None
+    X = torch.cholesky_solve(torch.cat([A12.T, y[3:]], 1), L)
+    Z, z = X[:, :3], X[:, 3:]
+    a = torch.linalg.solve(A11 - A12 @ Z, y[:3] - A12 @ z)
+    w = torch.cat([a, z - Z @ a], 0)
+    for u in reversed(us):
+        w -= 2.0 * torch.outer(u, u @ w)
+    return w.contiguous()
+
+
+def v4_coefficients(points, G, v):
+    """v4 weights: the positive-definite route of v4_solve, LU where it does not apply."""
+    G0 = G.clone() if G.shape[0] > 3 else G
+    w = v4_solve(points, G, v)
+    return dense_solve_f64(G0, v.reshape(G0.shape[0], -1)) if w is None else w
+
+
 def v4_gpu(points, values, flat_queries, device, chunk=1 << 20):
     """Device FP64 v4: returns a (Q, C) float64 tensor on `device`."""
     lib = nat.lib()
@@ -256,7 +322,7 @@ def v4_gpu(points, values, flat_queries, device, chunk=1 << 20):
     P, C = v.shape
     G = torch.empty((P, P), dtype=torch.float64, device=device)
     nat.check(lib.maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G), st), "green matrix")
-    coef = dense_solve_f64(G, v)
+    coef = v4_coefficients(p, G, v)
     out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
     for s in range(0, q.shape[0], chunk):
         qs, os_ = q[s:s + chunk], out[s:s + chunk]

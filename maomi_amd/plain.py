@@ -121,8 +121,8 @@ def v4_masks(points, values, shape, device):
         return v[0].expand(shape[0], shape[1], C).clone()
     G = torch.empty((P, P), dtype=torch.float64, device=device)
     nat.check(lib.maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G), st), "green matrix")
-    from .numerics import dense_solve_f64
-    coef = dense_solve_f64(G, v)
+    from .numerics import v4_coefficients
+    coef = v4_coefficients(p, G, v)
     out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
     nat.check(lib.maomi_v4_apply_f64(nat.ptr(p), nat.ptr(coef), nat.ptr(q), nat.ptr(out), P,
                                      q.shape[0], C, st), "v4 apply")

@@ -1,0 +1,46 @@
+"""The v4 solve through the positive-definite part of the Green matrix against a plain LU
+solve (host logic on CPU tensors; the GPU runs the same torch operations)."""
+
+import numpy as np
+import pytest
+import torch
+
+
+def _green(p):
+    d2 = ((p[:, None, :] - p[None, :, :]) ** 2).sum(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = d2 * (np.log(d2) * 0.5 - 1)
+    g[d2 == 0] = 0
+    return g
+
+
+def _cases():
+    rng = np.random.default_rng(0)
+    t = np.round(np.linspace(0, 256, 33))
+    xx, yy = np.meshgrid(t, t, indexing="ij")
+    yield "lattice", np.stack([xx.ravel(), yy.ravel()], 1)
+    yield "random", rng.random((400, 2)) * 100
+    yield "collinear", np.stack([np.linspace(0, 10, 40), np.linspace(0, 5, 40)], 1)
+    yield "three", rng.random((3, 2))
+    yield "two", rng.random((2, 2))
+
+
+@pytest.mark.parametrize("name,p", list(_cases()), ids=[c[0] for c in _cases()])
+def test_v4_coefficients_equal_lu_solution(name, p):
+    from maomi_amd.numerics import v4_coefficients, v4_solve
+    rng = np.random.default_rng(len(p))
+    G = _green(p)
+    v = rng.random((len(p), 3))
+    ref = np.linalg.solve(G, v)
+    w = v4_coefficients(torch.as_tensor(p), torch.as_tensor(G.copy()), torch.as_tensor(v))
+    assert np.linalg.norm(w.numpy() - ref) / np.linalg.norm(ref) < 1e-9
+    if name == "collinear":  # rank-2 polynomial space: the positive-definite route declines
+        assert v4_solve(torch.as_tensor(p), torch.as_tensor(G.copy()), torch.as_tensor(v)) is None
+
+
+def test_green_matrix_has_three_negative_eigenvalues():
+    # conditional positive definiteness of order 2: only the span of 1, x, y is negative
+    t = np.round(np.linspace(0, 256, 17))
+    xx, yy = np.meshgrid(t, t, indexing="ij")
+    w = np.linalg.eigvalsh(_green(np.stack([xx.ravel(), yy.ravel()], 1)))
+    assert (w < 0).sum() == 3
