@@ -240,6 +240,13 @@ int maomi_v4_green_matrix_f64(const double* points, int64_t P, double* G, maomi_
  * out[t][c] = sum_i g(|q_t - p_i|^2) coef[i][c], g(r2) = r2 (ln(r2)/2 - 1), 1 <= C <= 8. */
 int maomi_v4_apply_f64(const double* points, const double* coef, const double* queries,
                        double* out, int64_t P, int64_t Q, int64_t C, maomi_stream_t stream);
+/* maomi_v4_apply_f64 with the point sum split over `rows` block rows for parallelism:
+ * partial (rows * Q * C doubles, unused for rows = 1) holds the row sums, added in row order;
+ * maomi_v4_apply_rows(P, Q) is a row count that fills the device */
+int maomi_v4_apply_split_f64(const double* points, const double* coef, const double* queries,
+                             double* out, int64_t P, int64_t Q, int64_t C, double* partial,
+                             int64_t rows, maomi_stream_t stream);
+int64_t maomi_v4_apply_rows(int64_t P, int64_t Q);
 
 /* ---------------------------------------------------------------------------
  * Learned PSF surrogate (no counterpart in the reference: the reference has no

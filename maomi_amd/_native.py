@@ -127,6 +127,9 @@ _SIGS = {
     "maomi_v4_green_matrix_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "maomi_v4_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
                                    c_int64, c_void_p]),
+    "maomi_v4_apply_split_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                                         c_int64, c_int64, c_void_p, c_int64, c_void_p]),
+    "maomi_v4_apply_rows": (c_int64, [c_int64, c_int64]),
 }
 
 

@@ -312,6 +312,19 @@ def v4_coefficients(points, G, v):
     return dense_solve_f64(G0, v.reshape(G0.shape[0], -1)) if w is None else w
 
 
+def v4_apply(p, coef, q, out, device):
+    """out (Q, C) = sum_i G(|q - p_i|) coef[i] on the device, the point sum split over block
+    rows so that small query sets still fill the chip (maomi_v4_apply_split_f64)."""
+    lib = nat.lib()
+    P, C, Q = p.shape[0], coef.shape[1], q.shape[0]
+    rows = int(lib.maomi_v4_apply_rows(P, Q))
+    part = torch.empty(rows * Q * C if rows > 1 else 0, dtype=torch.float64, device=device)
+    nat.check(lib.maomi_v4_apply_split_f64(nat.ptr(p), nat.ptr(coef), nat.ptr(q), nat.ptr(out),
+                                           P, Q, C, nat.ptr(part) if rows > 1 else None, rows,
+                                           nat.stream_handle(device)), "v4 apply")
+    return out
+
+
 def v4_gpu(points, values, flat_queries, device, chunk=1 << 20):
     """Device FP64 v4: returns a (Q, C) float64 tensor on `device`."""
     lib = nat.lib()
@@ -325,7 +338,5 @@ def v4_gpu(points, values, flat_queries, device, chunk=1 << 20):
     coef = v4_coefficients(p, G, v)
     out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
     for s in range(0, q.shape[0], chunk):
-        qs, os_ = q[s:s + chunk], out[s:s + chunk]
-        nat.check(lib.maomi_v4_apply_f64(nat.ptr(p), nat.ptr(coef), nat.ptr(qs), nat.ptr(os_), P,
-                                         qs.shape[0], C, st), "v4 apply")
+        v4_apply(p, coef, q[s:s + chunk], out[s:s + chunk], device)
     return out

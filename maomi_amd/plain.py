@@ -123,9 +123,9 @@ def v4_masks(points, values, shape, device):
     nat.check(lib.maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G), st), "green matrix")
     from .numerics import v4_coefficients
     coef = v4_coefficients(p, G, v)
+    from .numerics import v4_apply
     out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
-    nat.check(lib.maomi_v4_apply_f64(nat.ptr(p), nat.ptr(coef), nat.ptr(q), nat.ptr(out), P,
-                                     q.shape[0], C, st), "v4 apply")
+    v4_apply(p, coef, q, out, device)
     return out.reshape(shape[0], shape[1], C)
 
 
