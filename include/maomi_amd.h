@@ -247,6 +247,14 @@ int maomi_v4_apply_split_f64(const double* points, const double* coef, const dou
                              double* out, int64_t P, int64_t Q, int64_t C, double* partial,
                              int64_t rows, maomi_stream_t stream);
 int64_t maomi_v4_apply_rows(int64_t P, int64_t Q);
+/* Part of the v4 solve that replaces np.linalg.solve (maomi/numerics.py:129): in-place FP64
+ * Cholesky factorisation A = L L^T of the positive-definite block of the projected Green
+ * matrix.  A is row-major with leading dimension lda >= m; only the lower triangle is read and
+ * written (it receives L).  *info (a device int) becomes 0, or the 1-based column of the first
+ * non-positive pivot.  workspace: maomi_cholesky_f64_workspace() bytes of device memory. */
+int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
+                       maomi_stream_t stream);
+int64_t maomi_cholesky_f64_workspace(void);
 
 /* ---------------------------------------------------------------------------
  * Learned PSF surrogate (no counterpart in the reference: the reference has no

@@ -83,6 +83,196 @@ __global__ void green_matrix_kernel(const double* __restrict__ p, double* __rest
   }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// FP64 Cholesky factorisation A = L L^T of the v4 system's positive-definite block (row-major,
+// lower triangle, in place), right-looking in steps of kNb columns.  Per step:
+//   chol_panel_kernel: every workgroup factors the kNb x kNb diagonal block in registers and
+//     LDS (one barrier per column), then each thread solves one row of the panel below,
+//     x L11^T = a; workgroup 0 also stores L11 in the side buffer;
+//   chol_update_kernel: block 0 copies L11 from the side buffer into the diagonal block (the
+//     panel workgroups read that block, so it cannot be written in the same launch); the others
+//     subtract L21 L21^T from the lower 64 x 64 tiles of the trailing matrix.
+// Only the lower triangle is read or written.  The first non-positive (or NaN) pivot sets
+// *info to its 1-based column, as LAPACK's potrf does.
+
+constexpr int kNb = 64;
+constexpr int kCholThreads = 256;
+
+struct CholShared {
+  double lt[kNb][kNb];  // lt[c][i] = L11[i][c] (zero above the diagonal)
+  double rd[kNb];       // 1 / L11[j][j]
+  double piv[kNb];      // pivots of the LDL^T form, L11[j][j]^2
+  double col[2][kNb];   // column j of the running factorisation, double-buffered
+};
+
+// Workgroup barrier that orders LDS only: global loads stay in flight across it.
+__device__ __forceinline__ void chol_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Factor the b x b block at D (b <= kNb).  Thread t holds row i = t % 64, columns
+// 16 g .. 16 g + 15 (g = t / 64, uniform in a wave) of the block; rows and columns past b are
+// the identity.  Step j broadcasts column j through LDS and updates
+// S[i][c] -= S[i][j] S[c][j] / S[j][j] for c > j (a select keeps c <= j); column j itself is
+// not written in step j, so one barrier per step suffices.
+__device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, int64_t lda, int b,
+                                                 CholShared& sm, int* info, int64_t k0,
+                                                 bool report) {
+  const int t = threadIdx.x, i = t & 63;
+  const int g = __builtin_amdgcn_readfirstlane(t >> 6);
+  double s[16];
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) {
+    const int c = 16 * g + cc;
+    s[cc] = (i < b && c <= i) ? D[(int64_t)i * lda + c] : (c == i ? 1.0 : 0.0);
+  }
+#pragma unroll
+  for (int j = 0; j < kNb; ++j) {
+    double* cj = sm.col[j & 1];
+    if (g == (j >> 4)) cj[i] = s[j & 15];
+    chol_lds_barrier();
+    const double piv = cj[j];
+    const double lij = cj[i] / piv;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+      const double u = fma(-lij, cj[16 * g + cc], s[cc]);
+      s[cc] = 16 * g + cc > j ? u : s[cc];
+    }
+    if (t == 0) sm.piv[j] = piv;
+  }
+  chol_lds_barrier();
+  if (report && t == 0) {
+    int bad = 0;
+    for (int j = b - 1; j >= 0; --j) bad = sm.piv[j] > 0.0 ? bad : j + 1;
+    if (bad) atomicCAS(info, 0, (int)(k0 + bad));
+  }
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) {
+    const int c = 16 * g + cc;
+    const double d = sqrt(sm.piv[c]);
+    sm.lt[c][i] = c < i ? s[cc] / d : (c == i ? d : 0.0);
+  }
+  if (t < kNb) sm.rd[t] = 1.0 / sqrt(sm.piv[t]);
+  chol_lds_barrier();
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(double* __restrict__ A,
+                                                                  int64_t lda, int64_t m,
+                                                                  int64_t k,
+                                                                  double* __restrict__ l11,
+                                                                  int* info) {
+  __shared__ CholShared sm;
+  const int b = (int)min((int64_t)kNb, m - k);
+  const int t = threadIdx.x;
+  chol_factor_diag(A + k * lda + k, lda, b, sm, info, k, blockIdx.x == 0);
+  if (blockIdx.x == 0) {
+    for (int e = t; e < kNb * kNb; e += kCholThreads) {
+      const int i = e >> 6, c = e & 63;
+      l11[e] = sm.lt[c][i];
+    }
+  }
+  // panel rows exist only when b == kNb.  They are loaded after the factorisation: held
+  // across it, the 64 row values leave the factorisation no registers to batch its LDS reads.
+  const int64_t r = k + b + (int64_t)blockIdx.x * kCholThreads + t;
+  if (b < kNb || r >= m) return;
+  double* row = A + r * lda + k;
+  double x[kNb];
+#pragma unroll
+  for (int l = 0; l < kNb; ++l) x[l] = row[l];
+#pragma unroll
+  for (int l = 0; l < kNb; ++l) {
+    x[l] *= sm.rd[l];
+#pragma unroll
+    for (int j = l + 1; j < kNb; ++j) x[j] = fma(-x[l], sm.lt[l][j], x[j]);
+  }
+#pragma unroll
+  for (int l = 0; l < kNb; ++l) row[l] = x[l];
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __restrict__ A,
+                                                                   int64_t lda, int64_t m,
+                                                                   int64_t k,
+                                                                   const double* __restrict__ l11) {
+  constexpr int LS = kNb + 1;  // odd row stride: the transposed stores spread over the banks
+  __shared__ double pi[kNb][LS], pj[kNb][LS];
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0) {
+    const int b = (int)min((int64_t)kNb, m - k);
+    for (int e = t; e < kNb * kNb; e += kCholThreads) {
+      const int i = e >> 6, c = e & 63;
+      if (i < b && c <= i) A[(k + i) * lda + k + c] = l11[e];
+    }
+    return;
+  }
+  const int64_t id = blockIdx.x - 1;
+  int64_t I = (int64_t)((sqrt(8.0 * (double)id + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= id) ++I;
+  while (I * (I + 1) / 2 > id) --I;
+  const int64_t J = id - I * (I + 1) / 2;
+  const int64_t k1 = k + kNb, r0 = k1 + kNb * I, c0 = k1 + kNb * J;
+  const int tx = t & 15, ty = t >> 4;
+  // panel rows r0.. and c0.. (columns k .. k + 63), loaded in one batch, stored transposed
+  double va[16], vb[16];
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const int rr = (t >> 6) + 4 * n, kk = t & 63;
+    va[n] = r0 + rr < m ? A[(r0 + rr) * lda + k + kk] : 0.0;
+    vb[n] = c0 + rr < m ? A[(c0 + rr) * lda + k + kk] : 0.0;
+  }
+  // the tile being updated, requested before the product
+  double cv[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t rw = r0 + 4 * ty + u;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t c = c0 + 4 * tx + v;
+      cv[u][v] = (rw < m && c <= rw) ? A[rw * lda + c] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const int rr = (t >> 6) + 4 * n, kk = t & 63;
+    pi[kk][rr] = va[n];
+    pj[kk][rr] = vb[n];
+  }
+  __syncthreads();
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+#pragma unroll 4
+  for (int kk = 0; kk < kNb; ++kk) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      av[u] = pi[kk][4 * ty + u];
+      bv[u] = pj[kk][4 * tx + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(av[u], bv[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t rw = r0 + 4 * ty + u;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t c = c0 + 4 * tx + v;
+      if (rw < m && c <= rw) A[rw * lda + c] = cv[u][v] - acc[u][v];
+    }
+  }
+}
+
+__global__ void chol_info_reset_kernel(int* info) {
+  if (threadIdx.x == 0) info[0] = 0;
+}
+
 }  // namespace
 
 MAOMI_API int maomi_v4_apply_split_f64(const double* points, const double* coef,
@@ -150,6 +340,33 @@ MAOMI_API int maomi_v4_apply_split_f64(const double* points, const double* coef,
     const unsigned b = (unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(b), dim3(256), 0, st, partial, (int)rows, n,
                        out);
+    MAOMI_CHECK_LAUNCH();
+  }
+  return MAOMI_OK;
+}
+
+MAOMI_API int64_t maomi_cholesky_f64_workspace(void) {
+  return (int64_t)kNb * kNb * (int64_t)sizeof(double);
+}
+
+MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
+                                 maomi_stream_t stream) {
+  if (!A || !workspace || !info || m < 0 || lda < m || lda < 1) {
+    maomi_set_error("maomi_cholesky_f64: bad arguments");
+    return MAOMI_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(chol_info_reset_kernel, dim3(1), dim3(64), 0, st, info);
+  MAOMI_CHECK_LAUNCH();
+  for (int64_t k = 0; k < m; k += kNb) {
+    const int64_t b = m - k < kNb ? m - k : kNb, mp = m - k - b;
+    const int64_t pblocks = mp > 0 ? (mp + kCholThreads - 1) / kCholThreads : 1;
+    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)pblocks), dim3(kCholThreads), 0, st, A,
+                       lda, m, k, workspace, info);
+    MAOMI_CHECK_LAUNCH();
+    const int64_t T = (mp + kNb - 1) / kNb;
+    hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(1 + T * (T + 1) / 2)),
+                       dim3(kCholThreads), 0, st, A, lda, m, k, workspace);
     MAOMI_CHECK_LAUNCH();
   }
   return MAOMI_OK;

@@ -246,6 +246,22 @@ def dense_solve_f64(G, v):
         torch.backends.cuda.preferred_linalg_library(prev)
 
 
+def cholesky_f64_(A):
+    """In-place FP64 Cholesky of a row-major device view A (m x m, unit column stride) with
+    maomi_cholesky_f64: the lower triangle receives L, the upper is left as it was.  Returns
+    0, or the 1-based column of the first non-positive pivot (one device-to-host read)."""
+    m = A.shape[0]
+    if A.dtype != torch.float64 or not A.is_cuda or A.shape != (m, m) or A.stride(1) != 1:
+        raise ValueError("cholesky_f64_ needs a square float64 device view with unit column stride")
+    lib = nat.lib()
+    work = torch.empty(int(lib.maomi_cholesky_f64_workspace()) // 8, dtype=torch.float64,
+                       device=A.device)
+    info = torch.empty(1, dtype=torch.int32, device=A.device)
+    nat.check(lib.maomi_cholesky_f64(nat.ptr(A), A.stride(0), m, nat.ptr(work), nat.ptr(info),
+                                     nat.stream_handle(A.device)), "cholesky")
+    return int(info)
+
+
 def _householder(x):
     """Unit u with (I - 2 u u^T) x = alpha e_0, alpha = -sign(x_0) |x|; None when x = 0."""
     nx = torch.linalg.vector_norm(x)
@@ -267,7 +283,8 @@ def v4_solve(points, G, v):
     QR of P = [1 x y], A = Q^T G Q has A22 = A[3:, 3:] positive definite: one Cholesky
     factorisation (half the work of LU, no pivoting) and a 3 x 3 Schur complement give
     w = Q [a; b].  Mathematically the solution of numerics.py:129's np.linalg.solve; points
-    on one line, repeated points or a failed factorisation fall back to LU.  G is modified."""
+    on one line, repeated points or a failed factorisation fall back to LU.  On the device the
+    factorisation is maomi_cholesky_f64.  G is modified."""
     n = G.shape[0]
     v = v.reshape(n, -1)
     if n <= 3:
@@ -293,9 +310,14 @@ def v4_solve(points, G, v):
         G.addr_(u, u, alpha=c)
         y -= 2.0 * torch.outer(u, u @ y)
     A11, A12, A22 = G[:3, :3], G[:3, 3:], G[3:, 3:]
-    L, info = torch.linalg.cholesky_ex(A22)
-    if int(info) != 0:
-        return None
+    if G.is_cuda:
+        if cholesky_f64_(A22) != 0:
+            return None
+        L = A22  # lower triangle; the upper keeps A22's own (symmetric) values
+    else:
+        L, info = torch.linalg.cholesky_ex(A22)
+        if int(info) != 0:
+            return None
     X = torch.cholesky_solve(torch.cat([A12.T, y[3:]], 1), L)
     Z, z = X[:, :3], X[:, 3:]
     a = torch.linalg.solve(A11 - A12 @ Z, y[:3] - A12 @ z)

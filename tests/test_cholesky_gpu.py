@@ -1,0 +1,75 @@
+"""maomi_cholesky_f64 against numpy's Cholesky, and the device v4 solve against LU.
+
+The factorisation is pinned by the textbook routine (np.linalg.cholesky) on SPD matrices of
+sizes around the 64-column step, inside larger row-major buffers (lda > m, offset views), and
+by its pivot report on indefinite matrices (LAPACK potrf's info convention).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(m, rng):
+    a = rng.standard_normal((m, m + 8))
+    return a @ a.T / m + np.eye(m) * 0.1
+
+
+@pytest.mark.parametrize("m", [1, 5, 63, 64, 65, 130, 300, 1000])
+def test_cholesky_matches_numpy(gpu_lib, m):
+    import torch
+    from maomi_amd.numerics import cholesky_f64_
+    rng = np.random.default_rng(m)
+    A = _spd(m, rng)
+    buf = rng.standard_normal((m + 5, m + 7))
+    buf[3:3 + m, 2:2 + m] = A
+    t = torch.as_tensor(buf, device="cuda")
+    print(f"[cholesky] m={m}", flush=True)
+    assert cholesky_f64_(t[3:3 + m, 2:2 + m]) == 0
+    got = t.cpu().numpy()
+    L = np.linalg.cholesky(A)
+    low = np.tril_indices(m)
+    assert np.abs(got[3:3 + m, 2:2 + m][low] - L[low]).max() <= 1e-12 * np.abs(L).max()
+    up = np.triu_indices(m, 1)
+    np.testing.assert_array_equal(got[3:3 + m, 2:2 + m][up], A[up])  # upper triangle untouched
+    outside = np.ones(buf.shape, bool)
+    outside[3:3 + m, 2:2 + m] = False
+    np.testing.assert_array_equal(got[outside], buf[outside])
+
+
+@pytest.mark.parametrize("m,bad", [(10, 2), (200, 70), (130, 129)])
+def test_cholesky_reports_first_bad_pivot(gpu_lib, m, bad):
+    import torch
+    from maomi_amd.numerics import cholesky_f64_
+    d = np.arange(1.0, m + 1)
+    d[bad] = -1.0
+    t = torch.as_tensor(np.diag(d), device="cuda")
+    assert cholesky_f64_(t) == bad + 1
+
+
+def _green(p):
+    d2 = ((p[:, None, :] - p[None, :, :]) ** 2).sum(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = d2 * (np.log(d2) * 0.5 - 1)
+    g[d2 == 0] = 0
+    return g
+
+
+@pytest.mark.parametrize("name", ["lattice", "random"])
+def test_device_v4_coefficients_equal_lu_solution(gpu_lib, name):
+    import torch
+    from maomi_amd.numerics import v4_coefficients
+    rng = np.random.default_rng(3)
+    if name == "lattice":
+        t = np.round(np.linspace(0, 256, 33))
+        xx, yy = np.meshgrid(t, t, indexing="ij")
+        p = np.stack([xx.ravel(), yy.ravel()], 1)
+    else:
+        p = rng.random((700, 2)) * 100
+    G = _green(p)
+    v = rng.random((len(p), 3))
+    ref = np.linalg.solve(G, v)
+    w = v4_coefficients(torch.as_tensor(p, device="cuda"), torch.as_tensor(G, device="cuda"),
+                        torch.as_tensor(v, device="cuda")).cpu().numpy()
+    assert np.linalg.norm(w - ref) / np.linalg.norm(ref) < 1e-9
