@@ -14,8 +14,40 @@ namespace {
 constexpr int kTile = 256;
 constexpr int kMaxC = 8;
 
+// Natural logarithm of a finite x > 0 in FP64, within 2 ulp: x = 2^e m with m in
+// [sqrt(1/2), sqrt(2)), log x = e ln2 + 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716, with
+// atanh(s) = s + s^3/3 + ... + s^23/23 (remainder below 1e-18 relative).  ln 2 is split into
+// a 32-bit head, so e * kLn2Hi is exact, and the rounded remainder.  About a third of the
+// instructions of the library log, which carries a double-double result.
+__device__ __forceinline__ double log_pos(double x) {
+  constexpr double kLn2Hi = 0x1.62e42fee00000p-1, kLn2Lo = 0x1.a39ef35793c76p-33;
+  int e;
+  double m = frexp(x, &e);
+  const bool low = m < 0.70710678118654752440;
+  m = low ? 2.0 * m : m;
+  e = low ? e - 1 : e;
+  const double s = (m - 1.0) / (m + 1.0), t = s * s;
+  double p = 1.0 / 23.0;
+  p = fma(t, p, 1.0 / 21.0);
+  p = fma(t, p, 1.0 / 19.0);
+  p = fma(t, p, 1.0 / 17.0);
+  p = fma(t, p, 1.0 / 15.0);
+  p = fma(t, p, 1.0 / 13.0);
+  p = fma(t, p, 1.0 / 11.0);
+  p = fma(t, p, 1.0 / 9.0);
+  p = fma(t, p, 1.0 / 7.0);
+  p = fma(t, p, 1.0 / 5.0);
+  p = fma(t, p, 1.0 / 3.0);
+  const double s2 = 2.0 * s;
+  const double lm = fma(s2 * t, p, s2);
+  const double ed = (double)e;
+  return fma(ed, kLn2Hi, fma(ed, kLn2Lo, lm));
+}
+
+// G as a function of r2 = d^2: r2 (ln(r2) / 2 - 1), 0 at r2 = 0; no branch.
 __device__ __forceinline__ double green_r2(double r2) {
-  return r2 > 0.0 ? r2 * (0.5 * log(r2) - 1.0) : 0.0;
+  const double g = r2 * fma(0.5, log_pos(r2 > 0.0 ? r2 : 1.0), -1.0);
+  return r2 > 0.0 ? g : 0.0;
 }
 
 // Points [p0, p1) of the sum go to block row blockIdx.y: p0 = blockIdx.y * pchunk.  With one
@@ -46,6 +78,7 @@ __global__ __launch_bounds__(256) void green_apply_kernel(const double* __restri
       for (int j = 0; j < C; ++j) pc[i * C + j] = coef[(base + i) * C + j];
     }
     __syncthreads();
+#pragma unroll 4
     for (int i = 0; i < n; ++i) {
       const double dx = x - px[i], dy = y - py[i];
       const double g = green_r2(dx * dx + dy * dy);
