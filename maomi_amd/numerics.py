@@ -246,10 +246,11 @@ def dense_solve_f64(G, v):
         torch.backends.cuda.preferred_linalg_library(prev)
 
 
-def cholesky_f64_(A):
+def cholesky_f64_(A, sync=True):
     """In-place FP64 Cholesky of a row-major device view A (m x m, unit column stride) with
     maomi_cholesky_f64: the lower triangle receives L, the upper is left as it was.  Returns
-    0, or the 1-based column of the first non-positive pivot (one device-to-host read)."""
+    0, or the 1-based column of the first non-positive pivot (one device-to-host read); with
+    sync=False the device int32 tensor that holds it, without reading it."""
     m = A.shape[0]
     if A.dtype != torch.float64 or not A.is_cuda or A.shape != (m, m) or A.stride(1) != 1:
         raise ValueError("cholesky_f64_ needs a square float64 device view with unit column stride")
@@ -259,79 +260,132 @@ def cholesky_f64_(A):
     info = torch.empty(1, dtype=torch.int32, device=A.device)
     nat.check(lib.maomi_cholesky_f64(nat.ptr(A), A.stride(0), m, nat.ptr(work), nat.ptr(info),
                                      nat.stream_handle(A.device)), "cholesky")
-    return int(info)
+    return int(info) if sync else info
 
 
-def _householder(x):
-    """Unit u with (I - 2 u u^T) x = alpha e_0, alpha = -sign(x_0) |x|; None when x = 0."""
-    nx = torch.linalg.vector_norm(x)
-    if float(nx) == 0.0:
-        return None, 0.0
-    alpha = -nx if float(x[0]) >= 0 else nx
-    u = x.clone()
-    u[0] = u[0] - alpha
-    nu = torch.linalg.vector_norm(u)
-    return (u / nu if float(nu) > 0 else None), float(alpha)
+def _reflectors(points):
+    """Householder vectors of the QR of P = [1 x y] (host FP64, n x 3): V (n x 3, unit
+    columns, H_k = I - 2 v_k v_k^T) and T (3 x 3, upper triangular) with
+    H1 H2 H3 = I - V T V^T (compact WY form).  None when P has rank below 3."""
+    p = np.asarray(points, np.float64)
+    n = len(p)
+    Pm = np.stack([np.ones(n), p[:, 0], p[:, 1]], 1)
+    scale = float(np.abs(Pm).max())
+    V = np.zeros((n, 3))
+    for k in range(3):
+        x = Pm[k:, k]
+        nx = float(np.linalg.norm(x))
+        alpha = -nx if x[0] >= 0 else nx
+        if nx == 0.0 or abs(alpha) <= 1e-10 * scale * (n ** 0.5):
+            return None
+        u = x.copy()
+        u[0] -= alpha
+        nu = float(np.linalg.norm(u))
+        if nu == 0.0:
+            return None
+        V[k:, k] = u / nu
+        Pm -= 2.0 * np.outer(V[:, k], V[:, k] @ Pm)
+    VtV = V.T @ V
+    T = np.zeros((3, 3))
+    for i in range(3):
+        T[i, i] = 2.0
+        if i:
+            T[:i, i] = -2.0 * (T[:i, :i] @ VtV[:i, i])
+    return V, T
 
 
-def v4_solve(points, G, v):
-    """Solve the v4 system G w = v (FP64, on G's device) through its positive-definite part.
+class V4Factor:
+    """The part of the v4 solve that depends on the points only (see v4_factor)."""
+
+    def __init__(self, G, G0, V, T, L, Z, S, A12, info):
+        self.G, self.G0, self.V, self.T, self.L = G, G0, V, T, L
+        self.Z, self.S, self.A12, self.info = Z, S, A12, info
+
+    def tensors(self):
+        return [t for t in (self.G, self.G0, self.V, self.T, self.Z, self.S, self.info)
+                if isinstance(t, torch.Tensor)]
+
+
+def v4_factor(points, G, *, keep_copy=True, reflectors=None):
+    """Point-only part of the v4 solve G w = v (FP64, on G's device), through its
+    positive-definite part.
 
     G_ij = r^2 (ln r - 1) is conditionally positive definite of order 2: on vectors orthogonal
     to the polynomials 1, x, y it is positive definite (the r^2 part vanishes there, the thin-
     plate r^2 ln r part is positive).  With Q = H1 H2 H3 the Householder reflectors of the
-    QR of P = [1 x y], A = Q^T G Q has A22 = A[3:, 3:] positive definite: one Cholesky
-    factorisation (half the work of LU, no pivoting) and a 3 x 3 Schur complement give
-    w = Q [a; b].  Mathematically the solution of numerics.py:129's np.linalg.solve; points
-    on one line, repeated points or a failed factorisation fall back to LU.  On the device the
-    factorisation is maomi_cholesky_f64.  G is modified."""
+    QR of P = [1 x y], A = Q^T G Q has A22 = A[3:, 3:] positive definite.  This computes A
+    (G is overwritten), the Cholesky factor L of A22 (maomi_cholesky_f64 on the device),
+    Z = A22^-1 A12^T and the 3 x 3 Schur complement S = A11 - A12 Z.  The values enter only
+    in v4_finish, so this part can run while the values are still being computed.
+    keep_copy keeps G's original for the LU fallback; reflectors = (V, T) as device tensors, or
+    False for rank below 3, from _reflectors computed earlier.  Returns None when P = [1 x y] has rank
+    below 3 (points on one line, repeated points); no host synchronisation after the
+    reflectors are built (they are built on the host from the points)."""
     n = G.shape[0]
-    v = v.reshape(n, -1)
+    G0 = G.clone() if keep_copy else None
     if n <= 3:
-        return dense_solve_f64(G, v)
-    pts = points.to(G.dtype)
-    Pm = torch.stack([torch.ones(n, dtype=G.dtype, device=G.device), pts[:, 0], pts[:, 1]], 1)
-    scale = float(Pm.abs().max())
-    us = []
-    for k in range(3):
-        u, alpha = _householder(Pm[k:, k])
-        if u is None or abs(alpha) <= 1e-10 * scale * (n ** 0.5):
-            return None  # rank < 3: the caller falls back to LU
-        full = torch.zeros(n, dtype=G.dtype, device=G.device)
-        full[k:] = u
-        Pm = Pm - 2.0 * torch.outer(full, full @ Pm)
-        us.append(full)
-    y = v.clone()
-    for u in us:  # G <- H G H (symmetric rank-2 update), y <- H y
-        g = G @ u
-        c = 4.0 * float(u @ g)
-        G.addr_(u, g, alpha=-2.0)
-        G.addr_(g, u, alpha=-2.0)
-        G.addr_(u, u, alpha=c)
-        y -= 2.0 * torch.outer(u, u @ y)
+        return None
+    if reflectors is None:
+        if isinstance(points, torch.Tensor):
+            points = points.detach().cpu().numpy()
+        ref = _reflectors(points)
+        reflectors = False if ref is None else tuple(
+            torch.as_tensor(a, dtype=G.dtype, device=G.device) for a in ref)
+    if reflectors is False:
+        return V4Factor(None, G0, None, None, None, None, None, None, None)
+    V, T = reflectors
+    # Q^T G Q = G - (N V^T + V N^T) with M = G V, N = M T - V S / 2, S = T^T V^T M T: one pass
+    # over G for M and one symmetric rank-6 update.
+    M = G @ V
+    Sm = T.T @ (V.T @ M) @ T
+    N = M @ T - 0.5 * (V @ Sm)
+    G.addmm_(torch.cat([N, V], 1), torch.cat([V, N], 1).T, alpha=-1.0)
     A11, A12, A22 = G[:3, :3], G[:3, 3:], G[3:, 3:]
     if G.is_cuda:
-        if cholesky_f64_(A22) != 0:
-            return None
+        info = cholesky_f64_(A22, sync=False)
         L = A22  # lower triangle; the upper keeps A22's own (symmetric) values
     else:
         L, info = torch.linalg.cholesky_ex(A22)
-        if int(info) != 0:
-            return None
-    X = torch.cholesky_solve(torch.cat([A12.T, y[3:]], 1), L)
-    Z, z = X[:, :3], X[:, 3:]
-    a = torch.linalg.solve(A11 - A12 @ Z, y[:3] - A12 @ z)
-    w = torch.cat([a, z - Z @ a], 0)
-    for u in reversed(us):
-        w -= 2.0 * torch.outer(u, u @ w)
-    return w.contiguous()
+    Z = torch.cholesky_solve(A12.T.contiguous(), L)
+    S = A11 - A12 @ Z
+    return V4Factor(G, G0, V, T, L, Z, S, A12, info)
 
 
-def v4_coefficients(points, G, v):
-    """v4 weights: the positive-definite route of v4_solve, LU where it does not apply."""
-    G0 = G.clone() if G.shape[0] > 3 else G
-    w = v4_solve(points, G, v)
-    return dense_solve_f64(G0, v.reshape(G0.shape[0], -1)) if w is None else w
+def v4_finish(f, v):
+    """w with G w = v from a v4_factor result; None when the positive-definite route does not
+    apply (the caller then solves by LU).  One host read of the factorisation's pivot report."""
+    if f is None or f.V is None or int(f.info) != 0:
+        return None
+    v = v.reshape(f.V.shape[0], -1)
+    V, T = f.V, f.T
+    y = v - V @ (T.T @ (V.T @ v))  # Q^T v
+    z = torch.cholesky_solve(y[3:].contiguous(), f.L)
+    a = torch.linalg.solve(f.S, y[:3] - f.A12 @ z)
+    x = torch.cat([a, z - f.Z @ a], 0)
+    return (x - V @ (T @ (V.T @ x))).contiguous()  # Q x
+
+
+def v4_solve(points, G, v):
+    """Solve the v4 system G w = v through its positive-definite part (v4_factor, v4_finish):
+    mathematically the solution of numerics.py:129's np.linalg.solve.  Returns None where that
+    route does not apply (the caller falls back to LU).  G is modified."""
+    n = G.shape[0]
+    if n <= 3:
+        return dense_solve_f64(G, v.reshape(n, -1))
+    return v4_finish(v4_factor(points, G, keep_copy=False), v)
+
+
+def v4_coefficients(points, G, v, factor=None):
+    """v4 weights: the positive-definite route, LU where it does not apply.  `factor` is a
+    v4_factor result for these points computed earlier (G is then not used)."""
+    if factor is None:
+        if G.shape[0] <= 3:
+            return dense_solve_f64(G, v.reshape(G.shape[0], -1))
+        factor = v4_factor(points, G)
+    w = v4_finish(factor, v)
+    if w is None:
+        return dense_solve_f64(factor.G0, v.reshape(factor.G0.shape[0], -1))
+    return w
 
 
 def v4_apply(p, coef, q, out, device):

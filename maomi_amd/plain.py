@@ -105,25 +105,74 @@ def base_beams(u, bp, dz_focus, edge2d, tw, out, device):
     return out
 
 
-def v4_masks(points, values, shape, device):
-    """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153)."""
-    lib = nat.lib()
-    st = nat.stream_handle(device)
-    p = torch.as_tensor(np.ascontiguousarray(points, np.float64), device=device)
+def _upload(a, device):
+    """Host array to the device through pinned memory, without blocking the host."""
+    return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(device, non_blocking=True)
+
+
+def v4_prepare(points, device):
+    """Green matrix and the point-only part of the v4 solve (numerics.v4_factor), queued on
+    the current stream with no host synchronisation.  Returns (points, G, factor)."""
+    from .numerics import _reflectors, v4_factor
+    pts = np.ascontiguousarray(points, np.float64)
+    P = len(pts)
+    p = _upload(pts, device)
+    ref = _reflectors(pts) if P > 3 else None
+    ref = False if ref is None else tuple(_upload(a, device) for a in ref)
+    G = torch.empty((P, P), dtype=torch.float64, device=device)
+    nat.check(nat.lib().maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G),
+                                                  nat.stream_handle(device)), "green matrix")
+    factor = v4_factor(p, G, reflectors=ref) if P > 3 else None
+    return p, G, factor
+
+
+class V4Ahead:
+    """The point-only part of the v4 solve queued on a side stream (default priority) while the
+    main stream computes the values; take() orders the main stream after it."""
+
+    def __init__(self, device):
+        self.device, self.side, self.ready = device, None, None
+
+    def start(self, points):
+        if self.side is None:
+            self.side = torch.cuda.Stream(self.device)
+        with torch.cuda.stream(self.side):
+            prep = v4_prepare(points, self.device)
+            ev = torch.cuda.Event()
+            ev.record(self.side)
+        self.ready = (prep, ev)
+
+    def take(self):
+        """The prepared (points, G, factor), with the current stream ordered after it and its
+        tensors marked as used there; None if nothing was started."""
+        if self.ready is None:
+            return None
+        prep, ev = self.ready
+        self.ready = None
+        main = torch.cuda.current_stream(self.device)
+        main.wait_event(ev)
+        p, G, factor = prep
+        for t in [p, G] + (factor.tensors() if factor is not None else []):
+            t.record_stream(main)
+        return prep
+
+
+def v4_masks(points, values, shape, device, prepared=None):
+    """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153).
+    prepared: a v4_prepare result for these points (possibly from another stream, already
+    ordered before this call)."""
+    from .numerics import v4_apply, v4_coefficients
     v = torch.as_tensor(values, dtype=torch.float64, device=device)
     if v.ndim == 1:
         v = v[:, None]
     P, C = v.shape
+    if P == 1:
+        return v[0].expand(shape[0], shape[1], C).clone()
+    p, G, factor = prepared if prepared is not None else v4_prepare(points, device)
     xs = torch.arange(shape[0], dtype=torch.float64, device=device) + 0.5
     ys = torch.arange(shape[1], dtype=torch.float64, device=device) + 0.5
     q = torch.stack(torch.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2).contiguous()
-    if P == 1:
-        return v[0].expand(shape[0], shape[1], C).clone()
-    G = torch.empty((P, P), dtype=torch.float64, device=device)
-    nat.check(lib.maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G), st), "green matrix")
-    from .numerics import v4_coefficients
-    coef = v4_coefficients(p, G, v)
-    from .numerics import v4_apply
+    coef = v4_coefficients(p, G, v, factor=factor)
     out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
     v4_apply(p, coef, q, out, device)
     return out.reshape(shape[0], shape[1], C)
@@ -278,10 +327,19 @@ class PlainStage:
         nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "persistent sites")
         return self.masks, self.energies
 
+    def prepare_v4(self):
+        """Queue the point-only part of the v4 solve (Green matrix, projection, Cholesky) on a
+        side stream.  Issued right after the sites launch, it runs on the CUs that the launch's
+        last partial round of sites leaves idle; aggregate() waits for it."""
+        if getattr(self, "_v4", None) is None:
+            self._v4 = V4Ahead(self.device)
+        self._v4.start(self.offsets.astype(np.float64))
+
     def aggregate(self):
         cfg = self.cfg
         pts = self.offsets.astype(np.float64)
-        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device)
+        prep = self._v4.take() if getattr(self, "_v4", None) is not None else None
+        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device, prepared=prep)
         m = masks[..., 0]
         mean = m.mean()
         self.mask_raw, self.mask_mean = m, mean
@@ -289,13 +347,15 @@ class PlainStage:
         return self.mask
 
     def run(self):
-        """plan + propagation + aggregation, each bracketed by a device synchronisation."""
+        """plan + propagation + aggregation, each bracketed by a device synchronisation.  The
+        point-only part of the v4 solve is queued during propagation (prepare_v4)."""
         t = time.perf_counter()
         self.plan()
         torch.cuda.synchronize(self.device)
         self.timing["plan"] = time.perf_counter() - t
         t = time.perf_counter()
         self.propagate()
+        self.prepare_v4()
         torch.cuda.synchronize(self.device)
         self.timing["propagation"] = time.perf_counter() - t
         t = time.perf_counter()
