@@ -167,7 +167,6 @@ def main():
         ev0.record()
         st.propagate()
         ev1.record()
-        st.prepare_v4()  # point-only v4 work on a side stream, as PlainStage.run does
         torch.cuda.synchronize()
         tq = time.perf_counter()
         st.aggregate()

@@ -297,13 +297,9 @@ def _reflectors(points):
 class V4Factor:
     """The part of the v4 solve that depends on the points only (see v4_factor)."""
 
-    def __init__(self, G, G0, V, T, L, Z, S, A12, info):
+    def __init__(self, G, G0, V, T, L, A11, A12, info):
         self.G, self.G0, self.V, self.T, self.L = G, G0, V, T, L
-        self.Z, self.S, self.A12, self.info = Z, S, A12, info
-
-    def tensors(self):
-        return [t for t in (self.G, self.G0, self.V, self.T, self.Z, self.S, self.info)
-                if isinstance(t, torch.Tensor)]
+        self.A11, self.A12, self.info = A11, A12, info
 
 
 def v4_factor(points, G, *, keep_copy=True, reflectors=None):
@@ -314,9 +310,8 @@ def v4_factor(points, G, *, keep_copy=True, reflectors=None):
     to the polynomials 1, x, y it is positive definite (the r^2 part vanishes there, the thin-
     plate r^2 ln r part is positive).  With Q = H1 H2 H3 the Householder reflectors of the
     QR of P = [1 x y], A = Q^T G Q has A22 = A[3:, 3:] positive definite.  This computes A
-    (G is overwritten), the Cholesky factor L of A22 (maomi_cholesky_f64 on the device),
-    Z = A22^-1 A12^T and the 3 x 3 Schur complement S = A11 - A12 Z.  The values enter only
-    in v4_finish, so this part can run while the values are still being computed.
+    (G is overwritten) and the Cholesky factor L of A22 (maomi_cholesky_f64 on the device).
+    The values enter only in v4_finish.
     keep_copy keeps G's original for the LU fallback; reflectors = (V, T) as device tensors, or
     False for rank below 3, from _reflectors computed earlier.  Returns None when P = [1 x y] has rank
     below 3 (points on one line, repeated points); no host synchronisation after the
@@ -332,7 +327,7 @@ def v4_factor(points, G, *, keep_copy=True, reflectors=None):
         reflectors = False if ref is None else tuple(
             torch.as_tensor(a, dtype=G.dtype, device=G.device) for a in ref)
     if reflectors is False:
-        return V4Factor(None, G0, None, None, None, None, None, None, None)
+        return V4Factor(None, G0, None, None, None, None, None, None)
     V, T = reflectors
     # Q^T G Q = G - (N V^T + V N^T) with M = G V, N = M T - V S / 2, S = T^T V^T M T: one pass
     # over G for M and one symmetric rank-6 update.
@@ -346,9 +341,7 @@ def v4_factor(points, G, *, keep_copy=True, reflectors=None):
         L = A22  # lower triangle; the upper keeps A22's own (symmetric) values
     else:
         L, info = torch.linalg.cholesky_ex(A22)
-    Z = torch.cholesky_solve(A12.T.contiguous(), L)
-    S = A11 - A12 @ Z
-    return V4Factor(G, G0, V, T, L, Z, S, A12, info)
+    return V4Factor(G, G0, V, T, L, A11, A12, info)
 
 
 def v4_finish(f, v):
@@ -359,9 +352,10 @@ def v4_finish(f, v):
     v = v.reshape(f.V.shape[0], -1)
     V, T = f.V, f.T
     y = v - V @ (T.T @ (V.T @ v))  # Q^T v
-    z = torch.cholesky_solve(y[3:].contiguous(), f.L)
-    a = torch.linalg.solve(f.S, y[:3] - f.A12 @ z)
-    x = torch.cat([a, z - f.Z @ a], 0)
+    X = torch.cholesky_solve(torch.cat([f.A12.T, y[3:]], 1), f.L)  # A22^-1 [A12^T y2]
+    Z, z = X[:, :3], X[:, 3:]
+    a = torch.linalg.solve(f.A11 - f.A12 @ Z, y[:3] - f.A12 @ z)  # 3 x 3 Schur complement
+    x = torch.cat([a, z - Z @ a], 0)
     return (x - V @ (T @ (V.T @ x))).contiguous()  # Q x
 
 

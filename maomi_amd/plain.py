@@ -126,41 +126,9 @@ def v4_prepare(points, device):
     return p, G, factor
 
 
-class V4Ahead:
-    """The point-only part of the v4 solve queued on a side stream (default priority) while the
-    main stream computes the values; take() orders the main stream after it."""
-
-    def __init__(self, device):
-        self.device, self.side, self.ready = device, None, None
-
-    def start(self, points):
-        if self.side is None:
-            self.side = torch.cuda.Stream(self.device)
-        with torch.cuda.stream(self.side):
-            prep = v4_prepare(points, self.device)
-            ev = torch.cuda.Event()
-            ev.record(self.side)
-        self.ready = (prep, ev)
-
-    def take(self):
-        """The prepared (points, G, factor), with the current stream ordered after it and its
-        tensors marked as used there; None if nothing was started."""
-        if self.ready is None:
-            return None
-        prep, ev = self.ready
-        self.ready = None
-        main = torch.cuda.current_stream(self.device)
-        main.wait_event(ev)
-        p, G, factor = prep
-        for t in [p, G] + (factor.tensors() if factor is not None else []):
-            t.record_stream(main)
-        return prep
-
-
 def v4_masks(points, values, shape, device, prepared=None):
     """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153).
-    prepared: a v4_prepare result for these points (possibly from another stream, already
-    ordered before this call)."""
+    prepared: a v4_prepare result for these points."""
     from .numerics import v4_apply, v4_coefficients
     v = torch.as_tensor(values, dtype=torch.float64, device=device)
     if v.ndim == 1:
@@ -327,19 +295,10 @@ class PlainStage:
         nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "persistent sites")
         return self.masks, self.energies
 
-    def prepare_v4(self):
-        """Queue the point-only part of the v4 solve (Green matrix, projection, Cholesky) on a
-        side stream.  Issued right after the sites launch, it runs on the CUs that the launch's
-        last partial round of sites leaves idle; aggregate() waits for it."""
-        if getattr(self, "_v4", None) is None:
-            self._v4 = V4Ahead(self.device)
-        self._v4.start(self.offsets.astype(np.float64))
-
     def aggregate(self):
         cfg = self.cfg
         pts = self.offsets.astype(np.float64)
-        prep = self._v4.take() if getattr(self, "_v4", None) is not None else None
-        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device, prepared=prep)
+        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device)
         m = masks[..., 0]
         mean = m.mean()
         self.mask_raw, self.mask_mean = m, mean
@@ -347,15 +306,13 @@ class PlainStage:
         return self.mask
 
     def run(self):
-        """plan + propagation + aggregation, each bracketed by a device synchronisation.  The
-        point-only part of the v4 solve is queued during propagation (prepare_v4)."""
+        """plan + propagation + aggregation, each bracketed by a device synchronisation."""
         t = time.perf_counter()
         self.plan()
         torch.cuda.synchronize(self.device)
         self.timing["plan"] = time.perf_counter() - t
         t = time.perf_counter()
         self.propagate()
-        self.prepare_v4()
         torch.cuda.synchronize(self.device)
         self.timing["propagation"] = time.perf_counter() - t
         t = time.perf_counter()

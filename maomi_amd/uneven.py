@@ -19,7 +19,7 @@ import torch
 
 from . import _native as nat
 from .plain import (SiteTables, _CLASSES, _SLOT, base_beams, check_screen_range, make_pupil,
-                    V4Ahead, v4_masks)
+                    v4_masks)
 from .workloads import PlainConfig, beam_parameters, zernike_weights
 
 
@@ -223,17 +223,10 @@ class UnevenStage:
         off, n = int(self.mask_off[i]), int(self.sizes[i])
         return self.masks[off:off + n * n].reshape(n, n)
 
-    def prepare_v4(self):
-        """Queue the point-only part of the v4 solve on a side stream, beside the launch."""
-        if getattr(self, "_v4", None) is None:
-            self._v4 = V4Ahead(self.device)
-        self._v4.start(self.centres.astype(np.float64))
-
     def aggregate(self):
         cfg = self.cfg
-        prep = self._v4.take() if getattr(self, "_v4", None) is not None else None
         m = v4_masks(self.centres.astype(np.float64), self.energies, (cfg.fov, cfg.fov),
-                     self.device, prepared=prep)[..., 0]
+                     self.device)[..., 0]
         self.mask_raw, self.mask_mean = m, m.mean()
         self.mask = m / self.mask_mean
         return self.mask
@@ -245,7 +238,6 @@ class UnevenStage:
         self.timing["plan"] = time.perf_counter() - t
         t = time.perf_counter()
         self.propagate()
-        self.prepare_v4()
         torch.cuda.synchronize(self.device)
         self.timing["propagation"] = time.perf_counter() - t
         t = time.perf_counter()

@@ -120,19 +120,3 @@ def test_complex_screens_equal_phase_screens(gpu_lib, n, slices, fov):
         out.append((st.masks.cpu().numpy(), st.energies.cpu().numpy()))
     assert rel_l2(out[1][0], out[0][0]) < 1e-7
     assert rel_l2(out[1][1], out[0][1]) < 1e-7
-
-
-def test_run_with_v4_on_side_stream_equals_one_stream(gpu_lib):
-    """run() queues the point-only part of the v4 solve on a side stream while the sites
-    launch runs; the mask equals propagate() + aggregate() on one stream, bit for bit, over
-    repeated runs with eager work between them."""
-    import torch
-    from maomi_amd.workloads import CONFIG1
-    cfg = replace(CONFIG1, sites_per_axis=11)  # 121 sites: two 64-column Cholesky steps
-    st, vol, offs, zw = _run(cfg)
-    st.aggregate()
-    ref = st.mask_raw.clone()
-    for _ in range(3):
-        st.run()
-        assert torch.equal(st.mask_raw, ref)
-        st.energies.sum().item()

@@ -79,24 +79,3 @@ def test_batched_base_beams_match_single_steps(gpu_lib):
         for b, bp in enumerate(bps):
             ref = base_beam(pup, bp, tables.edge2d[n], tw, "cuda")
             assert rel_l2(out[b].cpu().numpy(), ref.cpu().numpy()) < 1e-6
-
-
-def test_uneven_run_with_v4_on_side_stream_equals_one_stream(gpu_lib):
-    from maomi_amd.uneven import UnevenStage
-    from maomi_amd.workloads import PlainConfig, index_volume
-    import torch
-    cfg = PlainConfig(fov=64, n=128, slices=6, sites_per_axis=3, corr_vox=2.0)
-    vol = index_volume(cfg)
-    rng = np.random.default_rng(5)
-    grid = np.stack(np.meshgrid(np.arange(0, 65, 8), np.arange(0, 65, 8), indexing="ij"), -1)
-    centres = grid.reshape(-1, 2)[rng.choice(81, 70, replace=False)]  # distinct points
-    sizes = rng.choice([64, 128], 70)
-    slices = rng.integers(2, 7, 70)
-    print("[uneven] 70 sites, v4 on a side stream", flush=True)
-    st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
-    st.propagate()
-    st.aggregate()
-    ref = st.mask_raw.clone()
-    for _ in range(2):
-        st.run()
-        assert torch.equal(st.mask_raw, ref)
