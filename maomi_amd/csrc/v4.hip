@@ -119,15 +119,18 @@ __global__ void green_matrix_kernel(const double* __restrict__ p, double* __rest
 
 // ---------------------------------------------------------------------------------------------
 // FP64 Cholesky factorisation A = L L^T of the v4 system's positive-definite block (row-major,
-// lower triangle, in place), right-looking in steps of kNb columns.  Per step:
-//   chol_panel_kernel: every workgroup factors the kNb x kNb diagonal block in registers and
-//     LDS (one barrier per column), then each thread solves one row of the panel below,
-//     x L11^T = a; workgroup 0 also stores L11 in the side buffer;
-//   chol_update_kernel: block 0 copies L11 from the side buffer into the diagonal block (the
-//     panel workgroups read that block, so it cannot be written in the same launch); the others
-//     subtract L21 L21^T from the lower 64 x 64 tiles of the trailing matrix.
-// Only the lower triangle is read or written.  The first non-positive (or NaN) pivot sets
-// *info to its 1-based column, as LAPACK's potrf does.
+// lower triangle, in place), right-looking in steps of kNb columns with a one-step lookahead
+// on the diagonal block.  L11 of step k (the factored diagonal block, zero above the diagonal,
+// identity past the matrix) lives in side buffer k % 2.  Per step:
+//   chol_panel_kernel: each thread solves one row of the panel below the diagonal block,
+//     x L11^T = a, with L11 from the side buffer;
+//   chol_update_kernel: block 0 copies L11 into the diagonal block of A; the others subtract
+//     L21 L21^T from the lower 64 x 64 tiles of the trailing matrix, and the block of the first
+//     tile (the next diagonal block) then factors it into the other side buffer, so that the
+//     sequential factorisation overlaps the rest of the update.
+// chol_diag_kernel factors the first diagonal block.  Only the lower triangle is read or
+// written.  The first non-positive (or NaN) pivot sets *info to its 1-based column, as
+// LAPACK's potrf does.
 
 constexpr int kNb = 64;
 constexpr int kCholThreads = 256;
@@ -192,34 +195,49 @@ __device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, i
   chol_lds_barrier();
 }
 
+// L11 of the factored block (sm.lt) to a side buffer, row-major 64 x 64.
+__device__ __forceinline__ void chol_store_l11(const CholShared& sm, double* __restrict__ l11) {
+  for (int e = threadIdx.x; e < kNb * kNb; e += kCholThreads) {
+    const int i = e >> 6, c = e & 63;
+    l11[e] = sm.lt[c][i];
+  }
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double* __restrict__ A,
+                                                                 int64_t lda, int64_t m,
+                                                                 int64_t k,
+                                                                 double* __restrict__ l11,
+                                                                 int* info) {
+  __shared__ __align__(16) CholShared sm;
+  chol_factor_diag(A + k * lda + k, lda, (int)min((int64_t)kNb, m - k), sm, info, k, true);
+  chol_store_l11(sm, l11);
+}
+
+// Panel rows k + kNb .. m - 1 (launched only when there are some, so the block is full).
 __global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(double* __restrict__ A,
                                                                   int64_t lda, int64_t m,
                                                                   int64_t k,
-                                                                  double* __restrict__ l11,
-                                                                  int* info) {
-  __shared__ CholShared sm;
-  const int b = (int)min((int64_t)kNb, m - k);
+                                                                  const double* __restrict__ l11) {
+  __shared__ __align__(16) double lt[kNb][kNb];  // lt[c][i] = L11[i][c]
+  __shared__ double rd[kNb];                     // 1 / L11[j][j]
   const int t = threadIdx.x;
-  chol_factor_diag(A + k * lda + k, lda, b, sm, info, k, blockIdx.x == 0);
-  if (blockIdx.x == 0) {
-    for (int e = t; e < kNb * kNb; e += kCholThreads) {
-      const int i = e >> 6, c = e & 63;
-      l11[e] = sm.lt[c][i];
-    }
+  for (int e = t; e < kNb * kNb; e += kCholThreads) {
+    const int i = e >> 6, c = e & 63;
+    lt[c][i] = l11[e];
   }
-  // panel rows exist only when b == kNb.  They are loaded after the factorisation: held
-  // across it, the 64 row values leave the factorisation no registers to batch its LDS reads.
-  const int64_t r = k + b + (int64_t)blockIdx.x * kCholThreads + t;
-  if (b < kNb || r >= m) return;
+  if (t < kNb) rd[t] = 1.0 / l11[t * kNb + t];
+  __syncthreads();
+  const int64_t r = k + kNb + (int64_t)blockIdx.x * kCholThreads + t;
+  if (r >= m) return;
   double* row = A + r * lda + k;
   double x[kNb];
 #pragma unroll
   for (int l = 0; l < kNb; ++l) x[l] = row[l];
 #pragma unroll
   for (int l = 0; l < kNb; ++l) {
-    x[l] *= sm.rd[l];
+    x[l] *= rd[l];
 #pragma unroll
-    for (int j = l + 1; j < kNb; ++j) x[j] = fma(-x[l], sm.lt[l][j], x[j]);
+    for (int j = l + 1; j < kNb; ++j) x[j] = fma(-x[l], lt[l][j], x[j]);
   }
 #pragma unroll
   for (int l = 0; l < kNb; ++l) row[l] = x[l];
@@ -228,9 +246,15 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(double* __rest
 __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __restrict__ A,
                                                                    int64_t lda, int64_t m,
                                                                    int64_t k,
-                                                                   const double* __restrict__ l11) {
+                                                                   const double* __restrict__ l11,
+                                                                   double* __restrict__ l11_next,
+                                                                   int* info) {
   constexpr int LS = kNb + 1;  // odd row stride: the transposed stores spread over the banks
-  __shared__ double pi[kNb][LS], pj[kNb][LS];
+  // the two transposed panel tiles; the first tile's block reuses the space to factor
+  __shared__ __align__(16) double smem[2 * kNb * LS];
+  static_assert(sizeof(CholShared) <= sizeof(double) * 2 * kNb * LS, "factor scratch fits");
+  double(*pi)[LS] = reinterpret_cast<double(*)[LS]>(smem);
+  double(*pj)[LS] = reinterpret_cast<double(*)[LS]>(smem + kNb * LS);
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
     const int b = (int)min((int64_t)kNb, m - k);
@@ -300,6 +324,13 @@ __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __res
       if (rw < m && c <= rw) A[rw * lda + c] = cv[u][v] - acc[u][v];
     }
   }
+  if (id != 0) return;
+  // first tile = next diagonal block: factor it now (its stores above are visible to the
+  // whole block after the barrier; the LDS of the product is reused)
+  __syncthreads();
+  CholShared& sm = *reinterpret_cast<CholShared*>(smem);
+  chol_factor_diag(A + k1 * lda + k1, lda, (int)min((int64_t)kNb, m - k1), sm, info, k1, true);
+  chol_store_l11(sm, l11_next);
 }
 
 __global__ void chol_info_reset_kernel(int* info) {
@@ -379,7 +410,7 @@ MAOMI_API int maomi_v4_apply_split_f64(const double* points, const double* coef,
 }
 
 MAOMI_API int64_t maomi_cholesky_f64_workspace(void) {
-  return (int64_t)kNb * kNb * (int64_t)sizeof(double);
+  return 2 * (int64_t)kNb * kNb * (int64_t)sizeof(double);
 }
 
 MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
@@ -391,15 +422,23 @@ MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* work
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(chol_info_reset_kernel, dim3(1), dim3(64), 0, st, info);
   MAOMI_CHECK_LAUNCH();
-  for (int64_t k = 0; k < m; k += kNb) {
+  if (m == 0) return MAOMI_OK;
+  double* buf[2] = {workspace, workspace + kNb * kNb};
+  hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(kCholThreads), 0, st, A, lda, m,
+                     (int64_t)0, buf[0], info);
+  MAOMI_CHECK_LAUNCH();
+  for (int64_t k = 0, step = 0; k < m; k += kNb, ++step) {
     const int64_t b = m - k < kNb ? m - k : kNb, mp = m - k - b;
-    const int64_t pblocks = mp > 0 ? (mp + kCholThreads - 1) / kCholThreads : 1;
-    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)pblocks), dim3(kCholThreads), 0, st, A,
-                       lda, m, k, workspace, info);
-    MAOMI_CHECK_LAUNCH();
+    double* cur = buf[step & 1];
+    double* next = buf[(step + 1) & 1];
+    if (mp > 0) {
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((mp + kCholThreads - 1) / kCholThreads)),
+                         dim3(kCholThreads), 0, st, A, lda, m, k, cur);
+      MAOMI_CHECK_LAUNCH();
+    }
     const int64_t T = (mp + kNb - 1) / kNb;
     hipLaunchKernelGGL(chol_update_kernel, dim3((unsigned)(1 + T * (T + 1) / 2)),
-                       dim3(kCholThreads), 0, st, A, lda, m, k, workspace);
+                       dim3(kCholThreads), 0, st, A, lda, m, k, cur, next, info);
     MAOMI_CHECK_LAUNCH();
   }
   return MAOMI_OK;
