@@ -122,8 +122,8 @@ __global__ void green_matrix_kernel(const double* __restrict__ p, double* __rest
 // lower triangle, in place), right-looking in steps of kNb columns with a one-step lookahead
 // on the diagonal block.  L11 of step k (the factored diagonal block, zero above the diagonal,
 // identity past the matrix) lives in side buffer k % 2.  Per step:
-//   chol_panel_kernel: each thread solves one row of the panel below the diagonal block,
-//     x L11^T = a, with L11 from the side buffer;
+//   chol_panel_kernel: solves the rows of the panel below the diagonal block, x L11^T = a,
+//     with L11 from the side buffer (64 rows per block, a wave per 16 columns);
 //   chol_update_kernel: block 0 copies L11 into the diagonal block of A; the others subtract
 //     L21 L21^T from the lower 64 x 64 tiles of the trailing matrix, and the block of the first
 //     tile (the next diagonal block) then factors it into the other side buffer, so that the
@@ -149,11 +149,47 @@ __device__ __forceinline__ void chol_lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Step J of the factorisation for the wave holding columns 16 G .. 16 G + 15 of every row:
+// S[i][c] -= S[i][J] S[c][J] / S[J][J] for its columns c > J.  G and J are template arguments,
+// so which columns a step updates is known at compile time.
+template <int G, int J>
+__device__ __forceinline__ void chol_factor_step(double (&s)[16], int i, const double* cj) {
+  if constexpr (G >= (J >> 4) && 16 * G + 15 > J) {
+    const double lij = cj[i] / cj[J];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc)
+      if (16 * G + cc > J) s[cc] = fma(-lij, cj[16 * G + cc], s[cc]);
+  }
+}
+
+// Steps J .. kNb - 1.  The barrier is in code every wave runs; only the update is dispatched on
+// the wave's (uniform) column group.
+template <int J>
+struct CholFactorSteps {
+  __device__ __forceinline__ static void run(double (&s)[16], int i, int t, int g,
+                                             CholShared& sm) {
+    double* cj = sm.col[J & 1];
+    if (g == (J >> 4)) cj[i] = s[J & 15];
+    chol_lds_barrier();
+    if (t == 0) sm.piv[J] = cj[J];
+    switch (g) {
+      case 0: chol_factor_step<0, J>(s, i, cj); break;
+      case 1: chol_factor_step<1, J>(s, i, cj); break;
+      case 2: chol_factor_step<2, J>(s, i, cj); break;
+      default: chol_factor_step<3, J>(s, i, cj); break;
+    }
+    CholFactorSteps<J + 1>::run(s, i, t, g, sm);
+  }
+};
+template <>
+struct CholFactorSteps<kNb> {
+  __device__ __forceinline__ static void run(double (&)[16], int, int, int, CholShared&) {}
+};
+
 // Factor the b x b block at D (b <= kNb).  Thread t holds row i = t % 64, columns
 // 16 g .. 16 g + 15 (g = t / 64, uniform in a wave) of the block; rows and columns past b are
-// the identity.  Step j broadcasts column j through LDS and updates
-// S[i][c] -= S[i][j] S[c][j] / S[j][j] for c > j (a select keeps c <= j); column j itself is
-// not written in step j, so one barrier per step suffices.
+// the identity.  Step j broadcasts column j through LDS; column j itself is not written in
+// step j, so one barrier per step suffices.
 __device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, int64_t lda, int b,
                                                  CholShared& sm, int* info, int64_t k0,
                                                  bool report) {
@@ -165,20 +201,7 @@ __device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, i
     const int c = 16 * g + cc;
     s[cc] = (i < b && c <= i) ? D[(int64_t)i * lda + c] : (c == i ? 1.0 : 0.0);
   }
-#pragma unroll
-  for (int j = 0; j < kNb; ++j) {
-    double* cj = sm.col[j & 1];
-    if (g == (j >> 4)) cj[i] = s[j & 15];
-    chol_lds_barrier();
-    const double piv = cj[j];
-    const double lij = cj[i] / piv;
-#pragma unroll
-    for (int cc = 0; cc < 16; ++cc) {
-      const double u = fma(-lij, cj[16 * g + cc], s[cc]);
-      s[cc] = 16 * g + cc > j ? u : s[cc];
-    }
-    if (t == 0) sm.piv[j] = piv;
-  }
+  CholFactorSteps<0>::run(s, i, t, g, sm);
   chol_lds_barrier();
   if (report && t == 0) {
     int bad = 0;
@@ -213,34 +236,72 @@ __global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double* _
   chol_store_l11(sm, l11);
 }
 
-// Panel rows k + kNb .. m - 1 (launched only when there are some, so the block is full).
+// Step L of the row solve x L11^T = a for the wave holding columns 16 G .. 16 G + 15 of its
+// 64 rows: x[j] -= x[L] L11[j][L] for its columns j > L (x[L] from LDS).
+template <int G, int L>
+__device__ __forceinline__ void chol_panel_step(double (&x)[16], int lane,
+                                                const double (*lt)[kNb], const double* xl) {
+  if constexpr (G >= (L >> 4) && 16 * G + 15 > L) {
+    const double v = xl[lane];
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj)
+      if (16 * G + jj > L) x[jj] = fma(-v, lt[L][16 * G + jj], x[jj]);
+  }
+}
+
+template <int L>
+struct CholPanelSteps {
+  __device__ __forceinline__ static void run(double (&x)[16], int lane, int g,
+                                             const double (*lt)[kNb], const double* rd,
+                                             double (*xs)[64]) {
+    if (g == (L >> 4)) {  // the owner wave finishes x[L] and hands it to the others
+      x[L & 15] *= rd[L];
+      xs[L & 1][lane] = x[L & 15];
+    }
+    chol_lds_barrier();
+    switch (g) {
+      case 0: chol_panel_step<0, L>(x, lane, lt, xs[L & 1]); break;
+      case 1: chol_panel_step<1, L>(x, lane, lt, xs[L & 1]); break;
+      case 2: chol_panel_step<2, L>(x, lane, lt, xs[L & 1]); break;
+      default: chol_panel_step<3, L>(x, lane, lt, xs[L & 1]); break;
+    }
+    CholPanelSteps<L + 1>::run(x, lane, g, lt, rd, xs);
+  }
+};
+template <>
+struct CholPanelSteps<kNb> {
+  __device__ __forceinline__ static void run(double (&)[16], int, int, const double (*)[kNb],
+                                             const double*, double (*)[64]) {}
+};
+
+// Panel rows k + kNb .. m - 1, 64 per block (launched only when there are some): lane = row,
+// wave g = columns 16 g .. 16 g + 15.
 __global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(double* __restrict__ A,
                                                                   int64_t lda, int64_t m,
                                                                   int64_t k,
                                                                   const double* __restrict__ l11) {
   __shared__ __align__(16) double lt[kNb][kNb];  // lt[c][i] = L11[i][c]
   __shared__ double rd[kNb];                     // 1 / L11[j][j]
-  const int t = threadIdx.x;
+  __shared__ double xs[2][64];                   // x[L] of the current step, double-buffered
+  const int t = threadIdx.x, lane = t & 63;
+  const int g = __builtin_amdgcn_readfirstlane(t >> 6);
   for (int e = t; e < kNb * kNb; e += kCholThreads) {
     const int i = e >> 6, c = e & 63;
     lt[c][i] = l11[e];
   }
   if (t < kNb) rd[t] = 1.0 / l11[t * kNb + t];
+  const int64_t r = k + kNb + (int64_t)blockIdx.x * 64 + lane;
+  const bool live = r < m;
+  double* row = A + (live ? r : k) * lda + k + 16 * g;
+  double x[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) x[jj] = live ? row[jj] : 0.0;
   __syncthreads();
-  const int64_t r = k + kNb + (int64_t)blockIdx.x * kCholThreads + t;
-  if (r >= m) return;
-  double* row = A + r * lda + k;
-  double x[kNb];
+  CholPanelSteps<0>::run(x, lane, g, lt, rd, xs);
+  if (live) {
 #pragma unroll
-  for (int l = 0; l < kNb; ++l) x[l] = row[l];
-#pragma unroll
-  for (int l = 0; l < kNb; ++l) {
-    x[l] *= rd[l];
-#pragma unroll
-    for (int j = l + 1; j < kNb; ++j) x[j] = fma(-x[l], lt[l][j], x[j]);
+    for (int jj = 0; jj < 16; ++jj) row[jj] = x[jj];
   }
-#pragma unroll
-  for (int l = 0; l < kNb; ++l) row[l] = x[l];
 }
 
 __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __restrict__ A,
@@ -432,8 +493,8 @@ MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* work
     double* cur = buf[step & 1];
     double* next = buf[(step + 1) & 1];
     if (mp > 0) {
-      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((mp + kCholThreads - 1) / kCholThreads)),
-                         dim3(kCholThreads), 0, st, A, lda, m, k, cur);
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((mp + 63) / 64)), dim3(kCholThreads), 0,
+                         st, A, lda, m, k, cur);
       MAOMI_CHECK_LAUNCH();
     }
     const int64_t T = (mp + kNb - 1) / kNb;
