@@ -102,15 +102,17 @@ def resample_matrix(n, ss):
     scale = 1 / ss
     shift = ss / 2 - 0.5
     idx, w = resize_plan(n, scale)
+    rows = np.broadcast_to(np.arange(idx.shape[0])[:, None], idx.shape)
+    pos = idx - shift
+    base = np.floor(pos).astype(np.int64)
+    frac = pos - base
     mat = np.zeros((idx.shape[0], n))
-    for i in range(idx.shape[0]):
-        for j, wt in zip(idx[i], w[i]):
-            base = int(np.floor(j - shift))
-            frac = j - shift - base
-            if 0 <= base < n:
-                mat[i, base] += wt * (1 - frac)
-            if 0 <= base + 1 < n:
-                mat[i, base + 1] += wt * frac
+    # the linear translation's two taps per resize tap, accumulated in tap order
+    cols = np.stack([base, base + 1], -1).reshape(idx.shape[0], -1)
+    vals = np.stack([w * (1 - frac), w * frac], -1).reshape(idx.shape[0], -1)
+    r = np.repeat(rows, 2, axis=1)
+    ok = (cols >= 0) & (cols < n)
+    np.add.at(mat, (r[ok], cols[ok]), vals[ok])
     return mat
 
 
