@@ -83,11 +83,13 @@ template <> struct Split<512> { static constexpr int N1 = 16, N2 = 32; };
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 // Tile geometry for a block of NT threads transforming L lines of length N.
+// N = 512 (16 x 32) sizes the tile by the smaller factor, so that its 32-point stages fill the
+// block; the 16-point stages then take two jobs per thread.
 template <int N, int NT>
 struct Tile {
   static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
-  static constexpr int JMAX = cmax(N1, N2);
-  static constexpr int L = NT / JMAX;                     // lines per tile
+  static constexpr int JMAX = cmax(N1, N2), JMIN = N1 < N2 ? N1 : N2;
+  static constexpr int L = N >= 512 ? NT / JMIN : NT / JMAX;  // lines per tile
   static constexpr int LS = cmax(N1 * (N2 + 1), N2 * (N1 + 1)) | 1;  // odd line stride
   static constexpr int LDS2 = L * LS;                     // float2 slots of exchange buffer
   static_assert(N % L == 0, "tile must divide the field");

@@ -68,10 +68,13 @@ template <> struct ClassIdx<256> { static constexpr int v = 2; };
 template <> struct ClassIdx<512> { static constexpr int v = 3; };
 
 // Block shapes: NT threads transform L = NT / max(N1, N2) lines per LDS tile.
-template <int NT>
+// Exchange-buffer slots for field sides up to NMAX.
+template <int NT, int NMAX>
 constexpr int lds_slots() {
-  return fftk::cmax(fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2),
-                    fftk::cmax(fftk::Tile<256, NT>::LDS2, fftk::Tile<512, NT>::LDS2));
+  int m = fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2);
+  if (NMAX >= 256) m = fftk::cmax(m, fftk::Tile<256, NT>::LDS2);
+  if (NMAX >= 512) m = fftk::cmax(m, fftk::Tile<512, NT>::LDS2);
+  return m;
 }
 constexpr int kNmax = 512;
 
@@ -101,9 +104,9 @@ struct SitesArgs {
   int psf_z0, psf_nz, psf_c0, psf_w;
 };
 
-template <int NT>
+template <int NT, int NMAX>
 struct Smem {
-  float2 x[lds_slots<NT>()];  // line exchange buffer
+  float2 x[lds_slots<NT, NMAX>()];  // line exchange buffer
   float2 tw[kNmax];           // W_N^j
   float2 q[kNmax];            // 1-D Fresnel factor (FFT order, times 1/N)
   float eg[kNmax];            // 1-D edge window
@@ -135,34 +138,34 @@ enum RowKind { kFirst = 0, kMiddle = 1, kLast = 2, kOnly = 3 };
 // Row pass of slice s: [IFFT_y] -> screen, |u|^4 trapezoid -> [FFT_y].
 // Barriers inside a pass only order LDS; each thread's mask pixels and the phase volume need
 // no cross-thread ordering.
-template <int N, int NT, int MODE, int KIND>
+template <int N, int NT, int MODE, int KIND, class SM>
 __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, int s, int tid,
-                                         Smem<NT>& sm, double& esum) {
+                                         SM& sm, double& esum) {
   constexpr bool half = MODE == kScreens16, PSF = MODE == kScreens32Psf;
   constexpr bool cplx = MODE == kScreensC64;
   constexpr bool FIRST = KIND == kFirst || KIND == kOnly, LAST = KIND == kLast || KIND == kOnly;
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
+  constexpr int J2 = (L * N2 + NT - 1) / NT;  // stage-2 jobs per thread
   constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
+  static_assert(!hoist || J2 == 1, "hoisted loads serve one stage-2 job per thread");
   const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
   const float sscale = half && !LAST ? a.slice_scale[s] : 1.0f;
   const int64_t pslice = (int64_t)s * a.H;
   const bool p1 = tid < L * N1;
   const int l1 = tid / N1, n1 = tid % N1;
-  const bool p2 = tid < L * N2;
-  const int l = tid / N2, k2 = tid % N2;
   float2 pre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
 #pragma unroll 1
   for (int t = 0; t < N / L; ++t) {
     const int row0 = t * L;
-    const int x = row0 + l;
-    float* mrow = c.mask + (int64_t)x * N + k2;
-    const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
-    // this tile's screen phases and mask pixels, requested before stage 1 so that their
-    // latency overlaps the stage-1 loads, transform and exchange
+    // this tile's screen phases and mask pixels (first stage-2 job), requested before stage 1
+    // so that their latency overlaps the stage-1 loads, transform and exchange
     float phv[cplx ? 1 : N1], mkv[N1];
     float2 scv[cplx ? N1 : 1];
-    if (hoist && p2) {
+    if (hoist && tid < L * N2) {
+      const int x = row0 + tid / N2, k2 = tid % N2;
+      const float* mrow = c.mask + (int64_t)x * N + k2;
+      const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
         if (!LAST) {
@@ -193,67 +196,79 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
       }
       lds_barrier();
     }
-    float2 v[N1];
-    if (p2) {
-      if (FIRST) {
-        const float2* src = c.base + (int64_t)x * N + k2;
+    // Stage-2 jobs, one at a time: job jj covers lines [jj NT / N2, (jj + 1) NT / N2) of the
+    // tile, so its results can be stored (after a barrier for the reads of its lines) while
+    // the next job reads the other lines.
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) v[k1] = src[N2 * k1];
-      } else {
-        const float2* src = sm.x + l * LS + k2;
-#pragma unroll
-        for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-        fftk::fft<N1, true>(v);
-      }
-      const float ex = sm.eg[x];
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) {
-        const int y = k2 + N2 * k1;
-        if (!LAST) {
-          float sn, cs;
-          if (cplx) {
-            const float2 e = hoist ? scv[k1] : a.screen[prow + N2 * k1];
-            cs = e.x;
-            sn = e.y;
-          } else {
-            const float phi = hoist ? phv[k1]
-                              : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                                     : a.phase[prow + N2 * k1];
-            screen_sincos(phi, &sn, &cs);
-          }
-          const float sg = ex * sm.eg[y];
-          v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
-        }
-        const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
-        if (PSF && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
-          const int cx = x - a.psf_c0, cy = y - a.psf_c0;
-          if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
-            a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + c.site) * a.psf_w + cx) * a.psf_w + cy] =
-                pw * pw;
-        }
-        const float m = wgt * (pw * pw);
-        const float acc = FIRST ? m : (hoist ? mkv[k1] : mrow[N2 * k1]) + m;
-        mrow[N2 * k1] = acc;
-        if (LAST) esum += (double)acc;
-      }
-      if (!LAST) {  // stage 1 of the forward transform along y (factors swapped)
-        fftk::fft<N1, false>(v);
-#pragma unroll
-        for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
-      }
-    }
-    if (kPrefetchRows && !FIRST && p1 && t + 1 < N / L) {
-      const float2* src = c.fld + (int64_t)(row0 + L + l1) * N + n1;
-#pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
-    }
-    lds_barrier();
-    if (!LAST) {
+    for (int jj = 0; jj < J2; ++jj) {
+      const int jb = tid + jj * NT;
+      const bool p2 = jb < L * N2;
+      const int l = jb / N2, k2 = jb % N2;
+      const int x = row0 + l;
+      float2 v[N1];
       if (p2) {
+        float* mrow = c.mask + (int64_t)x * N + k2;
+        const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
+        if (FIRST) {
+          const float2* src = c.base + (int64_t)x * N + k2;
+#pragma unroll
+          for (int k1 = 0; k1 < N1; ++k1) v[k1] = src[N2 * k1];
+        } else {
+          const float2* src = sm.x + l * LS + k2;
+#pragma unroll
+          for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+          fftk::fft<N1, true>(v);
+        }
+        const float ex = sm.eg[x];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+          const int y = k2 + N2 * k1;
+          if (!LAST) {
+            float sn, cs;
+            if (cplx) {
+              const float2 e = hoist ? scv[k1] : a.screen[prow + N2 * k1];
+              cs = e.x;
+              sn = e.y;
+            } else {
+              const float phi = hoist ? phv[k1]
+                                : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
+                                       : a.phase[prow + N2 * k1];
+              screen_sincos(phi, &sn, &cs);
+            }
+            const float sg = ex * sm.eg[y];
+            v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
+          }
+          const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
+          if (PSF && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
+            const int cx = x - a.psf_c0, cy = y - a.psf_c0;
+            if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
+              a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + c.site) * a.psf_w + cx) * a.psf_w +
+                        cy] = pw * pw;
+          }
+          const float m = wgt * (pw * pw);
+          const float acc = FIRST ? m : (hoist ? mkv[k1] : mrow[N2 * k1]) + m;
+          mrow[N2 * k1] = acc;
+          if (LAST) esum += (double)acc;
+        }
+        if (!LAST) {  // stage 1 of the forward transform along y (factors swapped)
+          fftk::fft<N1, false>(v);
+#pragma unroll
+          for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
+        }
+      }
+      if (kPrefetchRows && jj == J2 - 1 && !FIRST && p1 && t + 1 < N / L) {
+        const float2* src = c.fld + (int64_t)(row0 + L + l1) * N + n1;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
+      }
+      lds_barrier();
+      if (!LAST && p2) {
         float2* dst = sm.x + l * LS + k2 * (N1 + 1);
 #pragma unroll
         for (int j = 0; j < N1; ++j) dst[j] = v[j];
       }
+    }
+    if (!LAST) {
       lds_barrier();
       if (p1) {  // stage 2 of the forward transform along y
         float2 w[N2];
@@ -272,14 +287,13 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
 }
 
 // Column pass: FFT_x -> * Q (separable, 1/N^2 folded in) -> IFFT_x.
-template <int N, int NT>
-__device__ __forceinline__ void column_pass(const SiteCtx& c, int tid, Smem<NT>& sm) {
+template <int N, int NT, class SM>
+__device__ __forceinline__ void column_pass(const SiteCtx& c, int tid, SM& sm) {
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
+  constexpr int J2 = (L * N2 + NT - 1) / NT;  // stage-2 jobs per thread
   const bool c1 = tid < L * N1;
   const int lc = tid % L, nc = tid / L;
-  const bool p2 = tid < L * N2;
-  const int l = tid % L, k2 = tid / L;
   constexpr bool prefetch = kPrefetchCols && N <= 256;  // N = 512: 32-point lines, no room
   float2 cpre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
 #pragma unroll 1
@@ -301,29 +315,38 @@ __device__ __forceinline__ void column_pass(const SiteCtx& c, int tid, Smem<NT>&
       for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[nc * j]);
     }
     lds_barrier();
-    float2 v[N1];
-    if (p2) {
-      const float2* src = sm.x + l * LS + k2;
+    // Stage-2 jobs, one at a time.  With one job per thread the line index runs fastest; with
+    // two, job jj covers lines [jj NT / N2, (jj + 1) NT / N2), so its results can be stored
+    // (after a barrier for the reads of its lines) while the next job reads the other lines.
 #pragma unroll
-      for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-      fftk::fft<N1, false>(v);
-      const float2 qc = sm.q[col0 + l];
+    for (int jj = 0; jj < J2; ++jj) {
+      const int jb = tid + jj * NT;
+      const bool p2 = jb < L * N2;
+      const int l = J2 == 1 ? jb % L : jb / N2, k2 = J2 == 1 ? jb / L : jb % N2;
+      float2 v[N1];
+      if (p2) {
+        const float2* src = sm.x + l * LS + k2;
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
-      fftk::fft<N1, true>(v);
+        for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+        fftk::fft<N1, false>(v);
+        const float2 qc = sm.q[col0 + l];
 #pragma unroll
-      for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
-    }
-    if (prefetch && c1 && t + 1 < N / L) {
-      const float2* src = c.fld + (int64_t)nc * N + col0 + L + lc;
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
+        fftk::fft<N1, true>(v);
 #pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
-    }
-    lds_barrier();
-    if (p2) {
-      float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+      }
+      if (prefetch && jj == J2 - 1 && c1 && t + 1 < N / L) {
+        const float2* src = c.fld + (int64_t)nc * N + col0 + L + lc;
 #pragma unroll
-      for (int j = 0; j < N1; ++j) dst[j] = v[j];
+        for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
+      }
+      lds_barrier();
+      if (p2) {
+        float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) dst[j] = v[j];
+      }
     }
     lds_barrier();
     if (c1) {
@@ -341,8 +364,8 @@ __device__ __forceinline__ void column_pass(const SiteCtx& c, int tid, Smem<NT>&
   __syncthreads();  // the column pass's stores are read by other threads in the next row pass
 }
 
-template <int N, int NT, int MODE>
-__device__ __forceinline__ void run_site(const SitesArgs& a, int site, Smem<NT>& sm) {
+template <int N, int NT, int MODE, class SM>
+__device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
   constexpr int C = ClassIdx<N>::v;
   // The thread index is made opaque once per site: values derived from it are then computed
   // inside this site's code path instead of being hoisted above the persistent site loop,
@@ -379,14 +402,14 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, Smem<NT>&
 // One field side for every site of the launch.
 template <int N, int NT, int MODE>
 __global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_kernel(SitesArgs a) {
-  __shared__ Smem<NT> sm;
+  __shared__ Smem<NT, N> sm;
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, MODE>(a, a.order[i], sm);
 }
 
 // Sites of mixed field sides in one launch (the order list puts large sites first).
 template <int NT>
 __global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_mixed_kernel(SitesArgs a) {
-  __shared__ Smem<NT> sm;
+  __shared__ Smem<NT, kNmax> sm;
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
     const int site = a.order[i];
     switch (a.site_n[site]) {
