@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of sites_kernel<256> on the 4225-site FP32 workload, from two rocprofv3
 # --pmc runs of this script: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy,
 # --copy-probe); the runs and their commands are in profiles/sites_kernel_256_config2_pmc.md.
-TRAFFIC_CONFIG2_FP32 = (2.0 * 1.658e8 + 1.0 * 3.254e8) * 1024
+TRAFFIC_CONFIG2_FP32 = (2.0 * 1.6565e8 + 1.0 * 3.2545e8) * 1024
 METRIC = "sites/s on 4225-site optical stage; surrogate vs propagator at ≤1% rel L2"
 
 
