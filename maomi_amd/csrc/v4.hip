@@ -149,41 +149,37 @@ __device__ __forceinline__ void chol_lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Step J of the factorisation for the wave holding columns 16 G .. 16 G + 15 of every row:
-// S[i][c] -= S[i][J] S[c][J] / S[J][J] for its columns c > J.  G and J are template arguments,
-// so which columns a step updates is known at compile time.
-template <int G, int J>
-__device__ __forceinline__ void chol_factor_step(double (&s)[16], int i, const double* cj) {
-  if constexpr (G >= (J >> 4) && 16 * G + 15 > J) {
-    const double lij = cj[i] / cj[J];
-#pragma unroll
-    for (int cc = 0; cc < 16; ++cc)
-      if (16 * G + cc > J) s[cc] = fma(-lij, cj[16 * G + cc], s[cc]);
-  }
-}
-
-// Steps J .. kNb - 1.  The barrier is in code every wave runs; only the update is dispatched on
-// the wave's (uniform) column group.
-template <int J>
-struct CholFactorSteps {
-  __device__ __forceinline__ static void run(double (&s)[16], int i, int t, int g,
+// Step j = 16 jg + JJ of the factorisation: column j is broadcast through LDS, then the wave
+// holding columns 16 g .. 16 g + 15 of every row updates S[i][c] -= S[i][j] S[c][j] / S[j][j]
+// for its columns c > j: all 16 when g > jg, those past JJ when g == jg (JJ is a template
+// argument, g and jg are uniform).  The column groups jg run in a loop and the 16 steps of a
+// group are unrolled, so a group's code stays in the instruction cache across the groups.
+template <int JJ>
+struct CholGroupSteps {
+  __device__ __forceinline__ static void run(double (&s)[16], int i, int t, int g, int jg,
                                              CholShared& sm) {
-    double* cj = sm.col[J & 1];
-    if (g == (J >> 4)) cj[i] = s[J & 15];
+    const int j = 16 * jg + JJ;
+    double* cj = sm.col[JJ & 1];
+    if (g == jg) cj[i] = s[JJ];
     chol_lds_barrier();
-    if (t == 0) sm.piv[J] = cj[J];
-    switch (g) {
-      case 0: chol_factor_step<0, J>(s, i, cj); break;
-      case 1: chol_factor_step<1, J>(s, i, cj); break;
-      case 2: chol_factor_step<2, J>(s, i, cj); break;
-      default: chol_factor_step<3, J>(s, i, cj); break;
+    if (t == 0) sm.piv[j] = cj[j];
+    if (g > jg) {
+      const double lij = cj[i] / cj[j];
+      const double* cg = cj + 16 * g;
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) s[cc] = fma(-lij, cg[cc], s[cc]);
+    } else if (g == jg && JJ < 15) {
+      const double lij = cj[i] / cj[j];
+      const double* cg = cj + 16 * g;
+#pragma unroll
+      for (int cc = JJ + 1; cc < 16; ++cc) s[cc] = fma(-lij, cg[cc], s[cc]);
     }
-    CholFactorSteps<J + 1>::run(s, i, t, g, sm);
+    CholGroupSteps<JJ + 1>::run(s, i, t, g, jg, sm);
   }
 };
 template <>
-struct CholFactorSteps<kNb> {
-  __device__ __forceinline__ static void run(double (&)[16], int, int, int, CholShared&) {}
+struct CholGroupSteps<16> {
+  __device__ __forceinline__ static void run(double (&)[16], int, int, int, int, CholShared&) {}
 };
 
 // Factor the b x b block at D (b <= kNb).  Thread t holds row i = t % 64, columns
@@ -201,7 +197,8 @@ __device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, i
     const int c = 16 * g + cc;
     s[cc] = (i < b && c <= i) ? D[(int64_t)i * lda + c] : (c == i ? 1.0 : 0.0);
   }
-  CholFactorSteps<0>::run(s, i, t, g, sm);
+#pragma unroll 1
+  for (int jg = 0; jg < 4; ++jg) CholGroupSteps<0>::run(s, i, t, g, jg, sm);
   chol_lds_barrier();
   if (report && t == 0) {
     int bad = 0;
@@ -236,42 +233,39 @@ __global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double* _
   chol_store_l11(sm, l11);
 }
 
-// Step L of the row solve x L11^T = a for the wave holding columns 16 G .. 16 G + 15 of its
-// 64 rows: x[j] -= x[L] L11[j][L] for its columns j > L (x[L] from LDS).
-template <int G, int L>
-__device__ __forceinline__ void chol_panel_step(double (&x)[16], int lane,
-                                                const double (*lt)[kNb], const double* xl) {
-  if constexpr (G >= (L >> 4) && 16 * G + 15 > L) {
-    const double v = xl[lane];
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj)
-      if (16 * G + jj > L) x[jj] = fma(-v, lt[L][16 * G + jj], x[jj]);
-  }
-}
-
-template <int L>
-struct CholPanelSteps {
-  __device__ __forceinline__ static void run(double (&x)[16], int lane, int g,
-                                             const double (*lt)[kNb], const double* rd,
+// Step l = 16 lg + LL of the row solve x L11^T = a: the wave holding column l finishes x[l]
+// (times 1 / L11[l][l], kept in registers) and hands it over through LDS; the wave holding
+// columns 16 g .. 16 g + 15 then applies x[j] -= x[l] L11[j][l] for its columns j > l.  Groups
+// in a loop, steps of a group unrolled, as in the factorisation.
+template <int LL>
+struct CholPanelGroupSteps {
+  __device__ __forceinline__ static void run(double (&x)[16], const double (&rdv)[16], int lane,
+                                             int g, int lg, const double (*lt)[kNb],
                                              double (*xs)[64]) {
-    if (g == (L >> 4)) {  // the owner wave finishes x[L] and hands it to the others
-      x[L & 15] *= rd[L];
-      xs[L & 1][lane] = x[L & 15];
+    const int l = 16 * lg + LL;
+    if (g == lg) {
+      x[LL] *= rdv[LL];
+      xs[LL & 1][lane] = x[LL];
     }
     chol_lds_barrier();
-    switch (g) {
-      case 0: chol_panel_step<0, L>(x, lane, lt, xs[L & 1]); break;
-      case 1: chol_panel_step<1, L>(x, lane, lt, xs[L & 1]); break;
-      case 2: chol_panel_step<2, L>(x, lane, lt, xs[L & 1]); break;
-      default: chol_panel_step<3, L>(x, lane, lt, xs[L & 1]); break;
+    if (g > lg) {
+      const double v = xs[LL & 1][lane];
+      const double* row = lt[l] + 16 * g;
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) x[jj] = fma(-v, row[jj], x[jj]);
+    } else if (g == lg && LL < 15) {
+      const double v = xs[LL & 1][lane];
+      const double* row = lt[l] + 16 * g;
+#pragma unroll
+      for (int jj = LL + 1; jj < 16; ++jj) x[jj] = fma(-v, row[jj], x[jj]);
     }
-    CholPanelSteps<L + 1>::run(x, lane, g, lt, rd, xs);
+    CholPanelGroupSteps<LL + 1>::run(x, rdv, lane, g, lg, lt, xs);
   }
 };
 template <>
-struct CholPanelSteps<kNb> {
-  __device__ __forceinline__ static void run(double (&)[16], int, int, const double (*)[kNb],
-                                             const double*, double (*)[64]) {}
+struct CholPanelGroupSteps<16> {
+  __device__ __forceinline__ static void run(double (&)[16], const double (&)[16], int, int, int,
+                                             const double (*)[kNb], double (*)[64]) {}
 };
 
 // Panel rows k + kNb .. m - 1, 64 per block (launched only when there are some): lane = row,
@@ -297,7 +291,11 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(double* __rest
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) x[jj] = live ? row[jj] : 0.0;
   __syncthreads();
-  CholPanelSteps<0>::run(x, lane, g, lt, rd, xs);
+  double rdv[16];  // 1 / L11[j][j] for this wave's columns
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) rdv[jj] = rd[16 * g + jj];
+#pragma unroll 1
+  for (int lg = 0; lg < 4; ++lg) CholPanelGroupSteps<0>::run(x, rdv, lane, g, lg, lt, xs);
   if (live) {
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) row[jj] = x[jj];
