@@ -28,6 +28,9 @@ PEAK_HBM_GBS = 8000.0
 # --pmc runs of this script: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy,
 # --copy-probe); the runs and their commands are in profiles/sites_kernel_256_config2_pmc.md.
 TRAFFIC_CONFIG2_FP32 = (2.0 * 1.6565e8 + 1.0 * 3.2545e8) * 1024
+# The same for segmented_sites_kernel on the reference demo geometry (4225 sites), from
+# profiles/segmented_sites_kernel_reference_geometry.md.
+TRAFFIC_REF_DEMO = (2.0 * 2.9498e7 + 1.0 * 6.2773e7) * 1024
 METRIC = "sites/s on 4225-site optical stage; surrogate vs propagator at ≤1% rel L2"
 
 
@@ -373,7 +376,8 @@ def bench_reference(args):
                           for k in ("plan", "propagation", "propagation_kernel",
                                     "aggregation")},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                     "frac": ach / PEAK_HBM_GBS,
+                     "traffic": TRAFFIC_REF_DEMO if P == 4225 else None,
                      "kernel": "segmented_sites_kernel",
                      "kernel_ms": kern * 1e3, "algorithmic_bytes_per_launch": per_site * P},
         "cpu_baseline": cpu,
