@@ -251,7 +251,8 @@ int64_t maomi_v4_apply_rows(int64_t P, int64_t Q);
  * Cholesky factorisation A = L L^T of the positive-definite block of the projected Green
  * matrix.  A is row-major with leading dimension lda >= m; only the lower triangle is read and
  * written (it receives L).  *info (a device int) becomes 0, or the 1-based column of the first
- * non-positive pivot.  workspace: maomi_cholesky_f64_workspace() bytes of device memory. */
+ * non-positive pivot.  workspace: maomi_cholesky_f64_workspace() bytes of device memory.
+ * m = 0 only sets *info (A and workspace may be NULL). */
 int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
                        maomi_stream_t stream);
 int64_t maomi_cholesky_f64_workspace(void);

@@ -474,7 +474,7 @@ MAOMI_API int64_t maomi_cholesky_f64_workspace(void) {
 
 MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
                                  maomi_stream_t stream) {
-  if (!A || !workspace || !info || m < 0 || lda < m || lda < 1) {
+  if (!info || m < 0 || lda < m || lda < 1 || (m > 0 && (!A || !workspace))) {
     maomi_set_error("maomi_cholesky_f64: bad arguments");
     return MAOMI_EINVAL;
   }

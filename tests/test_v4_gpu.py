@@ -46,3 +46,31 @@ def test_v4_apply_matches_direct_sum(gpu_lib, P, Q, C, rows):
               "v4 apply")
     got = out.cpu().numpy()
     assert (np.abs(got - ref) <= 1e-13 * bound + 1e-300).all()
+
+
+@pytest.mark.parametrize("name", ["collinear", "three", "two", "one", "lattice"])
+def test_v4_interpolate_cuda_matches_cpu_backend(gpu_lib, name):
+    """Device solve routes (Cholesky of the projected system, LU fallback for collinear or
+    P <= 3, single point) against the NumPy backend (numerics.py:122-138)."""
+    from maomi_amd.numerics import v4_interpolate
+    rng = np.random.default_rng(11)
+    if name == "collinear":  # rank-2 polynomial space: the LU fallback
+        p = np.stack([np.linspace(0, 10, 40), np.linspace(0, 5, 40)], 1)
+    elif name == "lattice":
+        t = np.arange(9) * 4.0
+        xx, yy = np.meshgrid(t, t, indexing="ij")
+        p = np.stack([xx.ravel(), yy.ravel()], 1)
+    else:
+        p = rng.random(({"three": 3, "two": 2, "one": 1}[name], 2)) * 10
+    v = rng.standard_normal((len(p), 2))
+    q = rng.random((17, 23, 2)) * 12 - 1
+    got = v4_interpolate(p, v, q, backend="cuda")
+    ref = v4_interpolate(p, v, q, backend="cpu")
+    assert got.shape == ref.shape == (17, 23, 2)
+    np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-8 * np.abs(ref).max())
+
+
+def test_cholesky_of_empty_matrix(gpu_lib):
+    import torch
+    from maomi_amd.numerics import cholesky_f64_
+    assert cholesky_f64_(torch.empty((0, 0), dtype=torch.float64, device="cuda")) == 0
