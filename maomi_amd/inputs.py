@@ -237,30 +237,57 @@ def prepare_fields(config, vessels=None, *, device="cpu", scatter=True, out_devi
 def _disk_runs(X, Y, radius):
     """The reference's full-grid disk x^2 + y^2 <= radius^2 on arange(-X/2, X/2) by
     arange(-Y/2, Y/2), cropped to its bounding box (optics.py:396-405); returns the first and
-    last column of each row, the cropped size and the cell count, or None when empty.
+    last column of each row, the cropped size and the cell count, or None when empty."""
+    return _disk_runs_many(X, Y, [radius])[0]
 
-    The same predicate is evaluated only where |x| and |y| are within radius + 1 of the centre:
-    beyond that x^2 alone exceeds radius^2 by far more than a rounding step."""
+
+def _disk_runs_many(X, Y, radii):
+    """_disk_runs for several radii in one vectorised pass.
+
+    Each row of a disk is one run of columns.  Its ends come from the closed form
+    y = +-sqrt(radius^2 - x^2), whose rounding moves them by at most one column, and are then
+    settled by the reference's own predicate on the columns within two of the estimate, so the
+    cells are exactly those of the full grid."""
     x = np.arange(-X / 2, X / 2)
-    y = np.arange(-Y / 2, Y / 2)
-    xs = np.flatnonzero(np.abs(x) <= radius + 1)
-    ys = np.flatnonzero(np.abs(y) <= radius + 1)
-    if not len(xs) or not len(ys):
-        return None
-    xx, yy = x[xs][:, None], y[ys][None, :]
-    disk = xx * xx + yy * yy <= radius * radius
-    rows, cols = np.flatnonzero(disk.any(1)), np.flatnonzero(disk.any(0))
-    if not len(rows):
-        return None
-    d = disk[rows[0]:rows[-1] + 1, cols[0]:cols[-1] + 1]
-    K0, K1 = d.shape
-    first = d.argmax(1)
-    last = K1 - 1 - d[:, ::-1].argmax(1)
-    count = d.sum(1)
-    if np.any(count != last - first + 1) or np.any(count == 0):
+    y0 = -Y / 2
+    radii = np.asarray(radii, np.float64)
+    sel = np.abs(x)[None, :] <= radii[:, None] + 1            # candidate rows per radius
+    slab, row = np.nonzero(sel)
+    out = [None] * len(radii)
+    if not len(row):
+        return out
+    xx = x[row] * x[row]
+    r2 = radii[slab] * radii[slab]
+    h = np.sqrt(np.maximum(r2 - xx, 0.0))
+    est_lo = np.clip(np.ceil(-h - y0), 0, Y - 1).astype(np.int64)
+    est_hi = np.clip(np.floor(h - y0), 0, Y - 1).astype(np.int64)
+    win = np.arange(-2, 3)
+    lo_c = est_lo[:, None] + win[None, :]        # ascending: first inside is the run start
+    hi_c = est_hi[:, None] - win[None, :]        # descending: first inside is the run end
+
+    def inside(j):
+        yj = y0 + j
+        return (j >= 0) & (j < Y) & (xx[:, None] + yj * yj <= r2[:, None])
+
+    ok_lo, ok_hi = inside(lo_c), inside(hi_c)
+    full = ok_lo.any(1)
+    if not np.array_equal(full, ok_hi.any(1)):
         raise AssertionError("collection disk rows must be single runs")
-    runs = np.stack([first, last], 1).astype(np.int32).ravel()
-    return runs, K0, K1, int(count.sum())
+    lo = lo_c[np.arange(len(row)), ok_lo.argmax(1)]
+    hi = hi_c[np.arange(len(row)), ok_hi.argmax(1)]
+    bounds = np.searchsorted(slab, np.arange(len(radii) + 1))
+    for s_, (b0, b1) in enumerate(zip(bounds[:-1], bounds[1:])):
+        f = full[b0:b1]
+        if not f.any():
+            continue
+        k = np.flatnonzero(f)
+        if np.any(np.diff(k) != 1):
+            raise AssertionError("collection disk rows must be consecutive")
+        l_, h_ = lo[b0:b1][k], hi[b0:b1][k]
+        c0 = int(l_.min())
+        runs = np.stack([l_ - c0, h_ - c0], 1).astype(np.int32).ravel()
+        out[s_] = (runs, len(k), int(h_.max()) - c0 + 1, int((h_ - l_ + 1).sum()))
+    return out
 
 
 def collection_mask(vessels, config, events=None):
@@ -287,12 +314,14 @@ def collection_mask(vessels, config, events=None):
     cone = np.tan(np.arcsin(config.objNA / config.n))
     used, runs_all, weights = [], [], []
     offset = 0
-    for start in range(0, Zv, pc):
+
+    def radius_at(start):  # the slab's collection radius in voxels
         depth = (start / pc + 0.5) * config.prop_um
-        radius = r * cone * (config.depth_um + config.volume_um[2] / 2 - depth)
-        if radius <= 0:
-            continue
-        disk = _disk_runs(X, Y, radius)
+        return r * cone * (config.depth_um + config.volume_um[2] / 2 - depth)
+
+    starts = [st for st in range(0, Zv, pc) if radius_at(st) > 0]
+    disks = _disk_runs_many(X, Y, [radius_at(st) for st in starts])
+    for start, disk in zip(starts, disks):
         if disk is None:
             continue
         runs, K0, K1, count = disk
