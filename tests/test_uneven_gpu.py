@@ -24,7 +24,8 @@ def _oracle_masks(cfg, vol, windows, sizes, slices, zw):
 
 
 @pytest.mark.gpu
-def test_mixed_sizes_one_launch_matches_oracle(gpu_lib):
+@pytest.mark.parametrize("threads", [512, 256])
+def test_mixed_sizes_one_launch_matches_oracle(gpu_lib, threads):
     from maomi_amd.uneven import UnevenStage
     from maomi_amd.workloads import PlainConfig, index_volume, zernike_weights
     cfg = PlainConfig(fov=64, n=256, slices=12, sites_per_axis=3, corr_vox=2.0,
@@ -33,9 +34,10 @@ def test_mixed_sizes_one_launch_matches_oracle(gpu_lib):
     centres = np.array([[0, 0], [32, 32], [64, 0], [16, 48], [48, 16], [64, 64]])
     sizes = np.array([256, 128, 64, 128, 256, 64])
     slices = np.array([12, 5, 9, 12, 3, 7])
-    print("[uneven] 6 mixed sites", flush=True)
+    print(f"[uneven] 6 mixed sites, {threads} threads", flush=True)
     st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
-    tuned = st.autotune(grids=(2, 6), reps=1)
+    tuned = st.autotune(grids=(2, 6), threads=(threads,), reps=1)
+    assert tuned["threads"] == threads
     print("autotuned:", tuned, flush=True)
     for _ in range(2):  # replays with eager work between them
         st.propagate()
@@ -79,3 +81,24 @@ def test_batched_base_beams_match_single_steps(gpu_lib):
         for b, bp in enumerate(bps):
             ref = base_beam(pup, bp, tables.edge2d[n], tw, "cuda")
             assert rel_l2(out[b].cpu().numpy(), ref.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("threads", [512, 256])
+def test_mixed_sizes_with_512_sites_match_oracle(gpu_lib, threads):
+    """The 512 field side (32-line tiles, two 16-point jobs per thread) inside the mixed launch,
+    with both block shapes, next to smaller sites."""
+    from maomi_amd.uneven import UnevenStage
+    from maomi_amd.workloads import PlainConfig, index_volume, zernike_weights
+    cfg = PlainConfig(fov=64, n=512, slices=6, sites_per_axis=3, corr_vox=2.0)
+    vol = index_volume(cfg)
+    centres = np.array([[0, 0], [32, 32], [64, 0], [16, 48]])
+    sizes = np.array([512, 128, 512, 256])
+    slices = np.array([6, 4, 3, 5])
+    print(f"[uneven] 512 sites, {threads} threads", flush=True)
+    st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
+    tuned = st.autotune(grids=(4,), threads=(threads,), orders=("lpt",), reps=1)
+    assert tuned["threads"] == threads
+    st.propagate()
+    ref = _oracle_masks(cfg, vol, st.windows, sizes, slices, zernike_weights(cfg))
+    for i in range(len(sizes)):
+        assert rel_l2(st.site_mask(i).cpu().numpy(), ref[i]) <= 1e-5, i
