@@ -16,15 +16,17 @@ from tests.conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _run(cfg, sites=None, precision="fp32"):
+def _run(cfg, sites=None, precision="fp32", block_threads=None):
     import torch
     from maomi_amd.plain import PlainStage
     from maomi_amd.workloads import index_volume, site_offsets, zernike_weights
     vol = index_volume(cfg)
     offs = site_offsets(cfg) if sites is None else sites
-    print(f"[plain] N={cfg.n} S={cfg.slices} fov={cfg.fov} sites={len(offs)} {precision}",
-          flush=True)
+    print(f"[plain] N={cfg.n} S={cfg.slices} fov={cfg.fov} sites={len(offs)} {precision} "
+          f"threads={block_threads or 'default'}", flush=True)
     st = PlainStage(cfg, vol, sites=offs, precision=precision).plan()
+    if block_threads:
+        st.block_threads = block_threads
     st.propagate()
     torch.cuda.synchronize()
     return st, vol, offs, zernike_weights(cfg)
@@ -43,12 +45,14 @@ def test_config1_fp32_matches_oracle(gpu_lib):
     np.testing.assert_allclose(float(st.mask.mean()), 1.0, rtol=1e-12)
 
 
+@pytest.mark.parametrize("threads", [512, 256])
 @pytest.mark.parametrize("n,slices,fov,nsites", [(64, 6, 64, 4), (256, 8, 64, 4), (512, 4, 32, 1)])
-def test_field_sizes_match_oracle(gpu_lib, n, slices, fov, nsites):
+def test_field_sizes_match_oracle(gpu_lib, n, slices, fov, nsites, threads):
     from maomi_amd.workloads import PlainConfig, site_offsets
     cfg = PlainConfig(fov=fov, n=n, slices=slices, sites_per_axis=2, corr_vox=2.0,
                       zernike_std_rad=0.05)
-    st, vol, offs, zw = _run(cfg, site_offsets(cfg)[:nsites])
+    st, vol, offs, zw = _run(cfg, site_offsets(cfg)[:nsites], block_threads=threads)
+    assert st.block_threads == threads
     ref = O.plain_stage(cfg, vol, offs, zw, with_masks=False)
     assert rel_l2(st.masks.cpu().numpy(), ref["masks"]) <= 1e-5
 
