@@ -83,6 +83,7 @@ def test_batched_base_beams_match_single_steps(gpu_lib):
             assert rel_l2(out[b].cpu().numpy(), ref.cpu().numpy()) < 1e-6
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("threads", [512, 256])
 def test_mixed_sizes_with_512_sites_match_oracle(gpu_lib, threads):
     """The 512 field side (32-line tiles, two 16-point jobs per thread) inside the mixed launch,
