@@ -38,12 +38,15 @@ def _envelope_tables(dims, l_scale, l_weights, device):
         raise ValueError("l_scale must have shape (K, 3)")
     w = (torch.ones(ls.shape[0], device=device) if l_weights is None
          else torch.as_tensor(l_weights, dtype=torch.float32, device=device))
-    sq = [torch.linspace(-np.pi / 2, np.pi / 2, d, device=device, dtype=torch.float32).pow(2)
-          for d in dims]
-    rows = [torch.cat([torch.exp(-sq[a] * ls[i, a] ** 2) for a in range(3)])
-            for i in range(ls.shape[0])]
+    sq = torch.cat([torch.linspace(-np.pi / 2, np.pi / 2, d, device=device,
+                                   dtype=torch.float32).pow(2) for d in dims])
+    axis = torch.repeat_interleave(torch.arange(3, device=device),
+                                   torch.tensor(dims, device=device))
+    # every scale and axis in one expression, element by element the reference's
+    # exp(-sq * l^2)
+    rows = torch.exp(-sq[None, :] * (ls ** 2)[:, axis])
     coef = [torch.stack([torch.prod(ls[i]), w[i] ** 2]) for i in range(ls.shape[0])]
-    return torch.stack(rows), torch.stack(coef)
+    return rows, torch.stack(coef)
 
 
 def _spectrum(re, im, l_scale, l_weights, table_device, events=None):
