@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--cpu-sites", type=int, default=12)
     ap.add_argument("--cpu-queries", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-overlap-v4", dest="overlap_v4", action="store_false",
+                    help="configs 1/2/4: one sites launch, the whole v4 solve after it")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "mixed"))
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run three device copies of GIB gibibytes (counter calibration)")
@@ -168,7 +170,7 @@ def main():
         torch.cuda.synchronize()
         tp = time.perf_counter()
         ev0.record()
-        st.propagate()
+        st.propagate_all(overlap=args.overlap_v4)
         ev1.record()
         torch.cuda.synchronize()
         tq = time.perf_counter()

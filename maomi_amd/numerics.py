@@ -303,6 +303,11 @@ class V4Factor:
         self.G, self.G0, self.V, self.T, self.L = G, G0, V, T, L
         self.A11, self.A12, self.info = A11, A12, info
 
+    def tensors(self):
+        """The tensors it owns (L, A11 and A12 are views of G)."""
+        return [t for t in (self.G, self.G0, self.V, self.T, self.info)
+                if isinstance(t, torch.Tensor)]
+
 
 def v4_factor(points, G, *, keep_copy=True, reflectors=None):
     """Point-only part of the v4 solve G w = v (FP64, on G's device), through its

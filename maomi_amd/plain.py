@@ -126,6 +126,42 @@ def v4_prepare(points, device):
     return p, G, factor
 
 
+class V4Ahead:
+    """The point-only part of the v4 solve (v4_prepare) queued on a side stream (default
+    priority), ordered after the work already queued on the current stream when start() is
+    called; take() orders the current stream after it."""
+
+    def __init__(self, device):
+        self.device, self.side, self.ready = device, None, None
+
+    def start(self, points):
+        main = torch.cuda.current_stream(self.device)
+        after = torch.cuda.Event()
+        after.record(main)
+        if self.side is None:
+            self.side = torch.cuda.Stream(self.device)
+        self.side.wait_event(after)
+        with torch.cuda.stream(self.side):
+            prep = v4_prepare(points, self.device)
+            done = torch.cuda.Event()
+            done.record(self.side)
+        self.ready = (prep, done)
+
+    def take(self):
+        """The prepared (points, G, factor), with the current stream ordered after it and its
+        tensors marked as used there; None if nothing was started."""
+        if self.ready is None:
+            return None
+        prep, done = self.ready
+        self.ready = None
+        main = torch.cuda.current_stream(self.device)
+        main.wait_event(done)
+        p, G, factor = prep
+        for t in [p, G] + (factor.tensors() if factor is not None else []):
+            t.record_stream(main)
+        return prep
+
+
 def v4_masks(points, values, shape, device, prepared=None):
     """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153).
     prepared: a v4_prepare result for these points."""
@@ -234,11 +270,15 @@ class PlainStage:
         self.base = base_beam(self.pupil, self.bp, self.tables.edge2d[cfg.n], tw, d)
         return self
 
-    def propagate(self, subset=None, psf=None):
+    def propagate(self, subset=None, psf=None, tail_hook=None):
         """All sites (or the indices `subset`) in one persistent launch.
 
         psf = (out, z0, nz, c0, w) also writes |u|^4 of planes z0..z0+nz-1 cropped to
         [c0, c0+w)^2 into out[nz][P][w][w] (surrogate training targets and audits).
+
+        tail_hook: when the sites do not fill whole rounds of the grid, the launch is issued as
+        the whole rounds and then the last partial round, and tail_hook() is called in between
+        (work it queues on another stream runs on the CUs that the last round leaves idle).
         """
         cfg, d = self.cfg, self.device
         lib = nat.lib()
@@ -292,13 +332,35 @@ class PlainStage:
             desc.psf_z0, desc.psf_nz, desc.psf_c0, desc.psf_w = z0, nz, c0, w
         self._desc = desc
         self._keep = (site_x, site_y, order)
-        nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "persistent sites")
+        stream = nat.stream_handle(d)
+        P1 = P - P % grid
+        if tail_hook is not None and psf is None and P > grid and P1 < P:
+            desc.P = P1
+            nat.check(lib.maomi_persistent_sites(desc, stream), "persistent sites")
+            tail_hook()
+            desc.P, desc.grid = P - P1, P - P1
+            desc.order = order.data_ptr() + 4 * P1
+            nat.check(lib.maomi_persistent_sites(desc, stream), "persistent sites, last round")
+        else:
+            nat.check(lib.maomi_persistent_sites(desc, stream), "persistent sites")
         return self.masks, self.energies
+
+    def propagate_all(self, overlap=True):
+        """propagate() for every site; with overlap, the point-only part of the v4 solve
+        (Green matrix, projection, Cholesky) is queued on a side stream after the whole rounds
+        of sites, beside the last partial round, and aggregate() picks it up."""
+        if not overlap:
+            return self.propagate()
+        if getattr(self, "_v4", None) is None:
+            self._v4 = V4Ahead(self.device)
+        pts = self.offsets.astype(np.float64)
+        return self.propagate(tail_hook=lambda: self._v4.start(pts))
 
     def aggregate(self):
         cfg = self.cfg
         pts = self.offsets.astype(np.float64)
-        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device)
+        prep = self._v4.take() if getattr(self, "_v4", None) is not None else None
+        masks = v4_masks(pts, self.energies, (cfg.fov, cfg.fov), self.device, prepared=prep)
         m = masks[..., 0]
         mean = m.mean()
         self.mask_raw, self.mask_mean = m, mean
@@ -312,7 +374,7 @@ class PlainStage:
         torch.cuda.synchronize(self.device)
         self.timing["plan"] = time.perf_counter() - t
         t = time.perf_counter()
-        self.propagate()
+        self.propagate_all()
         torch.cuda.synchronize(self.device)
         self.timing["propagation"] = time.perf_counter() - t
         t = time.perf_counter()

@@ -124,3 +124,31 @@ def test_complex_screens_equal_phase_screens(gpu_lib, n, slices, fov):
         out.append((st.masks.cpu().numpy(), st.energies.cpu().numpy()))
     assert rel_l2(out[1][0], out[0][0]) < 1e-7
     assert rel_l2(out[1][1], out[0][1]) < 1e-7
+
+
+def test_last_round_split_with_v4_overlap_equals_one_launch(gpu_lib):
+    """Sites that do not fill whole rounds of the grid: the whole rounds, then the last round
+    with the point-only v4 work on a side stream beside it.  Masks, energies and the v4 mask
+    equal the single launch with the whole v4 solve after it, bit for bit, over repeated runs
+    with eager work between them."""
+    import torch
+    from maomi_amd.plain import V4Ahead
+    from maomi_amd.workloads import CONFIG1
+    cfg = replace(CONFIG1, sites_per_axis=11)  # 121 sites
+    st, vol, offs, zw = _run(cfg)              # grid min(256, 121): one launch
+    st.aggregate()
+    ref_masks, ref_e, ref = st.masks.clone(), st.energies.clone(), st.mask_raw.clone()
+    st.grid = 32                               # 121 = 3 x 32 + 25
+    calls = []
+    st._v4 = V4Ahead(st.device)
+    pts = st.offsets.astype(np.float64)
+    st.propagate(tail_hook=lambda: (calls.append(1), st._v4.start(pts)))
+    st.aggregate()
+    assert calls == [1]
+    assert torch.equal(st.masks, ref_masks) and torch.equal(st.energies, ref_e)
+    assert torch.equal(st.mask_raw, ref)
+    for _ in range(3):
+        st.run()
+        assert torch.equal(st.masks, ref_masks) and torch.equal(st.energies, ref_e)
+        assert torch.equal(st.mask_raw, ref)
+        st.energies.sum().item()
