@@ -12,6 +12,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .numerics import V4Ahead, v4_prepare
 from .workloads import PlainConfig, beam_parameters, site_offsets, zernike_weights
 
 _CLASSES = {64: 0, 128: 1, 256: 2, 512: 3}
@@ -103,63 +104,6 @@ def base_beams(u, bp, dz_focus, edge2d, tw, out, device):
                                       nat.ptr(edge2d), nat.ptr(edge2d), nat.ptr(tw),
                                       nat.ptr(dwork), st), "fresnel steps")
     return out
-
-
-def _upload(a, device):
-    """Host array to the device through pinned memory, without blocking the host."""
-    return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(device, non_blocking=True)
-
-
-def v4_prepare(points, device):
-    """Green matrix and the point-only part of the v4 solve (numerics.v4_factor), queued on
-    the current stream with no host synchronisation.  Returns (points, G, factor)."""
-    from .numerics import _reflectors, v4_factor
-    pts = np.ascontiguousarray(points, np.float64)
-    P = len(pts)
-    p = _upload(pts, device)
-    ref = _reflectors(pts) if P > 3 else None
-    ref = False if ref is None else tuple(_upload(a, device) for a in ref)
-    G = torch.empty((P, P), dtype=torch.float64, device=device)
-    nat.check(nat.lib().maomi_v4_green_matrix_f64(nat.ptr(p), P, nat.ptr(G),
-                                                  nat.stream_handle(device)), "green matrix")
-    factor = v4_factor(p, G, reflectors=ref) if P > 3 else None
-    return p, G, factor
-
-
-class V4Ahead:
-    """The point-only part of the v4 solve (v4_prepare) queued on a side stream (default
-    priority), ordered after the work already queued on the current stream when start() is
-    called; take() orders the current stream after it."""
-
-    def __init__(self, device):
-        self.device, self.side, self.ready = device, None, None
-
-    def start(self, points):
-        main = torch.cuda.current_stream(self.device)
-        after = torch.cuda.Event()
-        after.record(main)
-        if self.side is None:
-            self.side = torch.cuda.Stream(self.device)
-        self.side.wait_event(after)
-        with torch.cuda.stream(self.side):
-            prep = v4_prepare(points, self.device)
-            done = torch.cuda.Event()
-            done.record(self.side)
-        self.ready = (prep, done)
-
-    def take(self):
-        """The prepared (points, G, factor), with the current stream ordered after it and its
-        tensors marked as used there; None if nothing was started."""
-        if self.ready is None:
-            return None
-        prep, done = self.ready
-        self.ready = None
-        main = torch.cuda.current_stream(self.device)
-        main.wait_event(done)
-        p, G, factor = prep
-        for t in [p, G] + (factor.tensors() if factor is not None else []):
-            t.record_stream(main)
-        return prep
 
 
 def v4_masks(points, values, shape, device, prepared=None):

@@ -132,7 +132,7 @@ def test_last_round_split_with_v4_overlap_equals_one_launch(gpu_lib):
     equal the single launch with the whole v4 solve after it, bit for bit, over repeated runs
     with eager work between them."""
     import torch
-    from maomi_amd.plain import V4Ahead
+    from maomi_amd.numerics import V4Ahead
     from maomi_amd.workloads import CONFIG1
     cfg = replace(CONFIG1, sites_per_axis=11)  # 121 sites
     st, vol, offs, zw = _run(cfg)              # grid min(256, 121): one launch
