@@ -183,6 +183,37 @@ int maomi_pupil_gaussian(void* out, int64_t N, double D1, double rad, double obj
                          double fl, const float* zern_abb, int64_t nz, float* zmap,
                          double* dwork, maomi_stream_t stream);
 
+/* Alternative input pupils of maomi/pupils.py: make_pupil (pupils.py:7-34; kinds gaussian,
+ * cylindrical, bessel) and one beam of make_vtwins_pupils (pupils.py:54-109), with the Bessel
+ * ring profiles of _bessel_amplitude (pupils.py:112-122).  The grid is pupils.py:37-51's:
+ * pixel (i, j) of an n1 x n0 row-major complex64 field has x = xs[j], y = ys[i].  Every scalar
+ * below is the float32 value the reference's expression yields, computed by the caller; the
+ * kernel does the per-pixel float32 arithmetic in the reference's operation order. */
+typedef struct maomi_pupil_params {
+  int32_t kind;      /* 0 gaussian (make_pupil gaussian/cylindrical), 1 bessel (make_pupil),
+                        2 pair beam, gaussian, 3 pair beam, bessel */
+  int32_t profile;   /* ring profile of kinds 1 and 3: 0 gaussian, 1 uniform, 2 posax, 3 negax */
+  int32_t clip;      /* kind 3: zero outside the objective aperture */
+  int32_t cylinder;  /* kind 0: multiply by the cylindrical support (scale_x, scale_y) */
+  float rr;          /* kinds 0 and 2: Gaussian radius squared */
+  float obj2;        /* objective aperture radius squared */
+  float focus;       /* imaginary part of the focus exponent (-i k / (2 fl)) */
+  float radius;      /* kinds 1 and 3: ring radius */
+  float width;       /* ring width as each profile uses it: gaussian w / (2 sqrt(ln 2)),
+                        uniform w / 2, posax and negax w / 1.133 */
+  float offset;      /* posax and negax: their width / sqrt(2) */
+  float shift;       /* kinds 2 and 3: sign * displacement along x */
+  float tilt;        /* kinds 2 and 3: imaginary part of the tilt exponent (i k sign) */
+  double scale_x, scale_y;  /* kind 0 with cylinder: the cylindrical scales */
+} maomi_pupil_params;
+
+/* The field out[n1][n0] (complex64).  cols[n0] is the pair beams' mean-removed tilt phase per
+ * column (pupils.py:99-105; ignored by kinds 0 and 1, may be NULL there).  Replaces the NumPy
+ * arithmetic of make_pupil / make_vtwins_pupils (one call per beam). */
+int maomi_pupil_field(void* out, int64_t n0, int64_t n1, const float* xs, const float* ys,
+                      const float* cols, const maomi_pupil_params* params,
+                      maomi_stream_t stream);
+
 /* One Fresnel step of numerics.fresnel (maomi/numerics.py:156-210) for B fields of
  * N x N (N even), in place: entry chirp and screen0, FFT, Q (scaled when dx1 != dx0),
  * inverse FFT, screen1, exit chirp.  This is the base-beam step of

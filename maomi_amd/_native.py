@@ -30,6 +30,15 @@ class SitesDesc(ctypes.Structure):
     ]
 
 
+class PupilParams(ctypes.Structure):
+    """Mirror of maomi_pupil_params."""
+    _fields_ = [("kind", ctypes.c_int32), ("profile", ctypes.c_int32), ("clip", ctypes.c_int32),
+                ("cylinder", ctypes.c_int32), ("rr", ctypes.c_float), ("obj2", ctypes.c_float),
+                ("focus", ctypes.c_float), ("radius", ctypes.c_float), ("width", ctypes.c_float),
+                ("offset", ctypes.c_float), ("shift", ctypes.c_float), ("tilt", ctypes.c_float),
+                ("scale_x", c_double), ("scale_y", c_double)]
+
+
 class Segment(ctypes.Structure):
     """Mirror of maomi_segment."""
     _fields_ = [("phase", c_void_p), ("H", c_int64), ("W", c_int64), ("Z", c_int64),
@@ -67,6 +76,8 @@ _SIGS = {
     "maomi_twiddles": (c_int, [c_void_p, c_int64, c_void_p]),
     "maomi_pupil_gaussian": (c_int, [c_void_p, c_int64, c_double, c_double, c_double, c_double,
                                      c_double, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "maomi_pupil_field": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                  ctypes.POINTER(PupilParams), c_void_p]),
     "maomi_fresnel_step": (c_int, [c_void_p, c_int64, c_int64, c_double, c_double, c_double,
                                    c_double, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "maomi_normalize_l2": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),

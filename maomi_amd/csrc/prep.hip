@@ -301,6 +301,65 @@ MAOMI_API int maomi_twiddles(void* tw, int64_t N, maomi_stream_t stream) {
   return MAOMI_OK;
 }
 
+// ---------------------------------------------------------------- pupil families
+// Ring amplitude of the Bessel pupils at radial offset dr (pupils.py:112-122).  p.width is
+// already the profile's own width; this file is built without contraction, so each float
+// operation rounds as the reference's float32 array arithmetic does.
+__device__ inline float ring_amplitude(float dr, const maomi_pupil_params& p) {
+  switch (p.profile) {
+    case 0: {
+      const float t = dr / p.width;
+      return expf(-(t * t));
+    }
+    case 1:
+      return (dr < p.width && dr > -p.width) ? 1.0f : 0.0f;
+    default: {
+      const float z = (p.profile == 2 ? dr : -dr) + p.offset;
+      const float t = z / p.width;
+      return z > 0.0f ? z * expf(-(t * t)) : 0.0f;
+    }
+  }
+}
+
+__global__ void pupil_field_kernel(float2* __restrict__ out, int n0, int64_t tot,
+                                   const float* __restrict__ xs, const float* __restrict__ ys,
+                                   const float* __restrict__ cols, maomi_pupil_params p) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n0), j = (int)(e - (int64_t)i * n0);
+    const float x = xs[j], y = ys[i];
+    const float y2 = y * y;
+    const float rho2 = x * x + y2;
+    const bool inside = rho2 < p.obj2;
+    const bool pair = p.kind >= 2;
+    const float u = pair ? x - p.shift : x;  // the pair beams' amplitude is shifted along x
+    const float r2 = pair ? u * u + y2 : rho2;
+    float amp;
+    if (p.kind == 0 || p.kind == 2) {
+      amp = inside ? expf(-r2 / p.rr) : 0.0f;
+      if (p.cylinder) {  // generateCylindricalBA's support, in FP64 as the reference has it
+        const double sx = (double)x / p.scale_x, sy = (double)y / p.scale_y;
+        const double sc = sx * sx + sy * sy;
+        if (!(sc < (double)p.obj2 && exp(-sc / (double)p.rr) > 0.0)) amp = 0.0f;
+      }
+    } else {
+      amp = ring_amplitude(sqrtf(r2) - p.radius, p);
+      if (p.kind == 3 && p.clip && !inside) amp = 0.0f;
+    }
+    float s, c;
+    sincosf(p.focus * rho2, &s, &c);
+    float re = amp * c, im = amp * s;
+    if (pair) {  // times exp(i k sign phase(x))
+      float ts, tc;
+      sincosf(p.tilt * cols[j], &ts, &tc);
+      const float r = re * tc - im * ts;
+      im = re * ts + im * tc;
+      re = r;
+    }
+    out[e] = make_float2(re, im);
+  }
+}
+
 MAOMI_API int maomi_pupil_gaussian(void* out, int64_t N, double D1, double rad, double objrad,
                                    double k, double fl, const float* zern_abb, int64_t nz,
                                    float* zmap, double* dwork, maomi_stream_t stream) {
@@ -322,6 +381,24 @@ MAOMI_API int maomi_pupil_gaussian(void* out, int64_t N, double D1, double rad, 
                        zmap, n, dwork, (float)k);
     MAOMI_CHECK_LAUNCH();
   }
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_pupil_field(void* out, int64_t n0, int64_t n1, const float* xs,
+                                const float* ys, const float* cols,
+                                const maomi_pupil_params* p, maomi_stream_t stream) {
+  if (!out || !xs || !ys || !p || n0 < 1 || n1 < 1 || n0 > 65536 || n1 > 65536 ||
+      p->kind < 0 || p->kind > 3 || p->profile < 0 || p->profile > 3 ||
+      (p->kind >= 2 && !cols) || (p->cylinder && (p->kind != 0 || !(p->scale_x > 0.0) ||
+                                                  !(p->scale_y > 0.0)))) {
+    maomi_set_error("maomi_pupil_field: bad arguments (n0 %lld, n1 %lld)", (long long)n0,
+                    (long long)n1);
+    return MAOMI_EINVAL;
+  }
+  const int64_t tot = n0 * n1;
+  hipLaunchKernelGGL(pupil_field_kernel, dim3(nblocks(tot)), dim3(256), 0, (hipStream_t)stream,
+                     (float2*)out, (int)n0, tot, xs, ys, cols, *p);
+  MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }
 

@@ -2,7 +2,7 @@
 
 This package restates, in NumPy/SciPy, the reference CPU algorithm of MaoMi's
 optical stage (maomi/optics.py, maomi/numerics.py, maomi/_propagation.py,
-maomi/_optical_math.py, maomi/excitation.py, maomi/fixed.py).  It is the
+maomi/_optical_math.py, maomi/excitation.py, maomi/fixed.py, maomi/pupils.py).  It is the
 checker that the HIP path is compared against, and the algorithm the same-run
 CPU baseline times.  Only tests/, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of bench.py may import it.  The product package

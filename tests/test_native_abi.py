@@ -48,3 +48,13 @@ def test_sites_descriptor_layout_matches_header():
     # screen pointer is the last member
     assert ctypes.sizeof(_native.SitesDesc) == 8 * (19 + 12 + 15)
     assert _native.SitesDesc.screen.offset == 8 * (18 + 12 + 15)
+
+
+def test_pupil_params_layout_and_rejection():
+    # 4 int32 + 8 float, then two doubles at offset 48
+    assert ctypes.sizeof(_native.PupilParams) == 64
+    assert _native.PupilParams.scale_x.offset == 48
+    p = _native.PupilParams()
+    p.kind = 2  # a pair beam needs the column phase
+    rc = _native.lib().maomi_pupil_field(None, 8, 8, None, None, None, p, None)
+    assert rc == -1 and b"maomi_pupil_field: bad arguments" in _native.lib().maomi_last_error()
