@@ -78,6 +78,33 @@ def to_host(tensors):
     return [h.numpy() for h in hosts]
 
 
+class DeferredHost:
+    """A device tensor's copy into pinned host memory, queued on a side stream after the work
+    already on the current stream; result() waits for it and returns the NumPy array."""
+
+    _streams = {}
+
+    def __init__(self, t):
+        dev = t.device
+        side = DeferredHost._streams.get(dev)
+        if side is None:
+            side = DeferredHost._streams[dev] = torch.cuda.Stream(dev)
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(dev))
+        side.wait_event(ready)
+        self.shape, self.dtype = tuple(t.shape), t.dtype
+        self._host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        with torch.cuda.stream(side):
+            self._host.copy_(t, non_blocking=True)
+            self._done = torch.cuda.Event()
+            self._done.record(side)
+        t.record_stream(side)
+
+    def result(self):
+        self._done.synchronize()
+        return self._host.numpy()
+
+
 class OpticalPlan:
     def __init__(self, config, raw_fields, *, uin=None, device="cuda", native=True):
         if not str(device).startswith("cuda"):
@@ -283,12 +310,14 @@ class OpticalPlan:
 
     # ------------------------------------------------------------------ API
     @torch.inference_mode()
-    def evaluate(self, points_um, *, batch_size=16, tails=False, tail_convmask="sites"):
+    def evaluate(self, points_um, *, batch_size=16, tails=False, tail_convmask="sites",
+                 defer_psfs=False):
         """PSFs (and tails) of every point in one persistent launch (batched.py:170-233).
 
         tail_convmask="mean" returns each tail's convmask averaged over the points (FP64 on
         the device) as "convmask_mean" instead of the per-point stack, which is all `assemble`
-        uses; "sites" keeps the reference's per-point "convmask"."""
+        uses; "sites" keeps the reference's per-point "convmask".  defer_psfs: "psfs" is a
+        DeferredHost whose copy runs beside the caller's next GPU work (run_fixed resolves it)."""
         if tail_convmask not in ("sites", "mean"):
             raise ValueError("tail_convmask must be 'sites' or 'mean'")
         pts, off = self._offsets(points_um)
@@ -301,7 +330,8 @@ class OpticalPlan:
         if P:
             self._launch(ox, oy, order, P, tails, outs)
         self.paths_evaluated += P
-        names, dev = ["psfs"], [outs["psf"]]
+        defer = defer_psfs and P
+        names, dev = ([], []) if defer else (["psfs"], [outs["psf"]])
         if P:
             energy, sums = _psf_stats_device(outs["psf"])
             names += ["energy", "sums"]
@@ -317,6 +347,8 @@ class OpticalPlan:
                 names += [pre + "w", pre + "prof"]
                 dev += [outs[pre + "w"], outs[pre + "prof"]]
         host = dict(zip(names, to_host(dev)))
+        if defer:  # after the statistics on the current stream, so they read the same buffer
+            host["psfs"] = DeferredHost(outs["psf"])
         out = {"psfs": host["psfs"], "points_um": pts, "tails_computed": tails}
         if P:
             s = host["sums"]

@@ -257,13 +257,17 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
         torch.cuda.synchronize()
         timings["plan"] = time.perf_counter() - t
         t = time.perf_counter()
-        outputs = plan.evaluate(points, batch_size=batch_size, tails=tails, tail_convmask="mean")
+        # the PSF stack's copy to the host runs beside the v4 solve; resolved after assemble
+        outputs = plan.evaluate(points, batch_size=batch_size, tails=tails, tail_convmask="mean",
+                                defer_psfs=True)
         timings["propagation"] = time.perf_counter() - t
         if plan.last_launch_seconds() is not None:
             timings["propagation_kernel"] = plan.last_launch_seconds()
         t = time.perf_counter()
         result = assemble(config, outputs, expected[:2], mask_mode=mask_mode,
                           z_positions=plan.z_positions, cut=plan.cut, device=device)
+        if hasattr(result["psf_3d"], "result"):
+            result["psf_3d"] = result["psf_3d"].result()
         result["colmask"] = col
         timings["aggregation"] = time.perf_counter() - t
         t = time.perf_counter()

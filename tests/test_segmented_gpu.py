@@ -167,3 +167,24 @@ def test_line_splits_match_oracle(gpu_lib, kw, sides):
     for t in ("psfT", "psfB"):
         for key in ("convmask", "weight", "psfZ"):
             assert rel_l2(res[t][key], ref[t][key]) < 2e-5, (t, key)
+
+
+def test_deferred_psf_copy_equals_synchronous(gpu_lib):
+    """evaluate(defer_psfs=True): the PSF stack's host copy queued on a side stream is the
+    synchronous copy, bit for bit; run_fixed returns it as a plain array."""
+    from maomi_amd.batched import OpticalPlan
+    from maomi_amd.fixed import run_fixed
+    meta = json.loads((GOLD / "propagate_cases.json").read_text())["scene2p"]
+    cfg, ocfg = _cfgs(meta["config"])
+    raw = golden_raw_fields(O.field_shapes(ocfg), seed=meta["seed"])
+    plan = OpticalPlan(cfg, raw, device="cuda")
+    pts = O.locations(ocfg.volume_um, 5)
+    print("deferred PSF copy, 25 sites", flush=True)
+    ref = plan.evaluate(pts, tails=True, tail_convmask="mean")
+    got = plan.evaluate(pts, tails=True, tail_convmask="mean", defer_psfs=True)
+    assert got["psfs"].shape == ref["psfs"].shape
+    np.testing.assert_array_equal(got["psfs"].result(), ref["psfs"])
+    np.testing.assert_array_equal(got["psf_stats"]["energies"], ref["psf_stats"]["energies"])
+    res = run_fixed(cfg, None, raw_fields=raw, count_per_axis=5, render_image=False)
+    assert isinstance(res["psf_3d"], np.ndarray) and res["psf_3d"].shape == ref["psfs"].shape
+    assert np.isfinite(res["psf_3d"]).all() and np.isfinite(res["mask"]).all()
