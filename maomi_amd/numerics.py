@@ -237,12 +237,24 @@ def v4_interpolate(points, values, queries, chunk=16384, *, backend="auto", devi
     return out.cpu().numpy().reshape(shape)
 
 
+def lu_library(on_default_stream):
+    """The linear-algebra library for dense_solve_f64's LU.  MAGMA's single-matrix getrf/getrs
+    (26 ms against 43 ms for hipSOLVER at P = 4225 on MI355X) run on MAGMA's own queues, which
+    are ordered with the device's default stream only; on any other current stream the solve
+    could read G before the kernels queued there have written it.  There hipSOLVER, which runs
+    on torch's current stream, is used instead."""
+    return "magma" if on_default_stream else "cusolver"
+
+
 def dense_solve_f64(G, v):
-    """FP64 LU solve of the v4 system on the device (MAGMA's getrf/getrs through torch,
-    26 ms against 43 ms for the default path at P = 4225 on MI355X)."""
+    """FP64 LU solve of the v4 system on the device (the library chosen by lu_library)."""
+    dev = G.device
+    if not G.is_cuda:
+        return torch.linalg.solve(G, v).contiguous()
+    lib = lu_library(torch.cuda.current_stream(dev) == torch.cuda.default_stream(dev))
     prev = torch.backends.cuda.preferred_linalg_library()
     try:
-        torch.backends.cuda.preferred_linalg_library("magma")
+        torch.backends.cuda.preferred_linalg_library(lib)
         return torch.linalg.solve(G, v).contiguous()
     finally:
         torch.backends.cuda.preferred_linalg_library(prev)

@@ -89,7 +89,7 @@ def index_volume(cfg, dtype=np.float32):
         f = np.fft.fftfreq(size) if d < 2 else np.fft.rfftfreq(size)
         axes.append(np.exp(-2 * (np.pi * cfg.corr_vox * f) ** 2))
     spec *= axes[0][:, None, None] * axes[1][None, :, None] * axes[2][None, None, :]
-    field_ = np.fft.irfftn(spec, s=shape)
+    field_ = np.fft.irfftn(spec, s=shape, axes=(0, 1, 2))
     field_ *= cfg.dn_std / field_.std()
     return (cfg.n0 + field_).astype(dtype)
 

@@ -11,8 +11,12 @@ Pinning (see tests/test_oracle.py):
     reference itself (tests/golden/make_golden.py) plus closed forms
     (Parseval, Q semigroup, Gaussian-beam waist, tilt shift, node
     reproduction of v4, Keys kernel reproduction of constants/ramps).
-  * plain_stage_masks (the generic split-step workload): built from fresnel
-    (pinned above); checked against brute-force DFT-matrix propagation.
+  * plain_stage_masks, plain_phase, plain_beam, plain_stage (the generic
+    split-step workload; tests/test_oracle_plain.py): the energy budget of a
+    unitary propagation, free propagation in closed form with zero screens,
+    covariance under a joint shift of volume and offsets, equality with the
+    golden-pinned segment A of propagate at cut = 0, the reference's own pupil
+    vectors and the beam's focal depth, and a direct voxel sum for the screens.
 """
 
 from dataclasses import dataclass

@@ -3,7 +3,8 @@
 Forward and backward of a k x k convolution (stride s, padding p) with bias
 and leaky ReLU, nearest 2x upsampling, and the per-sample relative squared
 error loss, on channel-major (C, B, H, W) arrays.  Used to check the GPU
-kernels' gradients.  Pinned by finite differences in tests/test_oracle_surrogate.py.
+kernels' gradients.  Pinned in tests/test_oracle_surrogate.py: the forward pass by a direct
+sum and by scipy's 2-D correlation, the backward passes by finite differences.
 """
 
 import numpy as np

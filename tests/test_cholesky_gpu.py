@@ -56,20 +56,37 @@ def _green(p):
     return g
 
 
-@pytest.mark.parametrize("name", ["lattice", "random"])
-def test_device_v4_coefficients_equal_lu_solution(gpu_lib, name):
-    import torch
-    from maomi_amd.numerics import v4_coefficients
-    rng = np.random.default_rng(3)
+def _points(name, rng):
     if name == "lattice":
         t = np.round(np.linspace(0, 256, 33))
         xx, yy = np.meshgrid(t, t, indexing="ij")
-        p = np.stack([xx.ravel(), yy.ravel()], 1)
-    else:
-        p = rng.random((700, 2)) * 100
+        return np.stack([xx.ravel(), yy.ravel()], 1)
+    if name == "config2":  # the 65 x 65 site lattice of the 4225-site plain workload
+        from maomi_amd.workloads import CONFIG2, site_offsets
+        return site_offsets(CONFIG2).astype(np.float64)
+    if name == "reference_demo":  # run_fixed's 65 x 65 lattice on the demo scene, in voxels
+        from maomi_amd.fixed import fixed_locations
+        from maomi_amd.optics import OpticalConfig
+        cfg = OpticalConfig(volume_um=(16.0, 16.0, 10.0), depth_um=20.0, vres=2.0, tail_um=10.0)
+        return fixed_locations(cfg, 65) * cfg.vres
+    return rng.random((700, 2)) * 100
+
+
+@pytest.mark.parametrize("name", ["lattice", "random", "config2", "reference_demo"])
+def test_device_v4_coefficients_equal_lu_solution(gpu_lib, name):
+    """The positive-definite route (projection + maomi_cholesky_f64 + Schur complement) against
+    LAPACK's LU, at the timed workloads' full size (4225 points) too.  The factorisation's
+    pivot report must be 0: the Cholesky route, not the LU fallback, produced the result."""
+    import torch
+    from maomi_amd.numerics import v4_coefficients, v4_factor
+    rng = np.random.default_rng(3)
+    p = _points(name, rng)
     G = _green(p)
     v = rng.random((len(p), 3))
     ref = np.linalg.solve(G, v)
-    w = v4_coefficients(torch.as_tensor(p, device="cuda"), torch.as_tensor(G, device="cuda"),
-                        torch.as_tensor(v, device="cuda")).cpu().numpy()
+    print(f"[v4 solve] {name}: P={len(p)}", flush=True)
+    tp, tG = torch.as_tensor(p, device="cuda"), torch.as_tensor(G, device="cuda")
+    f = v4_factor(tp, tG)
+    assert f is not None and f.V is not None and int(f.info) == 0
+    w = v4_coefficients(tp, None, torch.as_tensor(v, device="cuda"), factor=f).cpu().numpy()
     assert np.linalg.norm(w - ref) / np.linalg.norm(ref) < 1e-9

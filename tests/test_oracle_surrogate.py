@@ -76,3 +76,34 @@ def test_adam_first_step_closed_form():
     np.testing.assert_allclose(w1, w - 0.01 * (g / (np.abs(g) + 1e-8) + 0.1 * w), rtol=1e-12)
     np.testing.assert_allclose(m, 0.1 * g)
     np.testing.assert_allclose(v, 0.001 * g * g)
+
+
+def test_conv_forward_equals_direct_sum_and_library_correlation():
+    """y[o, b, i, j] = leaky(b[o] + sum_{c, u, v} w[o, (c, u, v)] xpad[c, b, s i + u, s j + v]):
+    a direct loop, and for stride 1 scipy's 2-D correlation per channel pair.  A flipped
+    kernel, a transposed tap order or a wrong padding offset fails one of them."""
+    from scipy.signal import correlate2d
+    rng = np.random.default_rng(5)
+    for k, s, p, slope in ((3, 2, 1, 0.1), (3, 1, 1, 0.2), (2, 1, 1, 1.0), (4, 2, 0, 0.05)):
+        C, B, H, W, co = 3, 2, 7, 6, 4
+        x = rng.standard_normal((C, B, H, W))
+        w = rng.standard_normal((co, C * k * k))
+        b = rng.standard_normal(co)
+        y, _ = S.conv_fwd(x, w, b, k, s, p, slope)
+        Ho, Wo = (H + 2 * p - k) // s + 1, (W + 2 * p - k) // s + 1
+        assert y.shape == (co, B, Ho, Wo)
+        xp = np.zeros((C, B, H + 2 * p, W + 2 * p))
+        xp[:, :, p:p + H, p:p + W] = x
+        wk = w.reshape(co, C, k, k)
+        want = np.empty_like(y)
+        for o, n, i, j in np.ndindex(co, B, Ho, Wo):
+            z = b[o]
+            for c, u, v in np.ndindex(C, k, k):
+                z += wk[o, c, u, v] * xp[c, n, s * i + u, s * j + v]
+            want[o, n, i, j] = z if z > 0 else slope * z
+        np.testing.assert_allclose(y, want, rtol=1e-12, atol=1e-12)
+        if s == 1:
+            for o, n in np.ndindex(co, B):
+                z = b[o] + sum(correlate2d(xp[c, n], wk[o, c], mode="valid") for c in range(C))
+                np.testing.assert_allclose(y[o, n], np.where(z > 0, z, slope * z), rtol=1e-12,
+                                           atol=1e-12)

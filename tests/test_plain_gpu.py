@@ -152,3 +152,47 @@ def test_last_round_split_with_v4_overlap_equals_one_launch(gpu_lib):
         assert torch.equal(st.masks, ref_masks) and torch.equal(st.energies, ref_e)
         assert torch.equal(st.mask_raw, ref)
         st.energies.sum().item()
+
+
+def _config2_sample(k):
+    from maomi_amd.workloads import CONFIG2, site_offsets
+    rng = np.random.default_rng(CONFIG2.seed + 3)
+    return site_offsets(CONFIG2)[np.sort(rng.choice(CONFIG2.num_sites, k, replace=False))]
+
+
+def test_config4_full_shape_against_fp64_oracle(gpu_lib):
+    """Mixed precision at the 4225-site workload's shape (256^2 field, 60 slices, Zernike
+    pupil): FP16 screens with a per-slice max-abs scale, FP32 fields and FFTs, against the FP64
+    oracle on 8 seeded sampled sites.  Bar from the measured 1.2e-4 (64 sites, worst 1.9e-4):
+    2e-4 over the sample, 3e-4 per site."""
+    from maomi_amd.workloads import CONFIG2
+    st, vol, offs, zw = _run(CONFIG2, _config2_sample(8), precision="mixed")
+    assert st.phase is None and st.phase16 is not None
+    ref = O.plain_stage(CONFIG2, vol.astype(np.float64), offs, zw, dtype=np.complex128,
+                        with_masks=False)["masks"]
+    got = st.masks.cpu().numpy()
+    per = [rel_l2(g, r) for g, r in zip(got, ref)]
+    err = rel_l2(got, ref)
+    print(f"config4 rel L2 vs FP64 oracle: all {err:.3e}, worst site {max(per):.3e}", flush=True)
+    assert err <= 2e-4 and max(per) <= 3e-4
+
+
+def test_config2_fp32_against_fp64_oracle(gpu_lib):
+    """The FP32 path at the 4225-site workload's shape against the FP64 oracle.  The float32
+    pupil (a focusing chirp of ~5e4 rad, computed in the reference's float32 order) puts the
+    complex64 reference path itself ~2.2e-5 from FP64, and this path follows it (bar 4e-5).
+    Given the FP64 oracle's entry beam, the FP32 propagation over 60 slices is within 1.5e-5
+    of the FP64 oracle (the complex64 CPU propagation measures 9.8e-6 there)."""
+    import torch
+    from maomi_amd.workloads import CONFIG2
+    st, vol, offs, zw = _run(CONFIG2, _config2_sample(4))
+    ref = O.plain_stage(CONFIG2, vol.astype(np.float64), offs, zw, dtype=np.complex128,
+                        with_masks=False)
+    err = rel_l2(st.masks.cpu().numpy(), ref["masks"])
+    st.base.copy_(torch.as_tensor(ref["base"].astype(np.complex64), device=st.device))
+    st.propagate()
+    torch.cuda.synchronize()
+    err64 = rel_l2(st.masks.cpu().numpy(), ref["masks"])
+    print(f"config2 FP32 vs FP64 oracle: {err:.3e}; from the FP64 beam: {err64:.3e}", flush=True)
+    assert err < 4e-5
+    assert err64 < 1.5e-5

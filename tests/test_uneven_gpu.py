@@ -103,3 +103,41 @@ def test_mixed_sizes_with_512_sites_match_oracle(gpu_lib, threads):
     ref = _oracle_masks(cfg, vol, st.windows, sizes, slices, zernike_weights(cfg))
     for i in range(len(sizes)):
         assert rel_l2(st.site_mask(i).cpu().numpy(), ref[i]) <= 1e-5, i
+
+
+@pytest.mark.gpu
+def test_config3_real_draw_sampled_sites_match_oracle(gpu_lib):
+    """The uneven workload as drawn: 4225 sites of field side 128/256/512 and 10..60 slices in
+    one autotuned, captured launch over the 512 x 512 x 60 volume.  Two seeded sites of each
+    field side, plus the deepest site (S = 60) of each side, against the oracle."""
+    from maomi_amd.uneven import CONFIG3, UnevenStage, draw_sites
+    from maomi_amd.workloads import index_volume, site_offsets, zernike_weights
+    cfg = CONFIG3.base
+    sizes, slices = draw_sites(CONFIG3)
+    assert slices.max() == 60
+    vol = index_volume(cfg)
+    print(f"[uneven] config 3 draw: {len(sizes)} sites", flush=True)
+    st = UnevenStage(cfg, vol, site_offsets(cfg), sizes, slices).plan()
+    tuned = st.autotune(grids=(256, 512), threads=(512,), orders=("lpt", "class_interleaved"),
+                        reps=1)
+    print("autotuned:", tuned, flush=True)
+    for _ in range(2):  # replays with eager work between them
+        st.propagate()
+        st.energies.sum().item()
+    rng = np.random.default_rng(cfg.seed + 5)
+    pick = []
+    for n in (128, 256, 512):
+        cls = np.flatnonzero(sizes == n)
+        pick += list(rng.choice(cls, 2, replace=False))
+        deep = cls[slices[cls] == slices[cls].max()]
+        pick.append(int(deep[0]))
+    pick = sorted(set(int(i) for i in pick))
+    ref = _oracle_masks(cfg, vol, st.windows[pick], sizes[pick], slices[pick],
+                        zernike_weights(cfg))
+    energies = st.energies.cpu().numpy()
+    for j, i in enumerate(pick):
+        got = st.site_mask(i).cpu().numpy()
+        err = rel_l2(got, ref[j])
+        print(f"site {i}: N={sizes[i]} S={slices[i]} rel L2 {err:.2e}", flush=True)
+        assert err <= 1e-5, i
+        assert abs(energies[i] - ref[j].sum(dtype=np.float64)) <= 1e-5 * energies[i]

@@ -44,3 +44,11 @@ def test_green_matrix_has_three_negative_eigenvalues():
     xx, yy = np.meshgrid(t, t, indexing="ij")
     w = np.linalg.eigvalsh(_green(np.stack([xx.ravel(), yy.ravel()], 1)))
     assert (w < 0).sum() == 3
+
+
+def test_lu_library_avoids_magma_off_the_default_stream():
+    """MAGMA's LU runs on its own queues, ordered only with the default stream: any other
+    current stream selects hipSOLVER (torch's "cusolver" name), which runs on that stream."""
+    from maomi_amd.numerics import lu_library
+    assert lu_library(True) == "magma"
+    assert lu_library(False) == "cusolver"
