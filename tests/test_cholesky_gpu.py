@@ -89,4 +89,27 @@ def test_device_v4_coefficients_equal_lu_solution(gpu_lib, name):
     f = v4_factor(tp, tG)
     assert f is not None and f.V is not None and int(f.info) == 0
     w = v4_coefficients(tp, None, torch.as_tensor(v, device="cuda"), factor=f).cpu().numpy()
-    assert np.linalg.norm(w - ref) / np.linalg.norm(ref) < 1e-9
+    # At 4225 lattice points cond(G) is about 1.1e8, so two backward-stable solves may differ by
+    # ~cond * eps in the weights.  Checked: the normwise backward error |G w - v| / (|G| |w|)
+    # against 50 eps (LAPACK's LU sits near 1e-17 here), the weights against LU, and the
+    # interpolated values, which are what the masks use.
+    res = np.linalg.norm(G @ w - v) / np.linalg.norm(v)
+    res_lu = np.linalg.norm(G @ ref - v) / np.linalg.norm(v)
+    eta = np.linalg.norm(G @ w - v) / (np.linalg.norm(G) * np.linalg.norm(w))
+    err = np.linalg.norm(w - ref) / np.linalg.norm(ref)
+    q = rng.random((2000, 2)) * (p.max() - p.min()) + p.min()
+    Gq = _green_pairs(q, p)
+    ev = np.linalg.norm(Gq @ w - Gq @ ref) / np.linalg.norm(Gq @ ref)
+    print(f"  residual {res:.2e} (LU {res_lu:.2e}), backward error {eta:.2e}, weights {err:.2e}, "
+          f"values {ev:.2e}", flush=True)
+    assert eta <= 50 * np.finfo(np.float64).eps
+    assert err < (1e-9 if len(p) < 2000 else 5e-9)
+    assert ev < (1e-9 if len(p) < 2000 else 3e-9)
+
+
+def _green_pairs(q, p):
+    d2 = ((q[:, None, :] - p[None, :, :]) ** 2).sum(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = d2 * (np.log(d2) * 0.5 - 1)
+    g[d2 == 0] = 0
+    return g
