@@ -157,6 +157,11 @@ def lib():
         if not _PATH.exists():
             raise RuntimeError(f"HIP library not built: {_PATH} is missing "
                                "(run `python -c 'import __graft_entry__ as g; g.build()'`)")
+        # torch ships its own HIP runtime (libamdhip64.so.7).  Loaded first, it also serves the
+        # library's libamdhip64.so.7 dependency, so kernels launch in the runtime that owns
+        # torch's memory and streams.  Loaded after the library, it would be a second runtime
+        # in the process, and the library's launches would fail ("no ROCm-capable device").
+        import torch  # noqa: F401
         handle = ctypes.CDLL(str(_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)

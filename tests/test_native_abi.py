@@ -58,3 +58,17 @@ def test_pupil_params_layout_and_rejection():
     p.kind = 2  # a pair beam needs the column phase
     rc = _native.lib().maomi_pupil_field(None, 8, 8, None, None, None, p, None)
     assert rc == -1 and b"maomi_pupil_field: bad arguments" in _native.lib().maomi_last_error()
+
+
+def test_library_binds_to_torchs_hip_runtime():
+    """Loading the library through the wrapper in a fresh process maps one HIP runtime: torch's
+    libamdhip64.so.7 also serves the library's dependency (a second runtime would launch the
+    library's kernels outside torch's device context)."""
+    import subprocess
+    import sys
+    code = ("from maomi_amd import _native; _native.lib(); "
+            "print(sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True,
+                         cwd=str(HEADER.parents[1])).stdout
+    runtimes = eval(out.strip().splitlines()[-1])
+    assert len(runtimes) == 1, runtimes
