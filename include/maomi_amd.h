@@ -278,6 +278,16 @@ int maomi_v4_apply_split_f64(const double* points, const double* coef, const dou
                              double* out, int64_t P, int64_t Q, int64_t C, double* partial,
                              int64_t rows, maomi_stream_t stream);
 int64_t maomi_v4_apply_rows(int64_t P, int64_t Q);
+/* maomi_v4_apply_split_f64 that allocates nothing and takes no partial-sum buffer.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The partial sums live in a static device
+ * array of MAOMI_V4_STATIC_PARTIALS doubles (32 MiB) that each GPU holds until the process
+ * ends; rows * Q * C above that (for rows > 1) is rejected with MAOMI_EINVAL.  Calls that
+ * overlap on one GPU share the array and race: order them on one stream. */
+#define MAOMI_V4_STATIC_PARTIALS ((int64_t)4 * 1024 * 1024)
+int maomi_v4_apply_split_f64_noalloc(const double* points, const double* coef,
+                                     const double* queries, double* out, int64_t P, int64_t Q,
+                                     int64_t C, int64_t rows, maomi_stream_t stream);
 /* Part of the v4 solve that replaces np.linalg.solve (maomi/numerics.py:129): in-place FP64
  * Cholesky factorisation A = L L^T of the positive-definite block of the projected Green
  * matrix.  A is row-major with leading dimension lda >= m; only the lower triangle is read and

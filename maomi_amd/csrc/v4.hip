@@ -6,6 +6,8 @@
 //   out(q, c) = sum_i G(|q - p_i|) coef(i, c).
 // Points and coefficients are staged through LDS in tiles; each thread owns
 // one query and keeps C accumulators in registers.
+#include <mutex>
+
 #include "common.hpp"
 #include "maomi_amd.h"
 
@@ -501,4 +503,71 @@ MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* work
     MAOMI_CHECK_LAUNCH();
   }
   return MAOMI_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// maomi_v4_apply_split_f64 without a caller-supplied partial-sum buffer, for callers that cannot
+// allocate device memory.  The rows * Q * C partial sums live in `scratch`, a file-scope device
+// array of kV4StaticPartials doubles (32 MiB) that each GPU holds until the process ends.  Host
+// code never passes the array's name to a kernel: it is a host-side stand-in.  Its device
+// address is looked up once per device (hipGetSymbolAddress, with that device current) on the
+// first call there and kept in g_scratch.  Every partial sum is written before it is read, so
+// nothing needs clearing between calls; calls that overlap on one GPU share the array and race.
+namespace {
+
+constexpr int64_t kV4StaticPartials = MAOMI_V4_STATIC_PARTIALS;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kV4StaticPartials];
+
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of this GPU's scratch array, or nullptr (message set) on failure.
+double* device_scratch() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    maomi_set_error("maomi_v4_apply_split_f64_noalloc: no current device (%s)",
+                    hipGetErrorString(e));
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      maomi_set_error("maomi_v4_apply_split_f64_noalloc: hipGetSymbolAddress failed on device "
+                      "%d (%s)", dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch[dev];
+}
+
+}  // namespace
+
+MAOMI_API int maomi_v4_apply_split_f64_noalloc(const double* points, const double* coef,
+                                               const double* queries, double* out, int64_t P,
+                                               int64_t Q, int64_t C, int64_t rows,
+                                               maomi_stream_t stream) {
+  if (!points || !coef || !queries || !out || P < 1 || Q < 0 || C < 1 || C > kMaxC ||
+      rows < 1 || rows > 65535) {
+    maomi_set_error("maomi_v4_apply_split_f64_noalloc: bad arguments (1 <= C <= %d)", kMaxC);
+    return MAOMI_EINVAL;
+  }
+  if (rows > 1 && Q > kV4StaticPartials / (rows * C)) {
+    maomi_set_error("maomi_v4_apply_split_f64_noalloc: rows * Q * C = %lld partial sums exceed "
+                    "the static buffer of %lld; use fewer rows or split the queries",
+                    (long long)(rows * C) * (long long)Q, (long long)kV4StaticPartials);
+    return MAOMI_EINVAL;
+  }
+  if (Q == 0) return MAOMI_OK;
+  double* part = nullptr;
+  if (rows > 1) {
+    part = device_scratch();
+    if (!part) return MAOMI_EINVAL;
+  }
+  return maomi_v4_apply_split_f64(points, coef, queries, out, P, Q, C, part, rows, stream);
 }
