@@ -24,41 +24,16 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// 1: each pass requests the next tile's field lines once the current tile's second stage
-// is done, so that the load latency overlaps the exchange and third stage.
-// Bit 0: row pass, bit 1: column pass.
-#ifndef MAOMI_SITES_PREFETCH
-#define MAOMI_SITES_PREFETCH 2
-#endif
-constexpr bool kPrefetchRows = (MAOMI_SITES_PREFETCH & 1) != 0;
-constexpr bool kPrefetchCols = (MAOMI_SITES_PREFETCH & 2) != 0;
+// The column pass requests the next tile's field lines once the current tile's second stage is
+// done, so that the load latency overlaps the exchange and third stage.  (The same prefetch in
+// the row pass measured slower: its registers are taken by the hoisted screen and mask loads.)
+constexpr bool kPrefetchCols = true;
 
-// 1: screens use trig::sincos_screen (|phi| < 2^17, checked on the host); 0: sincosf.
-// 1: the row pass requests a tile's phases and mask pixels before its first stage.
-#ifndef MAOMI_SITES_HOIST
-#define MAOMI_SITES_HOIST 1
-#endif
-constexpr bool kHoist = MAOMI_SITES_HOIST != 0;
-
-#ifndef MAOMI_SITES_FAST_TRIG
-#define MAOMI_SITES_FAST_TRIG 1
-#endif
-// Occupancy target in waves per SIMD for the sites kernels (0: compiler's choice).  4 caps
-// them at 128 VGPRs, so two 512-thread blocks share a CU.
-#ifndef MAOMI_SITES_WAVES
-#define MAOMI_SITES_WAVES 0
-#endif
-#if MAOMI_SITES_WAVES > 0
-#define MAOMI_SITES_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MAOMI_SITES_WAVES)))
-#else
-#define MAOMI_SITES_OCCUPANCY
-#endif
+// The row pass requests a tile's screen phases and mask pixels before its first stage.
+constexpr bool kHoist = true;
 
 __device__ __forceinline__ void screen_sincos(float phi, float* s, float* c) {
-  if (MAOMI_SITES_FAST_TRIG)
-    trig::sincos_screen(phi, s, c);
-  else
-    sincosf(phi, s, c);
+  trig::sincos_screen(phi, s, c);  // |phi| < 2^17, checked on the host
 }
 
 template <int N> struct ClassIdx;
@@ -154,7 +129,6 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
   const int64_t pslice = (int64_t)s * a.H;
   const bool p1 = tid < L * N1;
   const int l1 = tid / N1, n1 = tid % N1;
-  float2 pre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
 #pragma unroll 1
   for (int t = 0; t < N / L; ++t) {
     const int row0 = t * L;
@@ -181,14 +155,9 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
     if (!FIRST) {
       if (p1) {  // stage 1 of the inverse transform along y
         float2 v[N2];
-        if (kPrefetchRows && t > 0) {
+        const float2* src = c.fld + (int64_t)(row0 + l1) * N + n1;
 #pragma unroll
-          for (int n2 = 0; n2 < N2; ++n2) v[n2] = pre[n2];
-        } else {
-          const float2* src = c.fld + (int64_t)(row0 + l1) * N + n1;
-#pragma unroll
-          for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
-        }
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
         fftk::fft<N2, true>(v);
         float2* dst = sm.x + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
@@ -255,11 +224,6 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
 #pragma unroll
           for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
         }
-      }
-      if (kPrefetchRows && jj == J2 - 1 && !FIRST && p1 && t + 1 < N / L) {
-        const float2* src = c.fld + (int64_t)(row0 + L + l1) * N + n1;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) pre[n2] = src[N1 * n2];
       }
       lds_barrier();
       if (!LAST && p2) {
@@ -401,14 +365,14 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
 
 // One field side for every site of the launch.
 template <int N, int NT, int MODE>
-__global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_kernel(SitesArgs a) {
+__global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
   __shared__ Smem<NT, N> sm;
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, MODE>(a, a.order[i], sm);
 }
 
 // Sites of mixed field sides in one launch (the order list puts large sites first).
 template <int NT>
-__global__ __launch_bounds__(NT) MAOMI_SITES_OCCUPANCY void sites_mixed_kernel(SitesArgs a) {
+__global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
   __shared__ Smem<NT, kNmax> sm;
   for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
     const int site = a.order[i];

@@ -265,11 +265,7 @@ __device__ __noinline__ void wide_stage2(int a_off, int b_off, int t_off, int nl
   }
 }
 
-#ifndef MAOMI_SEG_FAST
-#define MAOMI_SEG_FAST 1
-#endif
-
-__device__ __forceinline__ int fast_r2(int N) { return MAOMI_SEG_FAST ? fast_split(N) : 0; }
+__device__ __forceinline__ int fast_r2(int N) { return fast_split(N); }
 
 #define MAOMI_SEG_SIZES(X)                                                                 \
   X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17)  \
@@ -731,7 +727,7 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
   // scratch line stride: the padded layout of the fast split (fast_r2 on the device)
   a.nb = (int)n;
   for (const int64_t side : {n, n2}) {
-    const int r2 = MAOMI_SEG_FAST ? fast_split((int)side) : 0;
+    const int r2 = fast_split((int)side);
     if (r2 && pad_stride((int)side, r2) > a.nb) a.nb = pad_stride((int)side, r2);
   }
   // line buffers A (L x n) and B (L x nb) complex plus tables, kept under 60 KiB of LDS
@@ -744,7 +740,7 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
   // the kernel of the largest line-split class among the two field sides
   int cls = 0;
   for (const int64_t side : {n, n2}) {
-    const int r2 = MAOMI_SEG_FAST ? fast_split((int)side) : 0;
+    const int r2 = fast_split((int)side);
     const int c = r2 ? split_class((int)side, r2) : 2;
     if (c > cls) cls = c;
   }
