@@ -77,6 +77,10 @@ typedef struct maomi_sites_desc {
   int64_t psf_c0, psf_w;      /*   of each site window (training targets of the surrogate) */
   const void* screen;         /* [Z][H][W] complex64 exp(i phase) (maomi_screen_exp), or NULL;
                                  exactly one of phase / phase16 / screen is given            */
+  int32_t* queue;             /* optional device int32 work counter: blocks take the next
+                                 site of the order list as they finish one (dynamic
+                                 scheduling; the launch resets it first).  NULL: block b takes
+                                 entries b, b + grid, ... (static striding)                  */
 } maomi_sites_desc;
 
 int maomi_persistent_sites(const maomi_sites_desc* desc, maomi_stream_t stream);
@@ -84,6 +88,40 @@ int maomi_persistent_sites(const maomi_sites_desc* desc, maomi_stream_t stream);
 int64_t maomi_sites_workspace_bytes(int64_t grid, int64_t nmax);
 /* Default persistent grid for this device (blocks resident at once). */
 int64_t maomi_sites_default_grid(void);
+
+/* Plan object of the sites launch.  Replaces the captured batches of OpticalPlan.prepare_graph
+ * / evaluate_graph (maomi/backends/cuda/batched.py:103-146): one HIP graph holds the whole
+ * run's launch (every site, mixed field sides and depths), with dynamic site scheduling, and
+ * the autotuner picks the grid (sites in flight), the block shape (256 or 512 threads: 16- or
+ * 32-line LDS tiles of the line transforms) and the site ordering by timing captured
+ * candidates.  The plan owns the orderings, the per-block workspace, the work counter and the
+ * hipGraphExec; the descriptor's other buffers stay the caller's and are read at each replay
+ * (its order, workspace, scratch_stride, grid, block_threads and queue are the plan's).
+ * create reads site_n / site_s back to the host (synchronous).  FP32 phase screens only.
+ * capture, tune and destroy must not overlap replays still running (they synchronise the
+ * device before freeing). */
+typedef struct maomi_sites_plan_t* maomi_sites_plan;
+enum { MAOMI_ORDER_NATURAL = 0,  /* the caller's site order                                  */
+       MAOMI_ORDER_LPT = 1,      /* longest first by S N^2 log2 N                            */
+       MAOMI_ORDER_CLASS = 2 };  /* field side, then depth, both descending                  */
+typedef struct maomi_sites_choice {
+  int64_t grid, block_threads, order;
+  double ms;                     /* best timed replay, or -1 for an untimed capture          */
+} maomi_sites_choice;
+int maomi_sites_plan_create(const maomi_sites_desc* desc, maomi_sites_plan* plan);
+/* Capture one given choice without timing. */
+int maomi_sites_plan_capture(maomi_sites_plan plan, int64_t grid, int64_t block_threads,
+                             int64_t order);
+/* Capture and time every (grid, threads, order) candidate (reps timed replays each, after one
+ * warm-up) on `stream` and keep the fastest; results (optional) receives every candidate in
+ * grid-major order, best the winner. */
+int maomi_sites_plan_tune(maomi_sites_plan plan, const int64_t* grids, int n_grids,
+                          const int64_t* threads, int n_threads, const int64_t* orders,
+                          int n_orders, int reps, maomi_stream_t stream,
+                          maomi_sites_choice* results, maomi_sites_choice* best);
+int maomi_sites_plan_replay(maomi_sites_plan plan, maomi_stream_t stream);
+int maomi_sites_plan_choice(maomi_sites_plan plan, maomi_sites_choice* out);
+int maomi_sites_plan_destroy(maomi_sites_plan plan);
 
 /* ---------------------------------------------------------------------------
  * Phase preparation from a refractive-index volume.
@@ -297,6 +335,25 @@ int maomi_v4_apply_split_f64_noalloc(const double* points, const double* coef,
 int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
                        maomi_stream_t stream);
 int64_t maomi_cholesky_f64_workspace(void);
+
+/* ---------------------------------------------------------------------------
+ * 2-D PSF contracts (maomi/psf.py), FP64.  maomi_amd.psf is the host side.
+ * ------------------------------------------------------------------------- */
+
+/* The interpolation of resample_psf (maomi/psf.py:154-176, scipy map_coordinates order 1,
+ * mode "constant"): dst[i][j] = bilinear(src, cx[i], cy[j]), 0 where cx[i] or cy[j] lies
+ * outside [0, size - 1].  src: sx x sy row-major; dst: nx x ny. */
+int maomi_resample_linear_f64(const double* src, int64_t sx, int64_t sy, const double* cx,
+                              int64_t nx, const double* cy, int64_t ny, double* dst,
+                              maomi_stream_t stream);
+/* The pupil of pupil_to_intensity (maomi/psf.py:179-203): amp exp(2 pi i opd) where opd is
+ * finite and amp > 0 (amp NULL: ones), else 0, embedded at n // 2 - p // 2 of an n x n grid and
+ * written ifftshifted (complex128, n x n), ready for a forward FFT. */
+int maomi_pupil_embed_f64(const double* opd, const double* amp, int64_t px, int64_t py,
+                          int64_t n, void* out, maomi_stream_t stream);
+/* out = fftshift(|F|^2) for an n x n complex128 F (pupil_to_intensity's intensity before its
+ * normalisation). */
+int maomi_power_shift_f64(const void* F, int64_t n, double* out, maomi_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Learned PSF surrogate (no counterpart in the reference: the reference has no

@@ -26,8 +26,17 @@ class SitesDesc(ctypes.Structure):
         ("grid", c_int64), ("uniform_n", c_int64), ("block_threads", c_int64),
         ("class_edge", c_int64 * 5), ("class_q", c_int64 * 5), ("class_tw", c_int64 * 5),
         ("psf_out", c_void_p), ("psf_z0", c_int64), ("psf_nz", c_int64), ("psf_c0", c_int64),
-        ("psf_w", c_int64), ("screen", c_void_p),
+        ("psf_w", c_int64), ("screen", c_void_p), ("queue", c_void_p),
     ]
+
+
+class SitesChoice(ctypes.Structure):
+    """Mirror of maomi_sites_choice."""
+    _fields_ = [("grid", c_int64), ("block_threads", c_int64), ("order", c_int64),
+                ("ms", c_double)]
+
+
+ORDERS = {"natural": 0, "lpt": 1, "class_interleaved": 2}
 
 
 class PupilParams(ctypes.Structure):
@@ -71,6 +80,15 @@ _SIGS = {
     "maomi_persistent_sites": (c_int, [ctypes.POINTER(SitesDesc), c_void_p]),
     "maomi_sites_workspace_bytes": (c_int64, [c_int64, c_int64]),
     "maomi_sites_default_grid": (c_int64, []),
+    "maomi_sites_plan_create": (c_int, [ctypes.POINTER(SitesDesc), ctypes.POINTER(c_void_p)]),
+    "maomi_sites_plan_capture": (c_int, [c_void_p, c_int64, c_int64, c_int64]),
+    "maomi_sites_plan_tune": (c_int, [c_void_p, ctypes.POINTER(c_int64), c_int,
+                                      ctypes.POINTER(c_int64), c_int, ctypes.POINTER(c_int64),
+                                      c_int, c_int, c_void_p, ctypes.POINTER(SitesChoice),
+                                      ctypes.POINTER(SitesChoice)]),
+    "maomi_sites_plan_replay": (c_int, [c_void_p, c_void_p]),
+    "maomi_sites_plan_choice": (c_int, [c_void_p, ctypes.POINTER(SitesChoice)]),
+    "maomi_sites_plan_destroy": (c_int, [c_void_p]),
     "maomi_phase_prepare": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_double,
                                     c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "maomi_twiddles": (c_int, [c_void_p, c_int64, c_void_p]),
@@ -145,6 +163,11 @@ _SIGS = {
     "maomi_v4_apply_rows": (c_int64, [c_int64, c_int64]),
     "maomi_cholesky_f64": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     "maomi_cholesky_f64_workspace": (c_int64, []),
+    "maomi_resample_linear_f64": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int64,
+                                          c_void_p, c_int64, c_void_p, c_void_p]),
+    "maomi_pupil_embed_f64": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p,
+                                      c_void_p]),
+    "maomi_power_shift_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
 }
 
 

@@ -26,6 +26,8 @@ SOURCES = {
     "surrogate.hip": [],
     "local_fft.hip": [],
     "inputs.hip": ["-ffp-contract=off"],
+    "psf2d.hip": ["-ffp-contract=off"],
+    "plan.hip": [],
 }
 
 

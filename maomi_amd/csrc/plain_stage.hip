@@ -77,7 +77,33 @@ struct SitesArgs {
   int64_t cedge[5], cq[5], ctw[5];
   float* psf_out;  // optional [psf_nz][P][psf_w][psf_w] planes |u|^4 around focus
   int psf_z0, psf_nz, psf_c0, psf_w;
+  int* queue;      // dynamic work counter, or nullptr for static striding
 };
+
+// The persistent blocks' site loop.  With a work counter each block takes the next entry of
+// the order list as it finishes a site, so sites of unequal cost (mixed field sides and depths)
+// balance over the blocks as greedy list scheduling does; every block leaves once the counter
+// passes P.  Without one, block b takes entries b, b + grid, ...
+template <class F>
+__device__ __forceinline__ void for_each_site(const SitesArgs& a, F&& run) {
+  if (!a.queue) {
+    for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run(a.order[i]);
+    return;
+  }
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int64_t i = next;
+    __syncthreads();  // every thread has read `next` before thread 0 takes the following one
+    if (i >= a.P) return;
+    run(a.order[i]);
+  }
+}
+
+__global__ void queue_reset_kernel(int* queue) {
+  if (threadIdx.x == 0) *queue = 0;
+}
 
 template <int NT, int NMAX>
 struct Smem {
@@ -367,15 +393,14 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
 template <int N, int NT, int MODE>
 __global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
   __shared__ Smem<NT, N> sm;
-  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, MODE>(a, a.order[i], sm);
+  for_each_site(a, [&](int site) { run_site<N, NT, MODE>(a, site, sm); });
 }
 
 // Sites of mixed field sides in one launch (the order list puts large sites first).
 template <int NT>
 __global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
   __shared__ Smem<NT, kNmax> sm;
-  for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) {
-    const int site = a.order[i];
+  for_each_site(a, [&](int site) {
     switch (a.site_n[site]) {
       case 64: run_site<64, NT, kScreens32>(a, site, sm); break;
       case 128: run_site<128, NT, kScreens32>(a, site, sm); break;
@@ -383,7 +408,7 @@ __global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
       case 512: run_site<512, NT, kScreens32>(a, site, sm); break;
       default: break;  // rejected on the host
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------- phase prep
@@ -503,6 +528,11 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
     maomi_set_error("maomi_persistent_sites: bad sizes");
     return MAOMI_EINVAL;
   }
+  if (d->uniform_n > 0 && d->scratch_stride < d->uniform_n * d->uniform_n) {
+    maomi_set_error("maomi_persistent_sites: scratch_stride %lld is below uniform_n^2 = %lld",
+                    (long long)d->scratch_stride, (long long)(d->uniform_n * d->uniform_n));
+    return MAOMI_EINVAL;
+  }
   SitesArgs a;
   a.bases = (const float2*)d->bases; a.base_off = d->base_off;
   a.phase = d->phase; a.phase16 = (const __half*)d->phase16; a.slice_scale = d->slice_scale;
@@ -516,7 +546,10 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   a.psf_out = d->psf_out;
   a.psf_z0 = (int)d->psf_z0; a.psf_nz = (int)d->psf_nz; a.psf_c0 = (int)d->psf_c0;
   a.psf_w = (int)d->psf_w;
-  if (a.psf_out && (a.psf_nz < 1 || a.psf_w < 1 || a.psf_z0 < 0 || a.psf_c0 < 0)) {
+  a.queue = d->queue;
+  if (a.psf_out && (a.psf_nz < 1 || a.psf_w < 1 || a.psf_z0 < 0 || a.psf_c0 < 0 ||
+                    (d->uniform_n > 0 && d->psf_c0 + d->psf_w > d->uniform_n) ||
+                    d->psf_z0 + d->psf_nz > d->Z + 1)) {
     maomi_set_error("maomi_persistent_sites: bad focal-plane output window");
     return MAOMI_EINVAL;
   }
@@ -558,6 +591,15 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
     default:                                                                                   \
       maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");     \
       return MAOMI_EUNSUPPORTED;                                                               \
+  }
+  if (d->uniform_n != 0 && d->uniform_n != 64 && d->uniform_n != 128 && d->uniform_n != 256 &&
+      d->uniform_n != 512) {
+    maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");
+    return MAOMI_EUNSUPPORTED;
+  }
+  if (a.queue) {  // every argument is checked: nothing is enqueued before this point
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, st, a.queue);
+    MAOMI_CHECK_LAUNCH();
   }
   if (mode == kScreens32Psf) {
     MAOMI_LAUNCH(512, kScreens32Psf)

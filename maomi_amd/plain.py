@@ -261,6 +261,9 @@ class PlainStage:
         desc.mask, desc.mask_off = self.masks.data_ptr(), self.mask_off.data_ptr()
         desc.energy = self.energies.data_ptr()
         desc.workspace = self.workspace.data_ptr()
+        if getattr(self, "queue", None) is None:
+            self.queue = torch.zeros(1, dtype=torch.int32, device=d)
+        desc.queue = self.queue.data_ptr()  # dynamic site scheduling
         desc.scratch_stride = stride
         desc.P, desc.H, desc.W, desc.Z = P, cfg.padded, cfg.padded, cfg.slices
         desc.grid, desc.uniform_n, desc.block_threads = grid, cfg.n, self.block_threads

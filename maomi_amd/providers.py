@@ -55,3 +55,50 @@ class OpticalStageProvider:
         self.result = run_fixed(self.config, self.volume, count_per_axis=self.count_per_axis,
                                 device=self.device, **kw)
         return native_field(self.result, config=self.config)
+
+
+class OpticalStagePSF:
+    """The GPU optical stage as a 2-D PSF provider (maomi/psf.py:66-71): a callable
+    (x_um, y_um) -> SampledPSF.
+
+    The kernel at a site is the stage's focal two-photon PSF (``OpticalPlan.evaluate``'s
+    ``psfs``, axes X, Y, Z at 1 / vres um) reduced over Z -- the axial sum by default, or the
+    central plane with ``projection="focus"`` -- as magnitudes (|psf|, as the native RANA
+    attachment takes them), normalised to unit sum.  Pitch 1 / vres um, wavelength the
+    excitation wavelength.  ``prefetch(x_um, y_um)`` evaluates every node of a grid in one
+    persistent launch, so ``PSFField.from_provider`` costs one launch; nodes are kept, and a
+    node outside every prefetched grid is evaluated on its own.  Sites must land on the
+    supersampled lattice (the plan's own check)."""
+
+    def __init__(self, config, raw_fields, *, uin=None, device="cuda", projection="sum"):
+        from .batched import OpticalPlan
+        if projection not in ("sum", "focus"):
+            raise ValueError("projection must be 'sum' or 'focus'")
+        self.config, self.projection = config, projection
+        self.plan = OpticalPlan(config, raw_fields, uin=uin, device=device)
+        self.pixel_size_um = (1.0 / config.vres, 1.0 / config.vres)
+        self.wavelength_nm = float(config.wavelength_um) * 1000.0
+        self.kernels = {}
+        self.launches = 0
+
+    def prefetch(self, x_um, y_um):
+        xs, ys = np.asarray(x_um, float).ravel(), np.asarray(y_um, float).ravel()
+        pts = np.stack(np.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2)
+        todo = np.array([p for p in pts if (float(p[0]), float(p[1])) not in self.kernels])
+        if not len(todo):
+            return
+        psfs = np.abs(np.asarray(self.plan.evaluate(todo)["psfs"], np.float64))
+        self.launches += 1
+        k = psfs.sum(-1) if self.projection == "sum" else psfs[..., psfs.shape[-1] // 2]
+        for p, kern in zip(todo, k):
+            self.kernels[(float(p[0]), float(p[1]))] = kern
+
+    def __call__(self, x_um, y_um):
+        from .psf import SampledPSF
+        key = (float(x_um), float(y_um))
+        if key not in self.kernels:
+            self.prefetch([key[0]], [key[1]])
+        return SampledPSF.normalized(
+            self.kernels[key], self.pixel_size_um, self.wavelength_nm,
+            {"source": "MI355X optical stage (two-photon focal PSF)",
+             "projection": self.projection, "x_um": key[0], "y_um": key[1]})

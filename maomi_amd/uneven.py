@@ -10,7 +10,7 @@ and replayed.  An autotuner picks the grid size, the tile shape (256- or
 timing the captured candidates on the device.
 """
 
-import itertools
+import ctypes
 import time
 from dataclasses import dataclass, replace
 
@@ -97,7 +97,7 @@ class UnevenStage:
         }
         self.tuned = None
         self.timing = {}
-        self._graph = None
+        self._plan = None
 
     def plan(self):
         cfg, d = self.cfg, self.device
@@ -151,7 +151,9 @@ class UnevenStage:
                        out.view(B, n, n), d)
         return self
 
-    def _desc(self, order, grid, threads, work):
+    def _desc(self):
+        """The launch descriptor of this run's buffers (the plan supplies the order, grid, block
+        shape, workspace and work counter)."""
         cfg = self.cfg
         desc = nat.SitesDesc()
         desc.bases, desc.base_off = self.bases.data_ptr(), self.base_off.data_ptr()
@@ -160,63 +162,65 @@ class UnevenStage:
         desc.twiddle = self.tables.tw.data_ptr()
         desc.site_x, desc.site_y = self.site_x.data_ptr(), self.site_y.data_ptr()
         desc.site_n, desc.site_s = self.site_n.data_ptr(), self.site_s.data_ptr()
-        desc.order = order.data_ptr()
         desc.mask, desc.mask_off = self.masks.data_ptr(), self.mask_off.data_ptr()
         desc.energy = self.energies.data_ptr()
-        desc.workspace = work.data_ptr()
-        desc.scratch_stride = int(self.sizes.max()) ** 2
         desc.P, desc.H, desc.W, desc.Z = len(self.sizes), cfg.padded, cfg.padded, cfg.slices
-        desc.grid, desc.uniform_n, desc.block_threads = grid, 0, threads
+        desc.uniform_n = 0
         for i in range(5):
             desc.class_edge[i] = desc.class_q[i] = desc.class_tw[i] = self.tables.offsets[i]
         return desc
 
-    def _capture(self, order_name, grid, threads):
-        d = self.device
-        order = torch.as_tensor(self.orders[order_name].astype(np.int32), device=d)
-        work = torch.empty(grid * int(self.sizes.max()) ** 2, dtype=torch.complex64, device=d)
-        desc = self._desc(order, grid, threads, work)
-        lib = nat.lib()
-        nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "warm-up launch")
-        torch.cuda.synchronize(d)
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            nat.check(lib.maomi_persistent_sites(desc, nat.stream_handle(d)), "captured launch")
-        return dict(graph=graph, keep=(order, work, desc), order=order_name, grid=grid,
-                    threads=threads)
+    def _plan_handle(self):
+        """The C-ABI plan (maomi_sites_plan_create) over this run's buffers, made once."""
+        if getattr(self, "_plan", None) is None:
+            h = ctypes.c_void_p()
+            nat.check(nat.lib().maomi_sites_plan_create(self._desc(), ctypes.byref(h)),
+                      "sites plan")
+            self._plan = h
+        return self._plan
+
+    def __del__(self):
+        h = getattr(self, "_plan", None)
+        if h is not None and nat._LIB is not None:
+            try:
+                nat.lib().maomi_sites_plan_destroy(h)
+            except Exception:  # interpreter shutdown: the runtime may already be gone
+                pass
 
     def autotune(self, grids=None, threads=(512, 256), orders=("lpt", "class_interleaved"),
                  reps=2):
-        """Time captured candidates and keep the fastest (grid, block shape, ordering)."""
+        """Time captured candidates (grid, block shape, ordering) in the plan and keep the
+        fastest (maomi_sites_plan_tune)."""
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-        grids = grids or (cus, 2 * cus)
-        results = []
-        best = None
-        for g, t, o in itertools.product(grids, threads, orders):
-            g = min(int(g), len(self.sizes))
-            cand = self._capture(o, g, t)
-            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            ms = []
-            for _ in range(reps):
-                ev0.record()
-                cand["graph"].replay()
-                ev1.record()
-                torch.cuda.synchronize(self.device)
-                ms.append(ev0.elapsed_time(ev1))
-            cand["ms"] = min(ms)
-            results.append({k: cand[k] for k in ("grid", "threads", "order", "ms")})
-            if best is None or cand["ms"] < best["ms"]:
-                best = cand
-        self._graph = best
-        self.tuned = {k: best[k] for k in ("grid", "threads", "order", "ms")}
-        self.tune_log = results
+        grids = [int(g) for g in (grids or (cus, 2 * cus))]
+        threads, orders = [int(t) for t in threads], [nat.ORDERS[o] for o in orders]
+        arr = lambda v: (ctypes.c_int64 * len(v))(*v)
+        n = len(grids) * len(threads) * len(orders)
+        results = (nat.SitesChoice * n)()
+        best = nat.SitesChoice()
+        nat.check(nat.lib().maomi_sites_plan_tune(
+            self._plan_handle(), arr(grids), len(grids), arr(threads), len(threads), arr(orders),
+            len(orders), int(reps), nat.stream_handle(self.device), results, ctypes.byref(best)),
+            "sites plan tune")
+        name = {v: k for k, v in nat.ORDERS.items()}
+        as_dict = lambda c: {"grid": int(c.grid), "threads": int(c.block_threads),
+                             "order": name[int(c.order)], "ms": float(c.ms)}
+        self.tune_log = [as_dict(c) for c in results]
+        self.tuned = as_dict(best)
         return self.tuned
 
     def propagate(self):
-        if self._graph is None:
+        """Replay the plan's captured launch; without autotune, capture the default choice
+        (one block per CU, 512 threads, longest first) first."""
+        h = self._plan_handle()
+        if self.tuned is None:
             cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-            self._graph = self._capture("lpt", min(cus, len(self.sizes)), 512)
-        self._graph["graph"].replay()
+            nat.check(nat.lib().maomi_sites_plan_capture(h, min(cus, len(self.sizes)), 512,
+                                                         nat.ORDERS["lpt"]), "sites plan capture")
+            self.tuned = {"grid": min(cus, len(self.sizes)), "threads": 512, "order": "lpt",
+                          "ms": -1.0}
+        nat.check(nat.lib().maomi_sites_plan_replay(h, nat.stream_handle(self.device)),
+                  "sites plan replay")
         return self.masks, self.energies
 
     def site_mask(self, i):

@@ -44,10 +44,45 @@ def test_error_reporting_without_gpu_work():
 
 
 def test_sites_descriptor_layout_matches_header():
-    # 19 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding; the complex
-    # screen pointer is the last member
-    assert ctypes.sizeof(_native.SitesDesc) == 8 * (19 + 12 + 15)
+    # 20 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding; the complex
+    # screen pointer and the work counter are the last members
+    assert ctypes.sizeof(_native.SitesDesc) == 8 * (20 + 12 + 15)
     assert _native.SitesDesc.screen.offset == 8 * (18 + 12 + 15)
+    assert _native.SitesDesc.queue.offset == 8 * (19 + 12 + 15)
+    assert ctypes.sizeof(_native.SitesChoice) == 32
+
+
+def test_sites_launch_and_plan_reject_bad_arguments_before_any_gpu_work():
+    lib = _native.lib()
+    d = _native.SitesDesc()
+    one = 8  # non-null addresses: every call below returns before touching them
+    for name in ("bases", "base_off", "phase", "edge", "q1d", "twiddle", "site_x", "site_y",
+                 "site_n", "site_s", "order", "mask", "mask_off", "energy", "workspace"):
+        setattr(d, name, one)
+    d.P, d.H, d.W, d.Z, d.grid, d.uniform_n = 4, 300, 300, 8, 4, 256
+    d.scratch_stride = 128 * 128  # below 256^2
+    assert lib.maomi_persistent_sites(d, None) == -1
+    assert b"scratch_stride" in lib.maomi_last_error()
+    d.scratch_stride = 256 * 256
+    d.psf_out, d.psf_z0, d.psf_nz, d.psf_c0, d.psf_w = one, 0, 3, 200, 64  # 200 + 64 > 256
+    assert lib.maomi_persistent_sites(d, None) == -1
+    assert b"focal-plane output window" in lib.maomi_last_error()
+    d.psf_c0, d.psf_z0, d.psf_nz = 0, 7, 3  # planes 7..9 of 0..8
+    assert lib.maomi_persistent_sites(d, None) == -1
+    d.psf_out = None
+    d.uniform_n = 96
+    assert lib.maomi_persistent_sites(d, None) == -2
+    assert b"uniform_n must be" in lib.maomi_last_error()
+    h = ctypes.c_void_p()
+    assert lib.maomi_sites_plan_create(None, ctypes.byref(h)) == -1
+    d.uniform_n, d.P = 0, 0
+    assert lib.maomi_sites_plan_create(d, ctypes.byref(h)) == -1 and not h.value
+    d.P, d.phase16, d.slice_scale = 4, one, one
+    assert lib.maomi_sites_plan_create(d, ctypes.byref(h)) == -2
+    assert b"FP32-phase" in lib.maomi_last_error()
+    assert lib.maomi_sites_plan_replay(None, None) == -1
+    assert lib.maomi_sites_plan_capture(None, 4, 512, 1) == -1
+    assert lib.maomi_sites_plan_destroy(None) == 0
 
 
 def test_pupil_params_layout_and_rejection():

@@ -141,3 +141,48 @@ def test_config3_real_draw_sampled_sites_match_oracle(gpu_lib):
         print(f"site {i}: N={sizes[i]} S={slices[i]} rel L2 {err:.2e}", flush=True)
         assert err <= 1e-5, i
         assert abs(energies[i] - ref[j].sum(dtype=np.float64)) <= 1e-5 * energies[i]
+
+
+@pytest.mark.gpu
+def test_plan_replay_equals_eager_launches(gpu_lib):
+    """The C-ABI plan's captured launch (dynamic scheduling, plan-owned orderings and
+    workspace) gives bit for bit the masks of eager maomi_persistent_sites launches with static
+    striding, for each candidate the tuner timed, over replays with eager work between them."""
+    import ctypes
+    import torch
+    from maomi_amd import _native as nat
+    from maomi_amd.uneven import UnevenStage
+    from maomi_amd.workloads import PlainConfig, index_volume
+    cfg = PlainConfig(fov=64, n=256, slices=8, sites_per_axis=3, corr_vox=2.0)
+    vol = index_volume(cfg)
+    rng = np.random.default_rng(9)
+    centres = rng.integers(0, 65, (13, 2))
+    sizes = rng.choice([64, 128, 256], 13)
+    slices = rng.integers(1, 9, 13)
+    st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
+    print("[plan] 13 mixed sites", flush=True)
+    tuned = st.autotune(grids=(3, 5), threads=(512, 256), orders=("lpt", "class_interleaved"),
+                        reps=1)
+    assert len(st.tune_log) == 8 and all(c["ms"] > 0 for c in st.tune_log)
+    assert tuned["ms"] == min(c["ms"] for c in st.tune_log)
+    got = []
+    for _ in range(3):
+        st.propagate()
+        got.append((st.masks.clone(), st.energies.clone()))
+        st.masks.sum().item()
+    # eager reference: the same buffers' launch with static striding, natural order
+    ref_masks = st.masks.clone().fill_(float("nan"))
+    ref_e = st.energies.clone().fill_(float("nan"))
+    desc = st._desc()
+    desc.mask, desc.energy = ref_masks.data_ptr(), ref_e.data_ptr()
+    order = torch.arange(len(sizes), dtype=torch.int32, device="cuda")
+    work = torch.empty(4 * int(sizes.max()) ** 2, dtype=torch.complex64, device="cuda")
+    desc.order, desc.workspace = order.data_ptr(), work.data_ptr()
+    desc.scratch_stride, desc.grid, desc.block_threads = int(sizes.max()) ** 2, 4, 512
+    nat.check(gpu_lib.maomi_persistent_sites(desc, nat.stream_handle("cuda")), "eager launch")
+    torch.cuda.synchronize()
+    for m, e in got:
+        assert torch.equal(m, ref_masks) and torch.equal(e, ref_e)
+    choice = nat.SitesChoice()
+    nat.check(gpu_lib.maomi_sites_plan_choice(st._plan, ctypes.byref(choice)), "choice")
+    assert (choice.grid, choice.block_threads) == (tuned["grid"], tuned["threads"])
