@@ -34,12 +34,37 @@ TRAFFIC_REF_DEMO = (2.0 * 2.9498e7 + 1.0 * 6.2773e7) * 1024
 METRIC = "sites/s on 4225-site optical stage; surrogate vs propagator at ≤1% rel L2"
 
 
-def cpu_cores():
+def affinity_cores():
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        return os.cpu_count() or 1
+
+
+def cpu_cores():
+    """Threads for the CPU baseline: every core this process may run on (sched_getaffinity),
+    capped by the host's declared CPU share (OMP_NUM_THREADS) where one is set."""
+    n = affinity_cores()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def host_description():
+    """CPU model (lscpu's "Model name", from /proc/cpuinfo) and the FFT library of the baseline."""
+    import scipy
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "affinity_cores": affinity_cores(),
+            "fft_library": f"scipy.fft (pocketfft) from scipy {scipy.__version__}, "
+                           f"numpy {np.__version__}"}
 
 
 def cpu_baseline(cfg, vol, offsets, zw, n_sites, q_sample, cores):
@@ -82,12 +107,16 @@ def cpu_baseline(cfg, vol, offsets, zw, n_sites, q_sample, cores):
         _ = b @ coef
     t_eval = (time.perf_counter() - t0) * len(q) / len(qs)
     total = t_plan + t_sites / len(pick) * len(offsets) + t_solve + t_eval
+    host = host_description()
     return dict(value=len(offsets) / total, unit="sites/s", cores=cores, kind="port",
-                sample=(f"{len(pick)} of {len(offsets)} sites propagated (scipy.fft, {cores} "
-                        f"threads, serial site loop, shifted FFT sequence), full plan "
-                        f"({t_plan:.2f} s) and v4 solve ({t_solve:.2f} s), v4 evaluation on "
-                        f"{len(qs)} of {len(q)} pixels; per-site time "
-                        f"{t_sites / len(pick):.3f} s; extrapolated total {total:.1f} s"))
+                sample=(f"{len(pick)} of {len(offsets)} seeded sites propagated ({cores} "
+                        f"threads of {host['affinity_cores']} in the affinity mask, serial "
+                        f"site loop, shifted FFT sequence), full plan ({t_plan:.2f} s) and v4 "
+                        f"solve ({t_solve:.2f} s), v4 evaluation on {len(qs)} of {len(q)} "
+                        f"pixels; sites {t_sites:.2f} s, per-site latency "
+                        f"{t_sites / len(pick):.3f} s; extrapolated total {total:.1f} s"),
+                measured_sites_seconds=t_sites, extrapolated_seconds=total,
+                site_latency_seconds=t_sites / len(pick), **host)
 
 
 def main():
@@ -104,7 +133,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=6, help="surrogate (config 5) epochs")
     ap.add_argument("--accuracy-sites", type=int, default=0,
                     help="FP64-oracle rel L2 on this many sampled sites (config 2/4)")
-    ap.add_argument("--cpu-sites", type=int, default=12)
+    ap.add_argument("--cpu-sites", type=int, default=64)
     ap.add_argument("--cpu-queries", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-overlap-v4", dest="overlap_v4", action="store_false",
@@ -185,6 +214,20 @@ def main():
     P = len(offsets)
     value = P * args.steps / elapsed
     kms = float(np.mean(kernel_ms))
+    # the same steps with the index volume uploaded from pinned host memory inside the plan,
+    # as the reference's plan uploads its raw fields (batched.py:52-60); reported beside value
+    pinned = torch.from_numpy(vol_np).pin_memory()
+    torch.cuda.synchronize()
+    up_ms = []
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        ev0.record()
+        st.volume.copy_(pinned, non_blocking=True)
+        ev1.record()
+        st.run()
+        up_ms.append(ev0.elapsed_time(ev1))
+    torch.cuda.synchronize()
+    elapsed_up = time.perf_counter() - t1
     half = args.precision == "mixed"
     bytes_launch = cfg.bytes_per_site(half) * P
     achieved = bytes_launch / (kms * 1e-3) / 1e9
@@ -224,6 +267,11 @@ def main():
             "timing_boundary": "plan + propagation + aggregation (fixed.py:337-378)",
         },
         "stage_seconds": {k: round(float(np.mean(v)), 6) for k, v in stage_s.items()},
+        "with_volume_upload": {
+            "sites_per_s": P * args.steps / elapsed_up,
+            "ms_per_step": elapsed_up / args.steps * 1e3,
+            "upload_ms": float(np.mean(up_ms)), "bytes": int(vol_np.nbytes),
+            "note": "index volume copied host (pinned) -> HBM at the start of every step"},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
@@ -267,7 +315,8 @@ def bench_uneven(args):
     cfg = CONFIG3.base
     sizes, slices = draw_sites(CONFIG3)
     centres = site_offsets(cfg)
-    vol = torch.as_tensor(index_volume(cfg), device="cuda:0")
+    vol_np = index_volume(cfg)
+    vol = torch.as_tensor(vol_np, device="cuda:0")
     st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
     tuned = st.autotune()
     print(f"[bench] autotune: {st.tune_log}", file=sys.stderr, flush=True)
@@ -299,6 +348,11 @@ def bench_uneven(args):
     kms = float(np.mean(kms))
     bl = float(bytes_per_site(sizes, slices).sum())
     ach = bl / (kms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_uneven(cfg, vol_np, st.windows, sizes, slices,
+                                  st.centres)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
     line = {
         "metric": METRIC, "value": P * args.steps / elapsed, "unit": "sites/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
@@ -313,9 +367,56 @@ def bench_uneven(args):
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "sites_mixed_kernel",
                      "kernel_ms": kms, "algorithmic_bytes_per_launch": bl},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_uneven(cfg, vol, windows, sizes, slices, centres, per_class=3):
+    """The oracle on a stratified sample of the uneven sites: per field side, `per_class`
+    seeded sites (the deepest among them), timed per slice; the per-side rate times each
+    site's slice count gives the extrapolated total, plus the full v4 solve and a sampled
+    evaluation as in cpu_baseline."""
+    import oracle as O
+    from dataclasses import replace
+    from maomi_amd.workloads import zernike_weights
+    cores = cpu_cores()
+    O.set_workers(cores)
+    zw = zernike_weights(cfg)
+    t0 = time.perf_counter()
+    k0dz = 2 * np.pi / cfg.wavelength_um * cfg.dz_um / cfg.voxels_per_slice
+    phase = O.plain_phase(vol, cfg.n0, k0dz, cfg.voxels_per_slice)
+    t_plan = time.perf_counter() - t0
+    rng = np.random.default_rng(cfg.seed + 6)
+    per_slice, sampled = {}, []
+    for n in np.unique(sizes):
+        cls = np.flatnonzero(sizes == n)
+        pick = rng.choice(cls, min(per_class, len(cls)), replace=False)
+        t, z = 0.0, 0
+        for i in pick:
+            sub = replace(cfg, n=int(n), slices=int(slices[i]), focus_um=slices[i] * cfg.dz_um)
+            t1 = time.perf_counter()
+            base, _ = O.plain_beam(sub, zw)
+            O.plain_stage_masks(base, phase, windows[i:i + 1], wvl=np.float32(cfg.wavelength_um * 1e-6),
+                                nr=np.float32(cfg.n0), dx=np.float32(cfg.dx_um * 1e-6),
+                                dz=cfg.dz_um * 1e-6, sizes=[int(n)], slices=[int(slices[i])])
+            t += time.perf_counter() - t1
+            z += int(slices[i])
+        per_slice[int(n)] = t / z
+        sampled.append(f"{len(pick)} of {len(cls)} N={int(n)} sites ({t:.2f} s)")
+    t_sites = float(sum(per_slice[int(n)] * int(s) for n, s in zip(sizes, slices)))
+    from oracle.optics_np import _green
+    p = centres.astype(np.float64)
+    t0 = time.perf_counter()
+    coef = np.linalg.solve(_green(p, p), np.ones(len(p)))
+    t_solve = time.perf_counter() - t0
+    total = t_plan + t_sites + t_solve
+    host = host_description()
+    return dict(value=len(sizes) / total, unit="sites/s", cores=cores, kind="port",
+                sample=(f"stratified: {', '.join(sampled)}, timed per slice and extrapolated "
+                        f"over every site's slice count ({t_sites:.1f} s), phase plan "
+                        f"({t_plan:.2f} s) and v4 solve ({t_solve:.2f} s); {cores} threads"),
+                extrapolated_seconds=total, **host)
 
 
 def bench_reference(args):
