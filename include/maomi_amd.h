@@ -81,7 +81,25 @@ typedef struct maomi_sites_desc {
                                  site of the order list as they finish one (dynamic
                                  scheduling; the launch resets it first).  NULL: block b takes
                                  entries b, b + grid, ... (static striding)                  */
+  const struct maomi_sites_audit* audit;  /* optional in-launch audit (see below), or NULL */
 } maomi_sites_desc;
+
+/* In-launch audit of surrogate predictions (no counterpart in the reference).  Needs a
+ * focal-plane launch (psf_out) with a work counter.  The first n_audit entries of the order
+ * list are the audited sites; as each is propagated, its focal planes are compared with its
+ * prediction (relative L2, FP64 sums) and counted as a miss above tol.  Once every audited site
+ * is done with at most max_fail misses, blocks stop taking unaudited sites: those are left for
+ * the surrogate (done[site] = 0).  With more misses every site is propagated, so the fallback
+ * costs no second launch.  state and done are zeroed by the launch. */
+typedef struct maomi_sites_audit {
+  const float* pred;        /* [psf_nz][n_audit][psf_w][psf_w] predictions of audited sites   */
+  const int32_t* slot;      /* [P] slot (0..n_audit-1) of each audited site, -1 otherwise     */
+  int32_t* state;           /* [2] device counters: audited sites done, misses of tol        */
+  float* rel;               /* [n_audit] relative L2 of each audited site (output)            */
+  int32_t* done;            /* [P] 1 for every propagated site (output)                       */
+  int64_t n_audit, max_fail;
+  double tol;
+} maomi_sites_audit;
 
 int maomi_persistent_sites(const maomi_sites_desc* desc, maomi_stream_t stream);
 /* Workspace bytes for `grid` blocks at largest side nmax. */

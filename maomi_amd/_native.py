@@ -26,8 +26,15 @@ class SitesDesc(ctypes.Structure):
         ("grid", c_int64), ("uniform_n", c_int64), ("block_threads", c_int64),
         ("class_edge", c_int64 * 5), ("class_q", c_int64 * 5), ("class_tw", c_int64 * 5),
         ("psf_out", c_void_p), ("psf_z0", c_int64), ("psf_nz", c_int64), ("psf_c0", c_int64),
-        ("psf_w", c_int64), ("screen", c_void_p), ("queue", c_void_p),
+        ("psf_w", c_int64), ("screen", c_void_p), ("queue", c_void_p), ("audit", c_void_p),
     ]
+
+
+class SitesAudit(ctypes.Structure):
+    """Mirror of maomi_sites_audit."""
+    _fields_ = [("pred", c_void_p), ("slot", c_void_p), ("state", c_void_p), ("rel", c_void_p),
+                ("done", c_void_p), ("n_audit", c_int64), ("max_fail", c_int64),
+                ("tol", c_double)]
 
 
 class SitesChoice(ctypes.Structure):

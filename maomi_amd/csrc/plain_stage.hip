@@ -78,7 +78,26 @@ struct SitesArgs {
   float* psf_out;  // optional [psf_nz][P][psf_w][psf_w] planes |u|^4 around focus
   int psf_z0, psf_nz, psf_c0, psf_w;
   int* queue;      // dynamic work counter, or nullptr for static striding
+  // In-launch audit of surrogate predictions (focal-plane launches with a work counter): the
+  // first n_audit entries of the order list are the audited sites.
+  const float* audit_pred;     // [psf_nz][n_audit][psf_w][psf_w], or nullptr
+  const int32_t* audit_slot;   // [P] slot of an audited site, -1 otherwise
+  int* audit_state;            // [2] audited sites done, audited sites above tol
+  float* audit_rel;            // [n_audit] relative L2 of each audited site
+  int32_t* site_done;          // [P] 1 once a site is propagated
+  int n_audit, audit_max_fail;
+  float audit_tol;
 };
+
+// True once every audited site is done and no more than audit_max_fail of them missed the
+// tolerance: the predictions are trusted and blocks stop taking unaudited sites.
+__device__ __forceinline__ bool audit_passed(const SitesArgs& a) {
+  const int done = __hip_atomic_load(&a.audit_state[0], __ATOMIC_ACQUIRE,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+  const int fail = __hip_atomic_load(&a.audit_state[1], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+  return done == a.n_audit && fail <= a.audit_max_fail;
+}
 
 // The persistent blocks' site loop.  With a work counter each block takes the next entry of
 // the order list as it finishes a site, so sites of unequal cost (mixed field sides and depths)
@@ -92,7 +111,12 @@ __device__ __forceinline__ void for_each_site(const SitesArgs& a, F&& run) {
   }
   __shared__ int next;
   for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    if (threadIdx.x == 0) {
+      int i = atomicAdd(a.queue, 1);
+      // an unaudited site after a passed audit is left to the surrogate
+      if (a.audit_pred && i >= a.n_audit && i < a.P && audit_passed(a)) i = (int)a.P;
+      next = i;
+    }
     __syncthreads();
     const int64_t i = next;
     __syncthreads();  // every thread has read `next` before thread 0 takes the following one
@@ -103,6 +127,38 @@ __device__ __forceinline__ void for_each_site(const SitesArgs& a, F&& run) {
 
 __global__ void queue_reset_kernel(int* queue) {
   if (threadIdx.x == 0) *queue = 0;
+}
+
+// Zeroes the audit counters and the per-site done flags before an audited launch.
+__global__ void audit_reset_kernel(int* state, int32_t* done, int64_t P) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < 2) state[i] = 0;
+  for (int64_t j = i; j < P; j += (int64_t)gridDim.x * blockDim.x) done[j] = 0;
+}
+
+// Relative L2 of an audited site's propagated focal planes against its prediction; counts a
+// miss of the tolerance, then the site as done (release: a block that sees every audited site
+// done also sees every miss).
+template <class SM>
+__device__ void audit_site(const SitesArgs& a, int site, int slot, int tid, int nt, SM& sm) {
+  const int64_t plane = (int64_t)a.psf_w * a.psf_w;
+  double num = 0.0, den = 0.0;
+  for (int64_t e = tid; e < a.psf_nz * plane; e += nt) {
+    const int64_t z = e / plane, k = e - z * plane;
+    const float t = a.psf_out[(z * a.P + site) * plane + k];
+    const float p = a.audit_pred[(z * a.n_audit + slot) * plane + k];
+    num += (double)(p - t) * (p - t);
+    den += (double)t * t;
+  }
+  num = block_sum_double(num, sm.red);
+  den = block_sum_double(den, sm.red);
+  if (tid == 0) {
+    const float rel = den > 0.0 ? (float)sqrt(num / den) : INFINITY;
+    a.audit_rel[slot] = rel;
+    if (!(rel <= a.audit_tol))
+      __hip_atomic_fetch_add(&a.audit_state[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&a.audit_state[0], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 template <int NT, int NMAX>
@@ -386,6 +442,14 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
   }
   const double tot = block_sum_double(esum, sm.red);
   if (tid == 0) a.energy[site] = tot;
+  if constexpr (MODE == kScreens32Psf) {
+    if (a.audit_pred) {
+      // this block's focal-plane stores are visible to it after the barriers above
+      const int slot = a.audit_slot[site];
+      if (slot >= 0) audit_site(a, site, slot, tid, NT, sm);
+      if (tid == 0) a.site_done[site] = 1;
+    }
+  }
   __syncthreads();
 }
 
@@ -547,6 +611,31 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   a.psf_z0 = (int)d->psf_z0; a.psf_nz = (int)d->psf_nz; a.psf_c0 = (int)d->psf_c0;
   a.psf_w = (int)d->psf_w;
   a.queue = d->queue;
+  a.audit_pred = nullptr;
+  a.audit_slot = nullptr;
+  a.audit_state = nullptr;
+  a.audit_rel = nullptr;
+  a.site_done = nullptr;
+  a.n_audit = 0;
+  a.audit_max_fail = 0;
+  a.audit_tol = 0.f;
+  if (const maomi_sites_audit* au = d->audit) {
+    if (!d->psf_out || !d->queue || !au->pred || !au->slot || !au->state || !au->rel ||
+        !au->done || au->n_audit < 1 || au->n_audit > d->P || au->max_fail < 0 ||
+        !(au->tol >= 0.0)) {
+      maomi_set_error("maomi_persistent_sites: an audit needs focal-plane output, a work "
+                      "counter, every audit array and 1 <= n_audit <= P");
+      return MAOMI_EINVAL;
+    }
+    a.audit_pred = au->pred;
+    a.audit_slot = au->slot;
+    a.audit_state = au->state;
+    a.audit_rel = au->rel;
+    a.site_done = au->done;
+    a.n_audit = (int)au->n_audit;
+    a.audit_max_fail = (int)au->max_fail;
+    a.audit_tol = (float)au->tol;
+  }
   if (a.psf_out && (a.psf_nz < 1 || a.psf_w < 1 || a.psf_z0 < 0 || a.psf_c0 < 0 ||
                     (d->uniform_n > 0 && d->psf_c0 + d->psf_w > d->uniform_n) ||
                     d->psf_z0 + d->psf_nz > d->Z + 1)) {
@@ -599,6 +688,12 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   }
   if (a.queue) {  // every argument is checked: nothing is enqueued before this point
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, st, a.queue);
+    MAOMI_CHECK_LAUNCH();
+  }
+  if (a.audit_pred) {
+    const unsigned rb = (unsigned)((d->P + 255) / 256 < 1024 ? (d->P + 255) / 256 : 1024);
+    hipLaunchKernelGGL(audit_reset_kernel, dim3(rb), dim3(256), 0, st, a.audit_state,
+                       a.site_done, (int64_t)d->P);
     MAOMI_CHECK_LAUNCH();
   }
   if (mode == kScreens32Psf) {

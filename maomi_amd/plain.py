@@ -6,6 +6,7 @@ assembly).  Every numerical step runs in libmaomi_amd.so; torch provides
 device memory, streams and the FP64 dense solve of the v4 system.
 """
 
+import ctypes
 import time
 
 import numpy as np
@@ -214,7 +215,7 @@ class PlainStage:
         self.base = base_beam(self.pupil, self.bp, self.tables.edge2d[cfg.n], tw, d)
         return self
 
-    def propagate(self, subset=None, psf=None, tail_hook=None):
+    def propagate(self, subset=None, psf=None, tail_hook=None, order=None, audit=None):
         """All sites (or the indices `subset`) in one persistent launch.
 
         psf = (out, z0, nz, c0, w) also writes |u|^4 of planes z0..z0+nz-1 cropped to
@@ -223,9 +224,15 @@ class PlainStage:
         tail_hook: when the sites do not fill whole rounds of the grid, the launch is issued as
         the whole rounds and then the last partial round, and tail_hook() is called in between
         (work it queues on another stream runs on the CUs that the last round leaves idle).
+
+        order: the processing order of the sites (a permutation of 0..P-1; default natural).
+        audit: with psf, an in-launch audit (maomi_sites_audit): dict of device tensors
+        pred (nz, A, w, w), slot (P,) int32, state (2,) int32, rel (A,) float32, done (P,)
+        int32, and max_fail, tol; the first A entries of `order` are the audited sites.
         """
         cfg, d = self.cfg, self.device
         lib = nat.lib()
+        given = order
         if subset is None:
             site_x, site_y, order = self.site_x, self.site_y, self.order
             P = len(self.offsets)
@@ -235,6 +242,11 @@ class PlainStage:
             site_x = torch.as_tensor(self.offsets[sub, 0].astype(np.int32), device=d)
             site_y = torch.as_tensor(self.offsets[sub, 1].astype(np.int32), device=d)
             order = torch.arange(P, dtype=torch.int32, device=d)
+        if given is not None:
+            order = given if torch.is_tensor(given) else torch.as_tensor(
+                np.asarray(given, np.int32), device=d)
+            if order.dtype != torch.int32 or order.numel() != P:
+                raise ValueError("order must hold one int32 entry per site")
         grid = self.grid or int(lib.maomi_sites_default_grid()) or 256
         grid = min(grid, P)
         stride = cfg.n * cfg.n
@@ -277,11 +289,21 @@ class PlainStage:
                 raise ValueError("psf window outside the propagated planes or field")
             desc.psf_out = out.data_ptr()
             desc.psf_z0, desc.psf_nz, desc.psf_c0, desc.psf_w = z0, nz, c0, w
+        if audit is not None:
+            if psf is None:
+                raise ValueError("an audit needs the focal-plane output (psf=)")
+            au = nat.SitesAudit()
+            for key in ("pred", "slot", "state", "rel", "done"):
+                setattr(au, key, audit[key].data_ptr())
+            au.n_audit, au.max_fail, au.tol = (audit["pred"].shape[1], int(audit["max_fail"]),
+                                               float(audit["tol"]))
+            self._audit = au
+            desc.audit = ctypes.addressof(au)
         self._desc = desc
         self._keep = (site_x, site_y, order)
         stream = nat.stream_handle(d)
         P1 = P - P % grid
-        if tail_hook is not None and psf is None and P > grid and P1 < P:
+        if tail_hook is not None and psf is None and audit is None and P > grid and P1 < P:
             desc.P = P1
             nat.check(lib.maomi_persistent_sites(desc, stream), "persistent sites")
             tail_hook()

@@ -351,13 +351,16 @@ def site_psfs(stage, subset=None):
 
 
 class SurrogateStage:
-    """Surrogate inference with a seeded audit, per-site and whole-run fallback.
+    """Surrogate inference with a seeded audit inside the propagator's launch.
 
-    The audit runs first: the network predicts only the audited sites, the propagator
-    computes them, and their relative L2 decides.  When more than `max_fail_frac` of the
-    audited sites miss `tol`, the propagator computes the remaining sites too (the run costs
-    one propagation of every site plus the audit's inference); otherwise the network
-    predicts the remaining sites and each failed audited site keeps its propagated PSF."""
+    The network predicts the audited sites (a seeded fraction) first.  One persistent
+    focal-plane launch then propagates the sites with the audited ones first in its queue; as
+    each audited site is done its prediction is compared on the device (maomi_sites_audit).
+    Once every audited site is done with at most `max_fail_frac` of them above `tol`, the
+    launch's blocks stop taking unaudited sites and the network predicts those that were left.
+    With more misses the launch simply propagates every site: the fallback costs the audit's
+    inference and no second launch.  Every site the launch propagated keeps its propagated PSF
+    (audited sites included)."""
 
     def __init__(self, net, scale, stage, *, audit_frac=0.05, tol=0.01, max_fail_frac=0.1,
                  seed=0):
@@ -375,34 +378,41 @@ class SurrogateStage:
         k = max(1, int(np.ceil(self.audit_frac * P)))
         audit = np.sort(self.rng.choice(P, k, replace=False))
         rest = np.setdiff1d(np.arange(P), audit)
-        aidx = torch.as_tensor(audit, device=dev)
-        ridx = torch.as_tensor(rest, device=dev)
         x_audit = site_features(st.volume, cfg, st.offsets[audit], cfg.focus)
-        pred_audit = self.net.predict(x_audit) * self.scale
-        true = site_psfs(st, audit)
-        rel = rel_l2_per_site(pred_audit, true)
-        fails = (rel > self.tol).cpu().numpy()
-        t1 = time.perf_counter()
+        pred = (self.net.predict(x_audit) * self.scale).contiguous()
+        slot = np.full(P, -1, np.int32)
+        slot[audit] = np.arange(k, dtype=np.int32)
+        au = {"pred": pred, "slot": torch.as_tensor(slot, device=dev),
+              "state": torch.zeros(2, dtype=torch.int32, device=dev),
+              "rel": torch.empty(k, dtype=torch.float32, device=dev),
+              "done": torch.zeros(P, dtype=torch.int32, device=dev),
+              "max_fail": int(np.floor(self.max_fail_frac * k)), "tol": self.tol}
         z0, nz, c0, w = focal_window(cfg)
         out = torch.empty((nz, P, w, w), dtype=torch.float32, device=dev)
-        out[:, aidx] = torch.where(torch.as_tensor(fails, device=dev)[None, :, None, None], true,
-                                   pred_audit)
-        if fails.mean() > self.max_fail_frac:
-            mode = "fallback_all"
-            if len(rest):
-                out[:, ridx] = site_psfs(st, rest)
-        else:
-            mode = "surrogate"
-            if len(rest):
-                x = site_features(st.volume, cfg, st.offsets[rest], cfg.focus)
-                out[:, ridx] = self.net.predict(x) * self.scale
+        st.propagate(psf=(out, z0, nz, c0, w), order=np.concatenate([audit, rest]), audit=au)
+        state = au["state"].cpu().numpy()
+        done = au["done"].cpu().numpy().astype(bool)
+        rel = au["rel"].cpu().numpy()
+        t1 = time.perf_counter()
+        fails = int(state[1])
+        if int(state[0]) != k:
+            raise RuntimeError(f"audit incomplete: {int(state[0])} of {k} audited sites done")
+        mode = "surrogate" if fails <= au["max_fail"] else "fallback_all"
+        left = np.flatnonzero(~done)
+        if mode == "fallback_all" and len(left):
+            raise RuntimeError("a failed audit must leave every site propagated")
+        if len(left):
+            lidx = torch.as_tensor(left, device=dev)
+            x = site_features(st.volume, cfg, st.offsets[left], cfg.focus)
+            out[:, lidx] = self.net.predict(x) * self.scale
         torch.cuda.synchronize(dev)
         t2 = time.perf_counter()
-        self.timing = {"audit": t1 - t0, "remaining_sites": t2 - t1, "total": t2 - t0}
-        rel_np = rel.cpu().numpy()
-        self.report = {"mode": mode, "audited": int(k), "audit_fail": int(fails.sum()),
-                       "audit_rel_l2_median": float(np.median(rel_np)),
-                       "audit_rel_l2_max": float(rel_np.max()),
-                       "audit_rel_l2_mean": float(rel_np.mean())}
+        self.timing = {"audit_and_propagation": t1 - t0, "predicted_sites": t2 - t1,
+                       "total": t2 - t0}
+        self.report = {"mode": mode, "audited": int(k), "audit_fail": fails,
+                       "propagated_sites": int(done.sum()), "predicted_sites": int(len(left)),
+                       "audit_rel_l2_median": float(np.median(rel)),
+                       "audit_rel_l2_max": float(rel.max()),
+                       "audit_rel_l2_mean": float(rel.mean())}
         self.psfs = out
         return out

@@ -44,9 +44,11 @@ def test_error_reporting_without_gpu_work():
 
 
 def test_sites_descriptor_layout_matches_header():
-    # 20 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding; the complex
-    # screen pointer and the work counter are the last members
-    assert ctypes.sizeof(_native.SitesDesc) == 8 * (20 + 12 + 15)
+    # 21 pointers + 12 int64 scalars + 3 x 5 int64 class tables, no padding; the complex
+    # screen pointer, the work counter and the audit are the last members
+    assert ctypes.sizeof(_native.SitesDesc) == 8 * (21 + 12 + 15)
+    assert _native.SitesDesc.audit.offset == 8 * (20 + 12 + 15)
+    assert ctypes.sizeof(_native.SitesAudit) == 8 * 8
     assert _native.SitesDesc.screen.offset == 8 * (18 + 12 + 15)
     assert _native.SitesDesc.queue.offset == 8 * (19 + 12 + 15)
     assert ctypes.sizeof(_native.SitesChoice) == 32
