@@ -403,6 +403,18 @@ int maomi_plane_bias(const float* in, const float* T, float* out, int64_t C, int
 int maomi_plane_bias_grad(const float* dY, float* gT, int64_t C, int64_t B, int64_t HW,
                           float scale, maomi_stream_t stream);
 int maomi_to_bf16(const float* x, void* y, int64_t n, maomi_stream_t stream);
+/* Implicit-GEMM convolution for inference (BF16 MFMA, FP32 accumulation): X is channel-last
+ * BF16 [B][Hs][Ws][C] (C a multiple of 8), read through a nearest 2x upsampling when
+ * upsample = 1; W is BF16 [Co][k][k][C]; Y = leaky(bias + conv) with negative slope `slope`,
+ * stored channel-last BF16 [B][Ho][Wo][Co] (out_bf16 = 1) or channel-major FP32
+ * [Co][B][Ho][Wo] (out_bf16 = 0).  Stride s, zero padding p. */
+int maomi_conv_bf16_fwd(const void* X, int64_t B, int64_t Hs, int64_t Ws, int64_t C,
+                        int64_t upsample, const void* W, const float* bias, int64_t Co, int64_t k,
+                        int64_t s, int64_t p, float slope, void* Y, int64_t out_bf16,
+                        maomi_stream_t stream);
+/* [C][B][HW] FP32 -> [B][HW][Cp] BF16, channels C..Cp-1 zero. */
+int maomi_to_channel_last_bf16(const float* in, int64_t C, int64_t B, int64_t HW, int64_t Cp,
+                               void* out, maomi_stream_t stream);
 /* nearest 2x upsampling of CB planes of H x W (backward = 2 x 2 sums) */
 int maomi_upsample2(const float* in, float* out, int64_t CB, int64_t H, int64_t W,
                     int64_t backward, maomi_stream_t stream);

@@ -28,6 +28,7 @@ SOURCES = {
     "inputs.hip": ["-ffp-contract=off"],
     "psf2d.hip": ["-ffp-contract=off"],
     "plan.hip": [],
+    "conv_implicit.hip": [],
 }
 
 

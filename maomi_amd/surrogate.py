@@ -255,11 +255,66 @@ class SurrogateNet:
                 H, W = 2 * H, 2 * W
         return f
 
-    def predict(self, x, batch=512):
+    def predict(self, x, batch=4096):
+        """Inference: channel-last BF16 activations through the implicit-GEMM convolution
+        (maomi_conv_bf16_fwd: no im2col buffer, bias and leaky ReLU on the accumulators, the
+        decoder's upsampling folded into the next convolution's reads).  x: (C, P, H, W)."""
         outs = []
         for s in range(0, x.shape[1], batch):
-            outs.append(self.forward(x[:, s:s + batch].contiguous()))
-        return torch.cat(outs, 1)
+            outs.append(self._infer(x[:, s:s + batch].contiguous()))
+        return torch.cat(outs, 1) if len(outs) > 1 else outs[0]
+
+    def _channel_last_weights(self):
+        """Per convolution: BF16 weights [co][ky][kx][c] with c padded to a multiple of 8 (the
+        same BF16 values as the training path's copies), rebuilt when the weights change."""
+        if getattr(self, "_cl_t", None) != self.t:
+            self._cl = []
+            for l in self.layers:
+                if isinstance(l, Conv):
+                    cp = -(-l.cin // 8) * 8
+                    w = l.wb.view(l.cout, l.cin, l.k, l.k).permute(0, 2, 3, 1)
+                    wp = torch.zeros((l.cout, l.k, l.k, cp), dtype=torch.bfloat16,
+                                     device=self.device)
+                    wp[..., :l.cin] = w
+                    self._cl.append(wp.reshape(l.cout, -1).contiguous())
+            self._cl_t = self.t
+        return self._cl
+
+    def _infer(self, x):
+        lib = nat.lib()
+        st = _st(x.device)
+        C, B, H, W = x.shape
+        cp = -(-C // 8) * 8
+        act = torch.empty((B, H, W, cp), dtype=torch.bfloat16, device=x.device)
+        nat.check(lib.maomi_to_channel_last_bf16(nat.ptr(x), C, B, H * W, cp, nat.ptr(act), st),
+                  "channel-last input")
+        weights = iter(self._channel_last_weights())
+        convs = [i for i, l in enumerate(self.layers) if isinstance(l, Conv)]
+        up, out = 0, None
+        for i, l in enumerate(self.layers):
+            if isinstance(l, Up2):
+                up = 1
+                continue
+            if isinstance(l, Template):
+                out = l.forward(out, False)
+                continue
+            last = i == convs[-1]
+            Hi, Wi = H << up, W << up
+            Ho, Wo = (Hi + 2 * l.p - l.k) // l.s + 1, (Wi + 2 * l.p - l.k) // l.s + 1
+            if last:
+                y = torch.empty((l.cout, B, Ho, Wo), dtype=torch.float32, device=x.device)
+            else:
+                if l.cout % 8:
+                    raise ValueError("hidden convolutions need a multiple of 8 channels")
+                y = torch.empty((B, Ho, Wo, l.cout), dtype=torch.bfloat16, device=x.device)
+            nat.check(lib.maomi_conv_bf16_fwd(nat.ptr(act), B, H, W, act.shape[-1], up,
+                                              nat.ptr(next(weights)), nat.ptr(l.b), l.cout, l.k,
+                                              l.s, l.p, float(l.slope), nat.ptr(y), int(not last),
+                                              st), "implicit conv")
+            act, H, W, up = y, Ho, Wo, 0
+            if last:
+                out = y
+        return out
 
 
 def to_cbhw(a):

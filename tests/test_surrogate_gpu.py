@@ -334,3 +334,61 @@ def test_pooled_patch_does_not_determine_the_psf(gpu_lib):
           f"min {min(per_b):.3f}; independent volume (D vs A): median {np.median(per_d):.3f}",
           flush=True)
     assert min(per_b) > 0.05 and np.median(per_b) > 0.2
+
+
+@pytest.mark.parametrize("C,Co,H,k,s,p,up,out_bf16", [
+    (16, 32, 64, 3, 2, 1, 0, 1), (32, 64, 32, 3, 2, 1, 0, 1), (128, 384, 8, 3, 1, 1, 0, 1),
+    (384, 96, 8, 3, 1, 1, 1, 1), (96, 48, 16, 3, 1, 1, 1, 1), (48, 9, 32, 2, 1, 1, 0, 0),
+    (24, 40, 9, 3, 1, 0, 1, 0), (8, 130, 5, 1, 1, 0, 0, 1)])
+def test_implicit_conv_matches_fp64_oracle(gpu_lib, C, Co, H, k, s, p, up, out_bf16):
+    """maomi_conv_bf16_fwd against the FP64 oracle's convolution (nearest 2x upsampling first
+    where asked) of the same BF16 operands: FP32 outputs to 1e-5, BF16 outputs to their own
+    rounding (2^-8 relative)."""
+    from maomi_amd import _native as nat
+    rng = np.random.default_rng(C * Co + H)
+    B, slope = 3, 0.1 if out_bf16 else 1.0
+    x = _bf16_round(rng.standard_normal((B, H, H, C)))            # channel-last
+    w = _bf16_round(rng.standard_normal((Co, k, k, C)) / np.sqrt(C * k * k))
+    b = rng.standard_normal(Co).astype(np.float32)
+    xc = np.ascontiguousarray(np.moveaxis(x, -1, 0))               # (C, B, H, W)
+    if up:
+        xc = S.up2_fwd(xc)
+    wc = np.moveaxis(w, -1, 1).reshape(Co, C * k * k)              # (co, c, ky, kx)
+    ref, _ = S.conv_fwd(xc, wc, b.astype(np.float64), k, s, p, slope)  # (Co, B, Ho, Wo)
+    Ho = ref.shape[2]
+    dev = "cuda"
+    X = torch.as_tensor(x, dtype=torch.bfloat16, device=dev)
+    Wt = torch.as_tensor(w.reshape(Co, -1), dtype=torch.bfloat16, device=dev)
+    bt = torch.as_tensor(b, device=dev)
+    Y = (torch.empty((B, Ho, Ho, Co), dtype=torch.bfloat16, device=dev) if out_bf16 else
+         torch.empty((Co, B, Ho, Ho), dtype=torch.float32, device=dev))
+    print(f"[implicit conv] C={C} Co={Co} H={H} k={k} s={s} p={p} up={up}", flush=True)
+    nat.check(gpu_lib.maomi_conv_bf16_fwd(nat.ptr(X), B, H, H, C, up, nat.ptr(Wt), nat.ptr(bt),
+                                          Co, k, s, p, slope, nat.ptr(Y), out_bf16,
+                                          nat.stream_handle(dev)), "implicit conv")
+    got = Y.float().cpu().numpy().astype(np.float64)
+    if out_bf16:
+        got = np.moveaxis(got, -1, 0)
+        np.testing.assert_allclose(got, ref, rtol=2 ** -8, atol=1e-6)
+    else:
+        assert rel_l2(got, ref) < 1e-5
+
+
+def test_implicit_inference_matches_training_forward(gpu_lib):
+    """SurrogateNet.predict (channel-last implicit convolutions, fused upsampling) against the
+    training path's forward (im2col, GEMM, bias/activation passes) on the default network:
+    the same BF16 weights and activations, FP32 sums in another order."""
+    from maomi_amd.surrogate import SurrogateNet
+    net = SurrogateNet(seed=3)
+    for l in net.layers[:-1]:  # non-trivial biases and output planes
+        if hasattr(l, "b"):
+            l.b.normal_(0, 0.05)
+    net.layers[-1].w.normal_(0, 0.1)
+    x = torch.randn((11, 70, 64, 64), device="cuda")
+    with torch.no_grad():
+        ref = net.forward(x)
+        got = net.predict(x, batch=32)
+    assert got.shape == ref.shape == (9, 70, 33, 33)
+    err = rel_l2(got.cpu().numpy(), ref.cpu().numpy())
+    print(f"implicit vs explicit forward: rel L2 {err:.2e}", flush=True)
+    assert err < 5e-3
