@@ -124,7 +124,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="2", choices=("1", "2", "3", "4", "5", "ref", "inputs"),
+    ap.add_argument("--config", default="2",
+                    choices=("1", "2", "3", "4", "5", "5infer", "ref", "inputs"),
                     help="1-5: the plain-stage workloads and the surrogate; ref: the reference's "
                          "own A/B/C geometry (demo scene) through run_fixed; inputs: raw index "
                          "fields and collection mask of a 256 x 256 x 50 um volume")
@@ -168,6 +169,8 @@ def main():
         return bench_reference(args)
     if args.config == "inputs":
         return bench_inputs(args)
+    if args.config == "5infer":
+        return bench_surrogate_inference(args)
     args.config = int(args.config)
     if args.config == 3:
         return bench_uneven(args)
@@ -566,6 +569,54 @@ def bench_inputs(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_surrogate_inference(args):
+    """The surrogate network's inference alone on the 4225 sites of the plain workload
+    (features from the index volume, then the implicit-GEMM convolutions): random initial
+    weights, since the arithmetic does not depend on their values.  One step = features +
+    inference of every site; the roofline is the BF16 MFMA peak."""
+    import torch
+    from maomi_amd.plain import PlainStage
+    from maomi_amd.surrogate import SurrogateNet, site_features
+    from maomi_amd.workloads import CONFIG2, index_volume, site_offsets
+    dev = torch.device("cuda:0")
+    vol = torch.as_tensor(index_volume(CONFIG2), device=dev)
+    offs = site_offsets(CONFIG2)
+    net = SurrogateNet(seed=0, device=dev)
+    P = len(offs)
+    for _ in range(args.warmup):
+        net.predict(site_features(vol, CONFIG2, offs, CONFIG2.focus))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    inf = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x = site_features(vol, CONFIG2, offs, CONFIG2.focus)
+        ev0.record()
+        net.predict(x)
+        ev1.record()
+        torch.cuda.synchronize()
+        inf.append(ev0.elapsed_time(ev1))
+    elapsed = time.perf_counter() - t0
+    ms = float(np.mean(inf))
+    flops = net.forward_flops(64, 64) * P
+    tf = flops / (ms * 1e-3) / 1e12
+    line = {
+        "metric": "surrogate inference sites/s (4225 sites)", "value": P * args.steps / elapsed,
+        "unit": "sites/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 weights and activations, f32 accumulate",
+        "data": "synthetic (seeded index volume, random initial weights)",
+        "config": {"workload": "surrogate network inference on the 4225 sites of the "
+                               "512x512x60 plain workload", "params": net.num_params()},
+        "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tf / 2500.0, "traffic": None,
+                     "kernel": "network inference (conv_implicit_kernel, every layer)",
+                     "kernel_ms": ms, "algorithmic_flops_per_launch": flops},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_surrogate(args):
     """Learned surrogate on the 4225-site workload: train on propagator PSFs of independent
     volumes with the same statistics, then per step predict every site, audit a seeded 5%
@@ -668,7 +719,7 @@ def bench_surrogate(args):
         },
         "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": tf / 2500.0, "traffic": None,
-                     "kernel": "network inference on all sites (im2col + gemm_bf16_kernel)",
+                     "kernel": "network inference on all sites (conv_implicit_kernel)",
                      "kernel_ms": float(np.mean(inf)) * 1e3,
                      "algorithmic_flops_per_launch": net.forward_flops(64, 64) * P},
         "cpu_baseline": None,

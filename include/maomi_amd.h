@@ -434,8 +434,10 @@ int maomi_adam(float* w, void* wb, const float* g, float* m, float* v, int64_t n
  * [X][Y][Zv] index volume; out[zbins][p][.][.] = depth[p]. */
 int maomi_patch_features(const float* index, int64_t X, int64_t Y, int64_t Zv, const int32_t* wx,
                          const int32_t* wy, const float* depth, int64_t P, int64_t side,
-                         int64_t pool, int64_t zbins, double n0, double inv_std, float* out,
-                         maomi_stream_t stream);
+                         int64_t pool, int64_t zbins, double n0, double inv_std, double* zsum,
+                         float* out, maomi_stream_t stream);
+/* (zsum: zbins * X * Y doubles of workspace: the volume's column sums per z bin, computed in
+ * one pass before the per-site block sums) */
 
 /* ---------------------------------------------------------------------------
  * Local-PSF FFT plan (replaces the array operations of LocalFFTPlan.__call__,

@@ -136,7 +136,7 @@ _SIGS = {
                            c_float, c_float, c_float, c_int64, c_float, c_void_p]),
     "maomi_patch_features": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
                                      c_void_p, c_int64, c_int64, c_int64, c_int64, c_double,
-                                     c_double, c_void_p, c_void_p]),
+                                     c_double, c_void_p, c_void_p, c_void_p]),
     "maomi_psf_stats": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p]),
     "maomi_lfft_gather": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,

@@ -628,10 +628,29 @@ MAOMI_API int maomi_adam(float* w, void* wb, const float* g, float* m, float* v,
 namespace {
 // out[c][p][y][x], c < zbins: mean over the pool x pool x (z-bin) voxels of
 // (n - n0) * inv_std in the site window; channel zbins holds the site's depth scalar.
-__global__ void patch_kernel(const float* __restrict__ index, int64_t X, int64_t Y, int64_t Zv,
+// Column sums of (n - n0) over each z bin: zsum[c][x][y] (FP64), one thread per column, the
+// column's voxels read once in order.
+__global__ void zbin_kernel(const float* __restrict__ index, int64_t XY, int Zv, int zbins,
+                            double n0, double* __restrict__ zsum) {
+  for (int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; col < XY;
+       col += (int64_t)gridDim.x * blockDim.x) {
+    const float* v = index + col * Zv;
+    int z = 0;
+    for (int c = 0; c < zbins; ++c) {
+      const int z1 = (int)((int64_t)(c + 1) * Zv / zbins);
+      double acc = 0.0;
+      for (; z < z1; ++z) acc += (double)v[z] - n0;
+      zsum[(int64_t)c * XY + col] = acc;
+    }
+  }
+}
+
+// out[c][p][yy][xx]: the pool x pool block sum of zsum[c] at the site's window cell, over the
+// block's voxel count and the index std; the depth channel after the bins.
+__global__ void patch_kernel(const double* __restrict__ zsum, int64_t X, int64_t Y, int Zv,
                              const int32_t* __restrict__ wx, const int32_t* __restrict__ wy,
                              const float* __restrict__ depth, int P, int side, int pool, int zbins,
-                             float n0, float inv_std, float* __restrict__ out) {
+                             float inv_std, float* __restrict__ out) {
   const int S = side / pool;
   const int64_t plane = (int64_t)P * S * S;
   const int64_t tot = (int64_t)zbins * plane;
@@ -640,14 +659,11 @@ __global__ void patch_kernel(const float* __restrict__ index, int64_t X, int64_t
     const int xx = (int)(e % S), yy = (int)((e / S) % S);
     const int p = (int)((e / ((int64_t)S * S)) % P), c = (int)(e / plane);
     const int z0 = (int)((int64_t)c * Zv / zbins), z1 = (int)((int64_t)(c + 1) * Zv / zbins);
+    const double* zc = zsum + (int64_t)c * X * Y;
     double acc = 0.0;
     for (int i = 0; i < pool; ++i) {
-      const int64_t gx = wx[p] + (int64_t)yy * pool + i;
-      for (int j = 0; j < pool; ++j) {
-        const int64_t gy = wy[p] + (int64_t)xx * pool + j;
-        const float* col = index + (gx * Y + gy) * Zv;
-        for (int z = z0; z < z1; ++z) acc += (double)col[z] - n0;
-      }
+      const double* row = zc + (wx[p] + (int64_t)yy * pool + i) * Y + wy[p] + (int64_t)xx * pool;
+      for (int j = 0; j < pool; ++j) acc += row[j];
     }
     out[e] = (float)(acc / ((double)pool * pool * (z1 - z0)) * inv_std);
     if (c == 0) out[(int64_t)zbins * plane + (int64_t)p * S * S + (int64_t)yy * S + xx] = depth[p];
@@ -658,16 +674,21 @@ __global__ void patch_kernel(const float* __restrict__ index, int64_t X, int64_t
 MAOMI_API int maomi_patch_features(const float* index, int64_t X, int64_t Y, int64_t Zv,
                                    const int32_t* wx, const int32_t* wy, const float* depth,
                                    int64_t P, int64_t side, int64_t pool, int64_t zbins, double n0,
-                                   double inv_std, float* out, maomi_stream_t stream) {
-  if (!index || !wx || !wy || !depth || !out || P < 1 || side < pool || pool < 1 || zbins < 1 ||
-      zbins > Zv || side % pool) {
+                                   double inv_std, double* zsum, float* out,
+                                   maomi_stream_t stream) {
+  if (!index || !wx || !wy || !depth || !zsum || !out || P < 1 || side < pool || pool < 1 ||
+      zbins < 1 || zbins > Zv || side % pool) {
     maomi_set_error("maomi_patch_features: bad arguments");
     return MAOMI_EINVAL;
   }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(zbin_kernel, dim3(blocks_for(X * Y)), dim3(256), 0, st, index, X * Y,
+                     (int)Zv, (int)zbins, n0, zsum);
+  MAOMI_CHECK_LAUNCH();
   const int64_t S = side / pool;
-  hipLaunchKernelGGL(patch_kernel, dim3(blocks_for(zbins * P * S * S)), dim3(256), 0,
-                     (hipStream_t)stream, index, X, Y, Zv, wx, wy, depth, (int)P, (int)side,
-                     (int)pool, (int)zbins, (float)n0, (float)inv_std, out);
+  hipLaunchKernelGGL(patch_kernel, dim3(blocks_for(zbins * P * S * S)), dim3(256), 0, st, zsum,
+                     X, Y, (int)Zv, wx, wy, depth, (int)P, (int)side, (int)pool, (int)zbins,
+                     (float)inv_std, out);
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

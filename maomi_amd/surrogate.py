@@ -383,10 +383,11 @@ def site_features(volume, cfg, offsets, depth_um, *, pool=PATCH_POOL, zbins=PATC
     out[zbins + 1].copy_(ramp[None, :, None].expand(P, S, S))
     out[zbins + 2].copy_(ramp[None, None, :].expand(P, S, S))
     X, Y, Zv = volume.shape
+    zsum = torch.empty((zbins, X, Y), dtype=torch.float64, device=d)
     nat.check(nat.lib().maomi_patch_features(nat.ptr(volume), X, Y, Zv, nat.ptr(wx), nat.ptr(wy),
                                              nat.ptr(dep), P, cfg.n, pool, zbins, float(cfg.n0),
-                                             1.0 / cfg.dn_std, nat.ptr(out), _st(d)),
-              "patch features")
+                                             1.0 / cfg.dn_std, nat.ptr(zsum), nat.ptr(out),
+                                             _st(d)), "patch features")
     return out
 
 
