@@ -465,6 +465,31 @@ int maomi_lfft_product(void* spec, const void* kspec, void* out, int64_t n, int6
 int maomi_lfft_scatter(const double* full, int64_t B, int64_t Lx, int64_t Ly, int64_t Lz,
                        const int32_t* desc, const double* weights, double* out, int64_t X,
                        int64_t Y, int64_t Zo, int64_t Tx, int64_t Ty, maomi_stream_t stream);
+/* The same gather and scatter with a per-z weight (zweights: Z, resp. Zo doubles, or NULL):
+ * a depth bank's z weights applied on the fly to the source (detection) or to the responses
+ * (excitation), rana/operator.py:127-166. */
+int maomi_lfft_gather_z(const double* source, int64_t X, int64_t Y, int64_t Z, const int32_t* desc,
+                        const double* weights, const double* zweights, double* tiles, int64_t B,
+                        int64_t Lx, int64_t Ly, int64_t Lz, maomi_stream_t stream);
+int maomi_lfft_scatter_z(const double* full, int64_t B, int64_t Lx, int64_t Ly, int64_t Lz,
+                         const int32_t* desc, const double* weights, const double* zweights,
+                         double* out, int64_t X, int64_t Y, int64_t Zo, int64_t Tx, int64_t Ty,
+                         maomi_stream_t stream);
+/* out[i] = sum_f fweights[f] spec[i] kspec[f][i] (nz = 0), or with the sum over z of a plane
+ * projection (nz > 0, z innermost): several fields' spectra (kstride complex elements apart)
+ * against one source spectrum, one inverse transform for all.  fweights NULL: ones. */
+int maomi_lfft_product_fields(const void* spec, const void* kspec, int64_t kstride,
+                              const double* fweights, int64_t F, void* out, int64_t n, int64_t nz,
+                              maomi_stream_t stream);
+/* Node weights of the acquisition operator (rana/operator.py:10-17) on the device: for pixel
+ * centres origin + (i + 1/2) pitch, the smoothstep split between the bracketing nodes;
+ * out: n_nodes x n_pos. */
+int maomi_rana_node_weights(const double* nodes, int64_t n_nodes, double origin, double pitch,
+                            int64_t n_pos, double* out, maomi_stream_t stream);
+/* Depth weights of a DepthPSFBank (rana/operator.py:55-62): each field's linear hat function
+ * over the depth nodes at the positions z (clamped at the ends); out: F x nz. */
+int maomi_rana_depth_weights(const double* depths, int64_t F, const double* z, int64_t nz,
+                             double* out, maomi_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Input generation: spectral Gaussian-process index fields and the collection

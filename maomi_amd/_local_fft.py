@@ -12,6 +12,12 @@ tiles, three HIP kernels of libmaomi_amd.so do the array work:
     (FP64 atomics; neighbouring tiles overlap).
 FP64 rFFTs go through torch.fft (rocFFT).  Kernel spectra are cached per tile batch under
 ``cache_mb``, as in the reference.  The plan runs on a GPU only; ``device='cpu'`` raises.
+
+The kernels are uploaded once, as a device stack with a leading field axis; ``apply`` projects
+through every field of the stack at once (rana.RanaOperator's depth banks): a non-detection plan
+transforms each source tile once for all fields and, for a plane projection with one weight per
+field, combines the fields' spectra before a single inverse transform; per-z weights of a field
+are applied inside the gather (detection) or the scatter (excitation) kernels.
 """
 
 from collections import OrderedDict, defaultdict
@@ -47,10 +53,19 @@ class LocalFFTPlan:
         self.cache_limit = int(cache_mb * 1024 ** 2)
         self.stats = dict(kernel_fft_builds=0, cache_hits=0)
         self.dims = (1, 2) if plane else (1, 2, 3)
-        kernels = np.asarray(kernels)
-        if kernels.ndim != 5:
+        # The kernels live on the device once, as a stack with a leading field axis: a depth
+        # bank's fields share this plan's tiles (rana.RanaOperator passes its whole stack).
+        if isinstance(kernels, torch.Tensor):
+            stack = kernels.to(device=self._dev, dtype=torch.float64)
+            if stack.ndim == 5:
+                stack = stack[None]
+        else:
+            stack = torch.as_tensor(np.ascontiguousarray(kernels, dtype=np.float64),
+                                    device=self._dev)[None]
+        if stack.ndim != 6:
             raise ValueError("kernels must have axes field_X, field_Y, kernel_X, kernel_Y, kernel_Z")
-        kshape = kernels.shape[2:]
+        self._kstack = stack
+        kshape = tuple(stack.shape[3:])
         if len(self.shape) != 3:
             raise ValueError("shape must be (X, Y, Z)")
         if plane and kshape[2] != self.shape[2]:
@@ -98,17 +113,18 @@ class LocalFFTPlan:
             return a.to(device=self._dev, dtype=torch.float64)
         return torch.as_tensor(np.array(a, dtype=np.float64, order="C"), device=self._dev)
 
-    def _kernel_fft(self, entries, lengths, Z):
-        key = (tuple((e["i"], e["j"]) for e in entries), lengths)
+    def _kernel_fft(self, entries, lengths, field=0):
+        key = (field, tuple((e["i"], e["j"]) for e in entries), lengths)
         if key in self.cache:
             self.stats["cache_hits"] += 1
             self.cache.move_to_end(key)
             return self.cache[key]
-        k = np.stack([np.asarray(self.kernels[e["i"], e["j"]], float) for e in entries])
+        ii = torch.as_tensor([e["i"] for e in entries], device=self._dev)
+        jj = torch.as_tensor([e["j"] for e in entries], device=self._dev)
+        k = self._kstack[field][ii, jj]  # gathered on the device: (B, kx, ky, kz)
         if self.correlate:
-            k = np.flip(k, axis=(1, 2) if self.plane else (1, 2, 3))
-        spec = torch.fft.rfftn(torch.as_tensor(np.ascontiguousarray(k), device=self._dev),
-                               s=lengths, dim=self.dims).contiguous()
+            k = torch.flip(k, dims=(1, 2) if self.plane else (1, 2, 3))
+        spec = torch.fft.rfftn(k.contiguous(), s=lengths, dim=self.dims).contiguous()
         size = spec.numel() * spec.element_size()
         self.stats["kernel_fft_builds"] += 1
         if size <= self.cache_limit:
@@ -120,13 +136,32 @@ class LocalFFTPlan:
         return spec
 
     def __call__(self, source):
+        return self.apply(source)
+
+    def apply(self, source, zweights=None, fweights=None, fields=None):
+        """sum_f of the projection of `source` through field f's kernels (every field of the
+        stack; one field for a plain plan).
+
+        zweights (F, Z) device tensor: per-z weights of each field, applied to the source when
+        the plan is a detection plan and to the responses otherwise (3-D output).
+        fweights (F,) device tensor: one weight per field for a non-detection plane projection;
+        the fields' spectra are then combined before one inverse transform.
+        fields: the field indices to project through (default: all)."""
         src = self.array(source).contiguous()
         if tuple(src.shape) != self.shape:
             raise ValueError(f"source shape {tuple(src.shape)} differs from the plan's {self.shape}")
+        F = self._kstack.shape[0]
+        use = list(range(F)) if fields is None else [int(f) for f in fields]
+        if zweights is not None and tuple(zweights.shape) != (F, self.shape[2]):
+            raise ValueError("zweights must be (fields, Z)")
+        if fweights is not None and (self.detection or not self.plane or
+                                     tuple(fweights.shape) != (F,)):
+            raise ValueError("fweights serve non-detection plane projections, one per field")
         X, Y, Z = self.shape
         out = torch.zeros(self.shape[:2] if self.plane else self.shape, dtype=torch.float64,
                           device=self._dev)
         lib, st = nat.lib(), nat.stream_handle(self._dev)
+        shared = not self.detection  # one source transform serves every field
         for (sshape, lengths), group in self.groups.items():
             Lx, Ly = lengths[0], lengths[1]
             Lz = Z if self.plane else lengths[2]
@@ -151,27 +186,45 @@ class LocalFFTPlan:
                         raise AssertionError("tile window outside its padded transform")
                 gdev = torch.as_tensor(gd, device=self._dev)
                 sdev = torch.as_tensor(sd, device=self._dev)
-                tiles = torch.empty((B, Lx, Ly, Lz), dtype=torch.float64, device=self._dev)
-                nat.check(lib.maomi_lfft_gather(nat.ptr(src), X, Y, Z, nat.ptr(gdev),
-                                                nat.ptr(self._weights), nat.ptr(tiles), B, Lx, Ly,
-                                                Lz, st), "local fft gather")
-                spec = torch.fft.rfftn(tiles, dim=self.dims).contiguous()
-                kspec = self._kernel_fft(entries, lengths, Z)
-                if self.plane:
-                    prod = torch.empty(spec.shape[:3], dtype=torch.complex128, device=self._dev)
-                    nat.check(lib.maomi_lfft_product(nat.ptr(spec), nat.ptr(kspec), nat.ptr(prod),
-                                                     prod.numel(), Z, st), "local fft product")
-                    full = torch.fft.irfftn(prod, s=(Lx, Ly), dim=(1, 2)).contiguous()
-                    fz, zo = 1, 1
-                else:
-                    nat.check(lib.maomi_lfft_product(nat.ptr(spec), nat.ptr(kspec), None,
-                                                     spec.numel(), 0, st), "local fft product")
-                    full = torch.fft.irfftn(spec, s=lengths, dim=(1, 2, 3)).contiguous()
-                    fz, zo = Lz, Z
                 Tx, Ty = int(sd[:, 2].max()), int(sd[:, 3].max())
-                nat.check(lib.maomi_lfft_scatter(nat.ptr(full), B, Lx, Ly, fz, nat.ptr(sdev),
-                                                 nat.ptr(self._weights), nat.ptr(out), X, Y, zo,
-                                                 Tx, Ty, st), "local fft scatter")
+                tiles = torch.empty((B, Lx, Ly, Lz), dtype=torch.float64, device=self._dev)
+                fz, zo = (1, 1) if self.plane else (Lz, Z)
+
+                def transform(zw):
+                    nat.check(lib.maomi_lfft_gather_z(nat.ptr(src), X, Y, Z, nat.ptr(gdev),
+                                                      nat.ptr(self._weights), nat.ptr(zw),
+                                                      nat.ptr(tiles), B, Lx, Ly, Lz, st),
+                              "local fft gather")
+                    return torch.fft.rfftn(tiles, dim=self.dims).contiguous()
+
+                def respond(spec, fields, fw, zw):
+                    ks = [self._kernel_fft(entries, lengths, f) for f in fields]
+                    kstack = ks[0] if len(ks) == 1 else torch.stack(ks)
+                    n = int(np.prod(spec.shape[:3])) if self.plane else spec.numel()
+                    prod = torch.empty(spec.shape[:3] if self.plane else spec.shape,
+                                       dtype=torch.complex128, device=self._dev)
+                    nat.check(lib.maomi_lfft_product_fields(
+                        nat.ptr(spec), nat.ptr(kstack), ks[0].numel(), nat.ptr(fw), len(ks),
+                        nat.ptr(prod), n, Z if self.plane else 0, st), "local fft product")
+                    s_ = (Lx, Ly) if self.plane else lengths
+                    full = torch.fft.irfftn(prod, s=s_, dim=(1, 2) if self.plane else (1, 2, 3))
+                    full = full.contiguous()
+                    nat.check(lib.maomi_lfft_scatter_z(
+                        nat.ptr(full), B, Lx, Ly, fz, nat.ptr(sdev), nat.ptr(self._weights),
+                        nat.ptr(zw), nat.ptr(out), X, Y, zo, Tx, Ty, st), "local fft scatter")
+
+                if shared:
+                    spec = transform(None)
+                    if zweights is None:  # every field at once, spectra combined
+                        fw = None if fweights is None else fweights[use].contiguous()
+                        respond(spec, use, fw, None)
+                    else:
+                        for f in use:
+                            respond(spec, [f], None, zweights[f].contiguous())
+                else:
+                    for f in use:
+                        zw = None if zweights is None else zweights[f].contiguous()
+                        respond(transform(zw), [f], None, None)
         return out
 
     def clear(self):

@@ -142,6 +142,18 @@ _SIGS = {
     "maomi_lfft_gather": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
                                   c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p]),
     "maomi_lfft_product": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
+    "maomi_lfft_gather_z": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64,
+                                    c_void_p]),
+    "maomi_lfft_scatter_z": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64,
+                                     c_int64, c_int64, c_void_p]),
+    "maomi_lfft_product_fields": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64,
+                                          c_void_p, c_int64, c_int64, c_void_p]),
+    "maomi_rana_node_weights": (c_int, [c_void_p, c_int64, c_double, c_double, c_int64,
+                                        c_void_p, c_void_p]),
+    "maomi_rana_depth_weights": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p,
+                                         c_void_p]),
     "maomi_lfft_scatter": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p,
                                    c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64,
                                    c_int64, c_void_p]),

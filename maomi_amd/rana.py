@@ -1,14 +1,27 @@
 """Acquisition through a field of local PSFs on the GPU, and the adapter that attaches it to an
 optical-stage result.
 
-``RanaOperator(field, *, native=False, origin_um=(0, 0), z_origin_um=0, device='cuda')``
-keeps the reference's operator (maomi/rana/operator.py:20-170): smooth XY node weights,
-linear focal-depth weights of a DepthPSFBank, ``static(volume)`` and
-``scan_slab(slab, z_start=0, scan_avg=1)``, a bounded cache of LocalFFTPlans, nonnegative
-results.  The projections run on ``maomi_amd._local_fft.LocalFFTPlan`` (HIP kernels plus
-rocFFT, FP64).  ``attach_rana(optics, rana=True, ...)`` follows
-maomi/rana/adapters/naomi.py:9-68: with no field it builds one from the run's own per-site
-PSFs, otherwise it resolves the given field or provider once.
+``RanaOperator(field, *, native=False, origin_um=(0, 0), z_origin_um=0, device='cuda')`` has
+the reference operator's public surface (maomi/rana/operator.py:20-170): ``static(volume)``,
+``scan_slab(slab, z_start=0, scan_avg=1)``, ``clear_cache``, ``cache_info``, ``settings``,
+``physical_tail``, with the same validation texts and settings keys, and the same results
+(smooth node weights in XY, linear weights between the depth fields of a DepthPSFBank,
+nonnegative output).  The work is organised for the GPU:
+  * every field of a depth bank is one device stack of kernels, uploaded once;
+  * node weights (smoothstep between bracketing nodes) and depth weights (hat functions over
+    the depth nodes) are computed by HIP kernels (maomi_rana_node_weights,
+    maomi_rana_depth_weights) and stay on the device, cached per shape and depth set;
+  * one local-FFT plan per (source shape, plane, scan variant) serves every field
+    (LocalFFTPlan.apply): the excitation operators transform each source tile once for all
+    fields, a plane projection with one weight per field combines the fields' spectra before a
+    single inverse transform, and per-z weights are applied inside the gather or scatter
+    kernels instead of on weighted copies of the volume;
+  * the scan kernels (depth-reversed for detection, summed over scan_avg planes) are built on
+    the device from the stack, once per variant.
+One upload of the source and one download of the result per call.
+``attach_rana(optics, rana=True, ...)`` follows maomi/rana/adapters/naomi.py:9-68: with no field
+it builds one from the run's own per-site PSFs, otherwise it resolves the given field or
+provider once.
 """
 
 from collections import OrderedDict
@@ -18,91 +31,109 @@ from collections.abc import Mapping
 import numpy as np
 import torch
 
+from . import _native as nat
+from ._local_fft import LocalFFTPlan
 from .fields import DepthPSFBank, FullFieldPSF, resolve_field
 from .providers import native_field
 
-
-def axis_weights(nodes, positions):
-    """Each position weighs its two bracketing nodes by a smoothstep of its fraction between
-    them (operator.py:10-17); the weights of a position sum to one."""
-    nodes = np.asarray(nodes, float)
-    positions = np.asarray(positions, float)
-    k = np.clip(np.searchsorted(nodes, positions, side="right") - 1, 0, len(nodes) - 2)
-    f = np.clip((positions - nodes[k]) / (nodes[k + 1] - nodes[k]), 0.0, 1.0)
-    f = f * f * (3.0 - 2.0 * f)
-    w = np.zeros((len(nodes), len(positions)))
-    cols = np.arange(len(positions))
-    w[k, cols] = 1.0 - f
-    w[k + 1, cols] = f
-    return w
+_PLAN_LIMIT = 4  # prepared plans kept (static and scan variants of the shapes in use)
 
 
 class RanaOperator:
     def __init__(self, field, *, native=False, origin_um=(0.0, 0.0), z_origin_um=0.0,
                  device="cuda"):
         field = resolve_field(field)
-        self.bank = field if isinstance(field, DepthPSFBank) else None
-        self.fields = field.fields if self.bank is not None else (field,)
-        if any(not isinstance(f, FullFieldPSF) or f.kernels.ndim != 5 for f in self.fields):
+        bank = field if isinstance(field, DepthPSFBank) else None
+        fields = field.fields if bank is not None else (field,)
+        if any(not isinstance(f, FullFieldPSF) or f.kernels.ndim != 5 for f in fields):
             raise TypeError("The acquisition add-on requires complete XYZ-kernel fields")
         if str(device) == "cpu":
             raise ValueError("this RanaOperator runs on the GPU: pass device='cuda[:index]'")
-        self.field = self.fields[0]
+        origin = np.asarray(origin_um, float)
+        if origin.shape != (2,) or not np.isfinite(origin).all() or not np.isfinite(z_origin_um):
+            raise ValueError("Supply finite XY and Z origins")
+        self.bank, self.fields, self.field = bank, fields, fields[0]
         self.kernel_shape = self.field.kernels.shape[2:]
         self.voxel_size_um = self.field.voxel_size_um
-        self.origin_um = np.asarray(origin_um, float)
-        self.z_origin_um = float(z_origin_um)
+        self.origin_um, self.z_origin_um = origin, float(z_origin_um)
         self.native, self.device = bool(native), str(device)
-        if (self.origin_um.shape != (2,) or not np.isfinite(self.origin_um).all()
-                or not np.isfinite(self.z_origin_um)):
-            raise ValueError("Supply finite XY and Z origins")
         self._dev = torch.device(self.device)
-        self._weights_cache = {}
+        op = self.field.operator
+        self._detection = not self.native and op == "detection"
+        self._correlate = not self.native and op == "excitation"
+        # the depth fields as one device stack (F, nx, ny, kx, ky, kz)
+        self._stack = torch.as_tensor(np.stack([f.kernels for f in fields]), device=self._dev)
+        self._depth_nodes = (None if bank is None else
+                             torch.as_tensor(bank.depths_um, dtype=torch.float64,
+                                             device=self._dev))
+        self._node_w = {}        # XY shape -> host (wx, wy) from the device kernel
+        self._depth_w = {}       # z positions -> (F, nz) device weights
+        self._scan_stacks = {}   # (scan_avg, detection) -> scan kernel stack
         self._plans = OrderedDict()
-        self._scan_kernels = {}
         self.settings = {
-            "enabled": True, "operator": "native_scan" if native else self.field.operator,
+            "enabled": True, "operator": "native_scan" if native else op,
             "interpolation": "positive local cubic tensor weights",
             "field_nodes": list(self.field.kernels.shape[:2]),
             "kernel_shape": list(self.kernel_shape),
-            "depth_nodes_um": None if self.bank is None else self.bank.depths_um.tolist(),
+            "depth_nodes_um": None if bank is None else bank.depths_um.tolist(),
             "source_masks": "existing excitation and collection masks retained",
-            "native_gain": "source excitation mask" if native else "custom kernel gain plus source masks",
+            "native_gain": ("source excitation mask" if native
+                            else "custom kernel gain plus source masks"),
             "tail_resampling": "nonnegative intensity projection, integrated gain preserved",
             "shared_psf_fallback": False, "device": self.device,
-            "provenance": [f.provenance for f in self.fields]}
+            "provenance": [f.provenance for f in fields]}
 
-    # ------------------------------------------------------------ weights
-    def _weights(self, shape):
-        shape = tuple(shape[:2])
-        if shape not in self._weights_cache:
+    # ------------------------------------------------------------ weights on the device
+    def _xy_weights(self, shape):
+        """Node weights of an X x Y acquisition grid (pixel centres from origin_um), computed
+        on the device; the plans take a host copy once per shape for their tile layout."""
+        shape = tuple(int(n) for n in shape[:2])
+        if shape not in self._node_w:
             pitch = np.array(self.voxel_size_um[:2])
             lo = np.array([self.field.x_um[0], self.field.y_um[0]])
             hi = np.array([self.field.x_um[-1], self.field.y_um[-1]])
-            if ((self.origin_um < lo - 1e-10).any()
-                    or (self.origin_um + np.array(shape) * pitch > hi + 1e-10).any()):
+            if (self.origin_um < lo - 1e-10).any() or \
+                    (self.origin_um + np.array(shape) * pitch > hi + 1e-10).any():
                 raise ValueError("RANA field must cover the entire acquisition FOV")
-            centres = [o + (np.arange(n) + 0.5) * p for o, n, p in zip(self.origin_um, shape, pitch)]
-            self._weights_cache[shape] = tuple(
-                axis_weights(nodes, c) for nodes, c in zip((self.field.x_um, self.field.y_um),
-                                                           centres))
-        return self._weights_cache[shape]
+            lib, st = nat.lib(), nat.stream_handle(self._dev)
+            out = []
+            for nodes, o, step, n in zip((self.field.x_um, self.field.y_um), self.origin_um,
+                                         pitch, shape):
+                nd = torch.as_tensor(nodes, dtype=torch.float64, device=self._dev)
+                w = torch.empty((len(nodes), n), dtype=torch.float64, device=self._dev)
+                nat.check(lib.maomi_rana_node_weights(nat.ptr(nd), len(nodes), float(o),
+                                                      float(step), n, nat.ptr(w), st),
+                          "node weights")
+                out.append(w)
+            self._node_w[shape] = tuple(w.cpu().numpy() for w in out)
+        return self._node_w[shape]
 
-    def _depth_weights(self, z):
+    def _z_weights(self, z):
+        """(F, nz) device weights of the depth fields at positions z; None for one field."""
+        z = np.atleast_1d(np.asarray(z, float))
         if self.bank is None:
-            return np.ones((1, np.size(z)))
-        z = np.atleast_1d(z)
+            return None
         d = self.bank.depths_um
         if (z < d[0] - 1e-10).any() or (z > d[-1] + 1e-10).any():
             raise ValueError("RANA depth bank does not cover the requested acquisition depths")
-        return np.stack([np.interp(z, d, row) for row in np.eye(len(self.fields))])
+        key = z.tobytes()
+        if key not in self._depth_w:
+            zt = torch.as_tensor(z, device=self._dev)
+            w = torch.empty((len(self.fields), len(z)), dtype=torch.float64, device=self._dev)
+            nat.check(nat.lib().maomi_rana_depth_weights(nat.ptr(self._depth_nodes),
+                                                         len(self.fields), nat.ptr(zt), len(z),
+                                                         nat.ptr(w), nat.stream_handle(self._dev)),
+                      "depth weights")
+            self._depth_w[key] = w
+        return self._depth_w[key]
 
     # ------------------------------------------------------------ plans
     def clear_cache(self):
         """Release prepared kernel FFTs and device-side interpolation weights."""
         self._plans.clear()
-        self._scan_kernels.clear()
-        self._weights_cache.clear()
+        self._scan_stacks.clear()
+        self._node_w.clear()
+        self._depth_w.clear()
 
     @property
     def cache_info(self):
@@ -111,24 +142,34 @@ class RanaOperator:
                     fft_bytes=sum(p.cache_bytes for p in self._plans.values()),
                     **{k: sum(p.stats[k] for p in self._plans.values()) for k in names})
 
-    def _project(self, source, kernels, *, plane=False):
-        from ._local_fft import LocalFFTPlan
-        key = (tuple(source.shape), id(kernels), plane)
+    def _plan(self, shape, stack, variant, plane):
+        key = (tuple(shape), variant, plane)
         if key not in self._plans:
-            limit = max(2, 2 * len(self.fields))
-            while len(self._plans) >= limit:
+            while len(self._plans) >= _PLAN_LIMIT:
                 self._plans.popitem(last=False)[1].clear()
-            op = self.field.operator
-            self._plans[key] = LocalFFTPlan(
-                source.shape, kernels, self._weights(source.shape), plane=plane,
-                detection=not self.native and op == "detection",
-                correlate=not self.native and op == "excitation", device=self.device,
-                cache_mb=256 / limit)
+            self._plans[key] = LocalFFTPlan(shape, stack, self._xy_weights(shape), plane=plane,
+                                            detection=self._detection,
+                                            correlate=self._correlate, device=self.device,
+                                            cache_mb=256 / _PLAN_LIMIT)
         self._plans.move_to_end(key)
-        return self._plans[key](source)
+        return self._plans[key]
 
-    def _array(self, a):
-        return torch.as_tensor(np.asarray(a, np.float64), device=self._dev)
+    def _scan_stack(self, Z, scan_avg):
+        """Kernels of a scan: depth reversed about the centre for detection (zero past the
+        end), then summed over groups of scan_avg planes."""
+        key = (scan_avg, self._detection)
+        if key not in self._scan_stacks:
+            k = self._stack
+            if self._detection:
+                centre = self.kernel_shape[2] // 2
+                ids = 2 * centre - np.arange(Z)
+                src = torch.as_tensor(np.minimum(ids, Z - 1), device=self._dev)
+                k = k.index_select(-1, src)
+                k[..., torch.as_tensor(ids >= Z, device=self._dev)] = 0.0
+            if scan_avg > 1:
+                k = _group_sum(k, scan_avg)
+            self._scan_stacks[key] = k.contiguous()
+        return self._scan_stacks[key]
 
     @staticmethod
     def _finish(result):
@@ -156,22 +197,13 @@ class RanaOperator:
 
     # ------------------------------------------------------------ acquisition
     def static(self, volume):
-        """Static XYZ image of a fluorescence volume through the local kernels."""
+        """Static XYZ image of a fluorescence volume through the local kernels: every depth
+        field in one plan, each weighted by its depth weight along z."""
         a = self._density(volume)
         z = self.z_origin_um + (np.arange(a.shape[2]) + 0.5) * self.voxel_size_um[2]
-        depth = self._depth_weights(z)
-        src = self._array(a)
-        out = torch.zeros_like(src)
-        detection = not self.native and self.field.operator == "detection"
-        for f, w in zip(self.fields, depth):
-            if not np.any(w):
-                continue
-            wd = self._array(w)
-            if detection:
-                out += self._project(src * wd, f.kernels)
-            else:
-                out += self._project(src, f.kernels) * wd
-        return self._finish(out)
+        zw = self._z_weights(z)
+        plan = self._plan(a.shape, self._stack, "static", False)
+        return self._finish(plan.apply(torch.as_tensor(a, device=self._dev), zweights=zw))
 
     def scan_slab(self, slab, *, z_start=0, scan_avg=1):
         """One scanned plane of a focal slab (depth equal to the kernel depth)."""
@@ -181,32 +213,37 @@ class RanaOperator:
         if not isinstance(scan_avg, (int, np.integer)) or scan_avg < 1:
             raise ValueError("scan_avg must be a positive integer")
         Z = a.shape[2]
-        centre = self.kernel_shape[2] // 2
         z = self.z_origin_um + (z_start + np.arange(Z) + 0.5) * self.voxel_size_um[2]
-        detection = not self.native and self.field.operator == "detection"
-        depth = self._depth_weights(z if detection else [z[centre]])
-        starts = np.arange(0, Z, scan_avg) if scan_avg > 1 else None
-        src = self._array(a)
-        out = torch.zeros(a.shape[:2], dtype=torch.float64, device=self._dev)
-        for index, (f, w) in enumerate(zip(self.fields, depth)):
-            if not np.any(w):
-                continue
-            s = src * self._array(w) if detection else src
-            key = (index, scan_avg, detection)
-            if key not in self._scan_kernels:
-                k = f.kernels
-                if detection:  # kernel depth reversed about its centre, zero past the end
-                    ids = 2 * centre - np.arange(Z)
-                    k = np.take(k, np.minimum(ids, Z - 1), axis=-1).copy()
-                    k[..., ids >= Z] = 0
-                if starts is not None:
-                    k = np.add.reduceat(k, starts, axis=-1)
-                self._scan_kernels[key] = k
-            if starts is not None:
-                s = torch.stack([s[..., b:b + scan_avg].sum(-1) for b in starts], dim=-1)
-            proj = self._project(s.contiguous(), self._scan_kernels[key], plane=True)
-            out += proj if detection else proj * float(w[0])
-        return self._finish(out)
+        stack = self._scan_stack(Z, int(scan_avg))
+        src = torch.as_tensor(a, device=self._dev)
+        if self._detection:
+            # per-z depth weights act on the source, before any grouping of planes
+            zw = self._z_weights(z)
+            if scan_avg > 1:
+                F = len(self.fields)
+                w = torch.ones((F, Z), dtype=torch.float64, device=self._dev) if zw is None else zw
+                srcs = torch.stack([_group_sum(src * w[f], scan_avg) for f in range(F)])
+                plan = self._plan(tuple(srcs.shape[1:]), stack, ("scan", scan_avg), True)
+                out = sum(plan.apply(srcs[f], fields=[f]) for f in range(F))
+                return self._finish(out)
+            plan = self._plan(a.shape, stack, ("scan", scan_avg), True)
+            return self._finish(plan.apply(src, zweights=zw))
+        # excitation / native: one weight per field, at the slab's centre plane
+        centre = self.kernel_shape[2] // 2
+        zw = self._z_weights([z[centre]])
+        s = _group_sum(src, scan_avg) if scan_avg > 1 else src
+        plan = self._plan(tuple(s.shape), stack, ("scan", scan_avg), True)
+        fw = None if zw is None else zw[:, 0].contiguous()
+        return self._finish(plan.apply(s.contiguous(), fweights=fw))
+
+
+def _group_sum(a, n):
+    """Sum of consecutive groups of n planes along the last axis (the last group may be
+    shorter), on the device."""
+    Z = a.shape[-1]
+    groups = torch.as_tensor(np.arange(Z) // n, device=a.device)
+    out = torch.zeros(a.shape[:-1] + (-(-Z // n),), dtype=a.dtype, device=a.device)
+    return out.index_add_(a.ndim - 1, groups, a)
 
 
 @dataclass(frozen=True)

@@ -75,3 +75,30 @@ def test_attach_native_field_renders_image(gpu_lib):
                    (0.5, 0.5, 1.0), "excitation", native=True)
     assert rel_l2(out["image"], ref.static(vol * (mask * col)[..., None])) < 1e-12
     assert attach_rana(optics, False) is optics
+
+
+def test_device_weights_match_oracle(gpu_lib):
+    """maomi_rana_node_weights against the oracle's smoothstep node weights (operator.py:10-17)
+    and maomi_rana_depth_weights against numpy.interp of the identity rows (operator.py:55-62),
+    including positions on nodes, between them and clamped past the ends."""
+    import torch
+    from maomi_amd import _native as nat
+    from oracle.local_fft_np import axis_weights
+    dev = torch.device("cuda")
+    st = nat.stream_handle(dev)
+    nodes = np.array([-1.0, 0.5, 3.0, 3.25, 9.0])
+    for origin, pitch, n in ((-1.0, 0.25, 40), (-3.0, 0.7, 25), (0.5, 1.0, 9)):
+        nd = torch.as_tensor(nodes, device=dev)
+        w = torch.empty((len(nodes), n), dtype=torch.float64, device=dev)
+        nat.check(gpu_lib.maomi_rana_node_weights(nat.ptr(nd), len(nodes), origin, pitch, n,
+                                                  nat.ptr(w), st), "node weights")
+        want = axis_weights(nodes, origin + (np.arange(n) + 0.5) * pitch)
+        np.testing.assert_allclose(w.cpu().numpy(), want, rtol=0, atol=1e-15)
+    depths = np.array([0.0, 2.0, 6.0, 6.5])
+    z = np.array([-1.0, 0.0, 0.5, 2.0, 3.9, 6.0, 6.2, 6.5, 9.0])
+    dt, zt = torch.as_tensor(depths, device=dev), torch.as_tensor(z, device=dev)
+    w = torch.empty((4, len(z)), dtype=torch.float64, device=dev)
+    nat.check(gpu_lib.maomi_rana_depth_weights(nat.ptr(dt), 4, nat.ptr(zt), len(z), nat.ptr(w),
+                                               st), "depth weights")
+    want = np.stack([np.interp(z, depths, row) for row in np.eye(4)])
+    np.testing.assert_allclose(w.cpu().numpy(), want, rtol=0, atol=1e-15)
