@@ -188,3 +188,59 @@ def test_deferred_psf_copy_equals_synchronous(gpu_lib):
     res = run_fixed(cfg, None, raw_fields=raw, count_per_axis=5, render_image=False)
     assert isinstance(res["psf_3d"], np.ndarray) and res["psf_3d"].shape == ref["psfs"].shape
     assert np.isfinite(res["psf_3d"]).all() and np.isfinite(res["mask"]).all()
+
+
+def _demo_vessel_scene():
+    """A demo scene of the reference's kind (16 x 16 x 10 um, 20 um deep, 2 voxels/um, 10 um
+    tail): two Gaussian fluorescent blobs and one slanted cylindrical vessel through the raw
+    field's depth."""
+    from maomi_amd.fixed import field_shapes
+    from maomi_amd.optics import OpticalConfig
+    cfg = OpticalConfig(volume_um=(16.0, 16.0, 10.0), depth_um=20.0, vres=2.0, tail_um=10.0)
+    x, y, z = np.meshgrid(np.arange(32), np.arange(32), np.arange(20), indexing="ij")
+    volume = (np.exp(-((x - 11) ** 2 + (y - 15) ** 2 + (z - 9) ** 2) / 9)
+              + 0.7 * np.exp(-((x - 23) ** 2 + (y - 21) ** 2 + (z - 12) ** 2) / 8))
+    shape = field_shapes(cfg)[0][:2]
+    depth = round((cfg.depth_um + cfg.volume_um[2] / 2 + cfg.psf_um[2] / 2 + cfg.tail_um)
+                  * cfg.vres)
+    xx, yy = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    vessels = np.stack([(xx - (shape[0] / 2 - 14 + 0.22 * iz)) ** 2 + (yy - shape[1] / 2) ** 2
+                        < 64 for iz in range(depth)], -1)
+    return cfg, volume.astype(np.float32), vessels
+
+
+def test_run_fixed_4225_sites_full_pipeline(gpu_lib):
+    """The reference's full-pipeline checks (tests/test_fixed.py:56-116) at its default 65 x 65
+    = 4225 sites: every descriptor present and finite, positive tail weights, tail masks of mean
+    1, the collection mask applied, the image equal to scipy's FFT convolution of the masked
+    volume with the mean PSF (MATLAB crop) to 2e-6, and a focal-only run giving the same image
+    bit for bit."""
+    from scipy.signal import fftconvolve
+    from maomi_amd.fixed import run_fixed
+    from maomi_amd.optics import prepare_fields
+    cfg, volume, vessels = _demo_vessel_scene()
+    print("[run_fixed] demo vessel scene, 4225 sites", flush=True)
+    raw = prepare_fields(cfg, vessels)
+    out = run_fixed(cfg, volume, vessels=vessels, raw_fields=raw)
+    assert out["unique_optical_paths"] == 4225 and out["mask_mode"] == "matlab_v4"
+    assert out["settings"]["count_per_axis"] == 65 and out["tails_computed"]
+    for key in ("psf", "psf_raw", "psf_3d", "mask", "mask_raw", "colmask", "psfT", "psfB",
+                "points_um", "image", "energies"):
+        assert key in out, key
+    for tail in ("psfT", "psfB"):
+        for name in ("convmask", "weight", "psfZ", "mask", "mask_raw", "weights_per_point",
+                     "profiles_per_point"):
+            assert np.isfinite(out[tail][name]).all(), (tail, name)
+        assert out[tail]["weight"] > 0
+        assert abs(out[tail]["mask"].mean() - 1) < 1e-12
+    assert out["colmask"].min() < 0.99
+    source = volume * out["mask"][..., None] * out["colmask"][..., None]
+    p = out["psf"]
+    full = fftconvolve(source.astype(np.float64), p.astype(np.float64), mode="full")
+    ref = full[tuple(slice(k // 2, k // 2 + n) for k, n in zip(p.shape, source.shape))]
+    err = np.linalg.norm(out["image"] - ref) / np.linalg.norm(ref)
+    print(f"image vs fftconvolve: {err:.2e}", flush=True)
+    assert err < 2e-6
+    focal = run_fixed(cfg, volume, vessels=vessels, raw_fields=raw, tails=False)
+    assert not focal["tails_computed"] and "psfT" not in focal
+    np.testing.assert_allclose(focal["image"], out["image"], rtol=0, atol=0)
