@@ -85,9 +85,9 @@ typedef struct maomi_sites_desc {
 } maomi_sites_desc;
 
 /* In-launch audit of surrogate predictions (no counterpart in the reference).  Needs a
- * focal-plane launch (psf_out) with a work counter.  The first n_audit entries of the order
- * list are the audited sites; as each is propagated, its focal planes are compared with its
- * prediction (relative L2, FP64 sums) and counted as a miss above tol.  Once every audited site
+ * focal-plane launch (psf_out), with or without a work counter.  The first n_audit entries of
+ * the order list are the audited sites; as each is propagated, its focal planes are compared
+ * with its prediction (relative L2, FP64 sums) and counted as a miss above tol.  Once every audited site
  * is done with at most max_fail misses, blocks stop taking unaudited sites: those are left for
  * the surrogate (done[site] = 0).  With more misses every site is propagated, so the fallback
  * costs no second launch.  state and done are zeroed by the launch. */
@@ -109,9 +109,10 @@ int64_t maomi_sites_default_grid(void);
 
 /* Plan object of the sites launch.  Replaces the captured batches of OpticalPlan.prepare_graph
  * / evaluate_graph (maomi/backends/cuda/batched.py:103-146): one HIP graph holds the whole
- * run's launch (every site, mixed field sides and depths), with dynamic site scheduling, and
+ * run's launch (every site, mixed field sides and depths), and
  * the autotuner picks the grid (sites in flight), the block shape (256 or 512 threads: 16- or
- * 32-line LDS tiles of the line transforms) and the site ordering by timing captured
+ * 32-line LDS tiles of the line transforms), the site ordering and the schedule (static
+ * striding of the order list over the blocks, or the work counter) by timing captured
  * candidates.  The plan owns the orderings, the per-block workspace, the work counter and the
  * hipGraphExec; the descriptor's other buffers stay the caller's and are read at each replay
  * (its order, workspace, scratch_stride, grid, block_threads and queue are the plan's).
@@ -122,21 +123,24 @@ typedef struct maomi_sites_plan_t* maomi_sites_plan;
 enum { MAOMI_ORDER_NATURAL = 0,  /* the caller's site order                                  */
        MAOMI_ORDER_LPT = 1,      /* longest first by S N^2 log2 N                            */
        MAOMI_ORDER_CLASS = 2 };  /* field side, then depth, both descending                  */
+enum { MAOMI_SCHEDULE_STATIC = 0,  /* block b takes order entries b, b + grid, ...        */
+       MAOMI_SCHEDULE_QUEUE = 1 };  /* each block takes the next entry from the counter    */
 typedef struct maomi_sites_choice {
-  int64_t grid, block_threads, order;
+  int64_t grid, block_threads, order, schedule;
   double ms;                     /* best timed replay, or -1 for an untimed capture          */
 } maomi_sites_choice;
 int maomi_sites_plan_create(const maomi_sites_desc* desc, maomi_sites_plan* plan);
 /* Capture one given choice without timing. */
 int maomi_sites_plan_capture(maomi_sites_plan plan, int64_t grid, int64_t block_threads,
-                             int64_t order);
-/* Capture and time every (grid, threads, order) candidate (reps timed replays each, after one
- * warm-up) on `stream` and keep the fastest; results (optional) receives every candidate in
- * grid-major order, best the winner. */
+                             int64_t order, int64_t schedule);
+/* Capture and time every (grid, threads, order, schedule) candidate (reps timed replays each,
+ * after one warm-up) on `stream` and keep the fastest; results (optional) receives every
+ * candidate in grid-major order (schedule innermost), best the winner. */
 int maomi_sites_plan_tune(maomi_sites_plan plan, const int64_t* grids, int n_grids,
                           const int64_t* threads, int n_threads, const int64_t* orders,
-                          int n_orders, int reps, maomi_stream_t stream,
-                          maomi_sites_choice* results, maomi_sites_choice* best);
+                          int n_orders, const int64_t* schedules, int n_schedules, int reps,
+                          maomi_stream_t stream, maomi_sites_choice* results,
+                          maomi_sites_choice* best);
 int maomi_sites_plan_replay(maomi_sites_plan plan, maomi_stream_t stream);
 int maomi_sites_plan_choice(maomi_sites_plan plan, maomi_sites_choice* out);
 int maomi_sites_plan_destroy(maomi_sites_plan plan);

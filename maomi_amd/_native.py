@@ -40,10 +40,11 @@ class SitesAudit(ctypes.Structure):
 class SitesChoice(ctypes.Structure):
     """Mirror of maomi_sites_choice."""
     _fields_ = [("grid", c_int64), ("block_threads", c_int64), ("order", c_int64),
-                ("ms", c_double)]
+                ("schedule", c_int64), ("ms", c_double)]
 
 
 ORDERS = {"natural": 0, "lpt": 1, "class_interleaved": 2}
+SCHEDULES = {"static": 0, "queue": 1}
 
 
 class PupilParams(ctypes.Structure):
@@ -88,10 +89,11 @@ _SIGS = {
     "maomi_sites_workspace_bytes": (c_int64, [c_int64, c_int64]),
     "maomi_sites_default_grid": (c_int64, []),
     "maomi_sites_plan_create": (c_int, [ctypes.POINTER(SitesDesc), ctypes.POINTER(c_void_p)]),
-    "maomi_sites_plan_capture": (c_int, [c_void_p, c_int64, c_int64, c_int64]),
+    "maomi_sites_plan_capture": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64]),
     "maomi_sites_plan_tune": (c_int, [c_void_p, ctypes.POINTER(c_int64), c_int,
                                       ctypes.POINTER(c_int64), c_int, ctypes.POINTER(c_int64),
-                                      c_int, c_int, c_void_p, ctypes.POINTER(SitesChoice),
+                                      c_int, ctypes.POINTER(c_int64), c_int, c_int, c_void_p,
+                                      ctypes.POINTER(SitesChoice),
                                       ctypes.POINTER(SitesChoice)]),
     "maomi_sites_plan_replay": (c_int, [c_void_p, c_void_p]),
     "maomi_sites_plan_choice": (c_int, [c_void_p, ctypes.POINTER(SitesChoice)]),

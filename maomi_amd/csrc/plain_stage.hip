@@ -78,8 +78,8 @@ struct SitesArgs {
   float* psf_out;  // optional [psf_nz][P][psf_w][psf_w] planes |u|^4 around focus
   int psf_z0, psf_nz, psf_c0, psf_w;
   int* queue;      // dynamic work counter, or nullptr for static striding
-  // In-launch audit of surrogate predictions (focal-plane launches with a work counter): the
-  // first n_audit entries of the order list are the audited sites.
+  // In-launch audit of surrogate predictions (focal-plane launches): the first n_audit
+  // entries of the order list are the audited sites.
   const float* audit_pred;     // [psf_nz][n_audit][psf_w][psf_w], or nullptr
   const int32_t* audit_slot;   // [P] slot of an audited site, -1 otherwise
   int* audit_state;            // [2] audited sites done, audited sites above tol
@@ -99,30 +99,25 @@ __device__ __forceinline__ bool audit_passed(const SitesArgs& a) {
   return done == a.n_audit && fail <= a.audit_max_fail;
 }
 
-// The persistent blocks' site loop.  With a work counter each block takes the next entry of
-// the order list as it finishes a site, so sites of unequal cost (mixed field sides and depths)
-// balance over the blocks as greedy list scheduling does; every block leaves once the counter
-// passes P.  Without one, block b takes entries b, b + grid, ...
-template <class F>
-__device__ __forceinline__ void for_each_site(const SitesArgs& a, F&& run) {
-  if (!a.queue) {
-    for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run(a.order[i]);
-    return;
+// Site loop of the uniform kernels' dynamic form: take_site returns the index into the order
+// list of the block's next site, or a value >= P when the block is done.  With a work counter
+// each block takes the next entry as it finishes a site (greedy list scheduling); every block
+// leaves once the counter passes P.  Without one, block b takes entries b, b + grid, ...
+// Either way, an audited launch stops taking unaudited sites once the audit has passed.
+__device__ __forceinline__ int64_t take_site(const SitesArgs& a, int64_t prev, int* next) {
+  const int64_t step = prev < 0 ? (int64_t)blockIdx.x : prev + gridDim.x;
+  if (threadIdx.x == 0) {
+    int64_t i = a.queue ? (int64_t)atomicAdd(a.queue, 1) : step;
+    // an unaudited site after a passed audit is left to the surrogate
+    if (a.audit_pred && i >= a.n_audit && i < a.P && audit_passed(a)) i = a.P;
+    *next = (int)(i < a.P ? i : a.P);
   }
-  __shared__ int next;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      int i = atomicAdd(a.queue, 1);
-      // an unaudited site after a passed audit is left to the surrogate
-      if (a.audit_pred && i >= a.n_audit && i < a.P && audit_passed(a)) i = (int)a.P;
-      next = i;
-    }
-    __syncthreads();
-    const int64_t i = next;
-    __syncthreads();  // every thread has read `next` before thread 0 takes the following one
-    if (i >= a.P) return;
-    run(a.order[i]);
-  }
+  __syncthreads();
+  // the same value in every lane: readfirstlane keeps it, and the site addresses derived from
+  // it, in scalar registers
+  const int64_t i = __builtin_amdgcn_readfirstlane(*next);
+  __syncthreads();  // every thread has read `next` before thread 0 takes the following one
+  return i;
 }
 
 __global__ void queue_reset_kernel(int* queue) {
@@ -236,14 +231,14 @@ __device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, i
     }
     if (!FIRST) {
       if (p1) {  // stage 1 of the inverse transform along y
-        float2 v[N2];
+        float2 v1[N2];
         const float2* src = c.fld + (int64_t)(row0 + l1) * N + n1;
 #pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[N1 * n2];
-        fftk::fft<N2, true>(v);
+        for (int n2 = 0; n2 < N2; ++n2) v1[n2] = src[N1 * n2];
+        fftk::fft<N2, true>(v1);
         float2* dst = sm.x + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
-        for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmulc(v[j], sm.tw[n1 * j]);
+        for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v1[j] : cmulc(v1[j], sm.tw[n1 * j]);
       }
       lds_barrier();
     }
@@ -454,17 +449,32 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
 }
 
 // One field side for every site of the launch.
-template <int N, int NT, int MODE>
+// DYN: sites from the work counter and/or an audited launch (take_site); otherwise the plain
+// static-striding loop.  The schedule is a compile-time choice because the loop's form changes
+// the code generated for the inlined site body: with take_site's barriers in the loop, the
+// uniform 4225-site launch runs 112 ms against 103 ms for the plain loop, static either way.
+template <int N, int NT, int MODE, bool DYN>
 __global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
   __shared__ Smem<NT, N> sm;
-  for_each_site(a, [&](int site) { run_site<N, NT, MODE>(a, site, sm); });
+  if constexpr (DYN) {
+    __shared__ int next;
+    for (int64_t i = take_site(a, -1, &next); i < a.P; i = take_site(a, i, &next))
+      run_site<N, NT, MODE>(a, a.order[i], sm);
+  } else {
+    for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run_site<N, NT, MODE>(a, a.order[i], sm);
+  }
 }
 
-// Sites of mixed field sides in one launch (the order list puts large sites first).
+// Sites of mixed field sides in one launch (the order list puts large sites first).  The
+// site body is a closure called from both loops (static striding, work counter), so it is not
+// inlined: compiled as a function of its own, the four field sides' values are not held across
+// the site loop.  This form runs the mixed 4225-site launch in 80 ms; inlined into the loops
+// (with the uniform kernels' take_site) the same body runs it in 93-97 ms.
 template <int NT>
 __global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
   __shared__ Smem<NT, kNmax> sm;
-  for_each_site(a, [&](int site) {
+  __shared__ int next;
+  auto run = [&](int site) {
     switch (a.site_n[site]) {
       case 64: run_site<64, NT, kScreens32>(a, site, sm); break;
       case 128: run_site<128, NT, kScreens32>(a, site, sm); break;
@@ -472,7 +482,19 @@ __global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
       case 512: run_site<512, NT, kScreens32>(a, site, sm); break;
       default: break;  // rejected on the host
     }
-  });
+  };
+  if (!a.queue) {
+    for (int64_t i = blockIdx.x; i < a.P; i += gridDim.x) run(a.order[i]);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int64_t i = next;
+    __syncthreads();  // every thread has read `next` before thread 0 takes the following one
+    if (i >= a.P) return;
+    run(a.order[i]);
+  }
 }
 
 // ---------------------------------------------------------------- phase prep
@@ -564,8 +586,8 @@ MAOMI_API int64_t maomi_sites_default_grid(void) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sites_kernel<256, 512, kScreens32>, 512, 0) !=
-          hipSuccess ||
+  const auto kernel = sites_kernel<256, 512, kScreens32, false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 512, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   return (int64_t)cus * per_cu;
@@ -588,7 +610,8 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
     maomi_set_error("maomi_persistent_sites: phase16 needs slice_scale");
     return MAOMI_EINVAL;
   }
-  if (d->scratch_stride < 64 * 64 || d->H <= 0 || d->W <= 0 || d->Z <= 0) {
+  if (d->scratch_stride < 64 * 64 || d->H <= 0 || d->W <= 0 || d->Z <= 0 ||
+      d->P > (1LL << 30)) {  // site indices and the int32 work counter stay far from overflow
     maomi_set_error("maomi_persistent_sites: bad sizes");
     return MAOMI_EINVAL;
   }
@@ -620,11 +643,10 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
   a.audit_max_fail = 0;
   a.audit_tol = 0.f;
   if (const maomi_sites_audit* au = d->audit) {
-    if (!d->psf_out || !d->queue || !au->pred || !au->slot || !au->state || !au->rel ||
-        !au->done || au->n_audit < 1 || au->n_audit > d->P || au->max_fail < 0 ||
-        !(au->tol >= 0.0)) {
-      maomi_set_error("maomi_persistent_sites: an audit needs focal-plane output, a work "
-                      "counter, every audit array and 1 <= n_audit <= P");
+    if (!d->psf_out || !au->pred || !au->slot || !au->state || !au->rel || !au->done ||
+        au->n_audit < 1 || au->n_audit > d->P || au->max_fail < 0 || !(au->tol >= 0.0)) {
+      maomi_set_error("maomi_persistent_sites: an audit needs focal-plane output, every audit "
+                      "array and 1 <= n_audit <= P");
       return MAOMI_EINVAL;
     }
     a.audit_pred = au->pred;
@@ -670,16 +692,23 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
                    : a.phase16 ? kScreens16
                    : a.screen  ? kScreensC64
                                : kScreens32;
-#define MAOMI_LAUNCH(NTV, MODEV)                                                               \
+  const bool dyn = a.queue || a.audit_pred;
+#define MAOMI_LAUNCH_D(NTV, MODEV, DV)                                                         \
   switch (d->uniform_n) {                                                                      \
-    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV, MODEV>), g, b, 0, st, a); break;        \
-    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV, MODEV>), g, b, 0, st, a); break;      \
-    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV, MODEV>), g, b, 0, st, a); break;      \
-    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV, MODEV>), g, b, 0, st, a); break;      \
+    case 64: hipLaunchKernelGGL((sites_kernel<64, NTV, MODEV, DV>), g, b, 0, st, a); break;    \
+    case 128: hipLaunchKernelGGL((sites_kernel<128, NTV, MODEV, DV>), g, b, 0, st, a); break;  \
+    case 256: hipLaunchKernelGGL((sites_kernel<256, NTV, MODEV, DV>), g, b, 0, st, a); break;  \
+    case 512: hipLaunchKernelGGL((sites_kernel<512, NTV, MODEV, DV>), g, b, 0, st, a); break;  \
     case 0: hipLaunchKernelGGL((sites_mixed_kernel<NTV>), g, b, 0, st, a); break;              \
     default:                                                                                   \
       maomi_set_error("maomi_persistent_sites: uniform_n must be 0, 64, 128, 256 or 512");     \
       return MAOMI_EUNSUPPORTED;                                                               \
+  }
+#define MAOMI_LAUNCH(NTV, MODEV)  \
+  if (dyn) {                      \
+    MAOMI_LAUNCH_D(NTV, MODEV, true) \
+  } else {                        \
+    MAOMI_LAUNCH_D(NTV, MODEV, false) \
   }
   if (d->uniform_n != 0 && d->uniform_n != 64 && d->uniform_n != 128 && d->uniform_n != 256 &&
       d->uniform_n != 512) {
@@ -712,6 +741,7 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
     MAOMI_LAUNCH(256, kScreens32)
   }
 #undef MAOMI_LAUNCH
+#undef MAOMI_LAUNCH_D
   MAOMI_CHECK_LAUNCH();
   return MAOMI_OK;
 }

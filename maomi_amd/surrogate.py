@@ -410,7 +410,7 @@ class SurrogateStage:
     """Surrogate inference with a seeded audit inside the propagator's launch.
 
     The network predicts the audited sites (a seeded fraction) first.  One persistent
-    focal-plane launch then propagates the sites with the audited ones first in its queue; as
+    focal-plane launch then propagates the sites with the audited ones first in its order; as
     each audited site is done its prediction is compared on the device (maomi_sites_audit).
     Once every audited site is done with at most `max_fail_frac` of them above `tol`, the
     launch's blocks stop taking unaudited sites and the network predicts those that were left.
@@ -419,8 +419,9 @@ class SurrogateStage:
     (audited sites included)."""
 
     def __init__(self, net, scale, stage, *, audit_frac=0.05, tol=0.01, max_fail_frac=0.1,
-                 seed=0):
+                 seed=0, schedule="static"):
         self.net, self.scale, self.stage = net, float(scale), stage
+        self.schedule = schedule  # of the focal-plane launch: "static" or "queue"
         self.audit_frac, self.tol, self.max_fail_frac = audit_frac, tol, max_fail_frac
         self.rng = np.random.default_rng(seed)
         self.timing, self.report = {}, {}
@@ -445,7 +446,8 @@ class SurrogateStage:
               "max_fail": int(np.floor(self.max_fail_frac * k)), "tol": self.tol}
         z0, nz, c0, w = focal_window(cfg)
         out = torch.empty((nz, P, w, w), dtype=torch.float32, device=dev)
-        st.propagate(psf=(out, z0, nz, c0, w), order=np.concatenate([audit, rest]), audit=au)
+        st.propagate(psf=(out, z0, nz, c0, w), order=np.concatenate([audit, rest]), audit=au,
+                     schedule=self.schedule)
         state = au["state"].cpu().numpy()
         done = au["done"].cpu().numpy().astype(bool)
         rel = au["rel"].cpu().numpy()

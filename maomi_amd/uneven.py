@@ -3,11 +3,12 @@
 Every site s has its own field side N_s in {128, 256, 512} and slice count S_s
 (focal depth d_s = S_s dz, focus on the last plane).  All sites run in one
 launch of the mixed-size persistent kernel (maomi_persistent_sites with
-uniform_n = 0); blocks take sites in the order of a list sorted by cost
-(S N^2 log N, largest first) and the launch is captured once in a HIP graph
-and replayed.  An autotuner picks the grid size, the tile shape (256- or
-512-thread blocks: 16- or 32-line LDS tiles) and the ordering policy by
-timing the captured candidates on the device.
+uniform_n = 0); blocks take sites from a work counter in the order of a list
+sorted by cost (S N^2 log N, largest first) and the launch is captured once in
+a HIP graph and replayed.  An autotuner picks the grid size, the tile shape
+(256- or 512-thread blocks: 16- or 32-line LDS tiles), the ordering policy and
+the schedule (work counter or static striding) by timing the captured
+candidates on the device.
 """
 
 import ctypes
@@ -188,37 +189,41 @@ class UnevenStage:
                 pass
 
     def autotune(self, grids=None, threads=(512, 256), orders=("lpt", "class_interleaved"),
-                 reps=2):
-        """Time captured candidates (grid, block shape, ordering) in the plan and keep the
-        fastest (maomi_sites_plan_tune)."""
+                 schedules=("queue", "static"), reps=2):
+        """Time captured candidates (grid, block shape, ordering, schedule) in the plan and
+        keep the fastest (maomi_sites_plan_tune)."""
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         grids = [int(g) for g in (grids or (cus, 2 * cus))]
         threads, orders = [int(t) for t in threads], [nat.ORDERS[o] for o in orders]
+        schedules = [nat.SCHEDULES[s] for s in schedules]
         arr = lambda v: (ctypes.c_int64 * len(v))(*v)
-        n = len(grids) * len(threads) * len(orders)
+        n = len(grids) * len(threads) * len(orders) * len(schedules)
         results = (nat.SitesChoice * n)()
         best = nat.SitesChoice()
         nat.check(nat.lib().maomi_sites_plan_tune(
             self._plan_handle(), arr(grids), len(grids), arr(threads), len(threads), arr(orders),
-            len(orders), int(reps), nat.stream_handle(self.device), results, ctypes.byref(best)),
-            "sites plan tune")
+            len(orders), arr(schedules), len(schedules), int(reps),
+            nat.stream_handle(self.device), results, ctypes.byref(best)), "sites plan tune")
         name = {v: k for k, v in nat.ORDERS.items()}
+        sname = {v: k for k, v in nat.SCHEDULES.items()}
         as_dict = lambda c: {"grid": int(c.grid), "threads": int(c.block_threads),
-                             "order": name[int(c.order)], "ms": float(c.ms)}
+                             "order": name[int(c.order)], "schedule": sname[int(c.schedule)],
+                             "ms": float(c.ms)}
         self.tune_log = [as_dict(c) for c in results]
         self.tuned = as_dict(best)
         return self.tuned
 
     def propagate(self):
         """Replay the plan's captured launch; without autotune, capture the default choice
-        (one block per CU, 512 threads, longest first) first."""
+        (one block per CU, 512 threads, longest first from the work counter) first."""
         h = self._plan_handle()
         if self.tuned is None:
             cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-            nat.check(nat.lib().maomi_sites_plan_capture(h, min(cus, len(self.sizes)), 512,
-                                                         nat.ORDERS["lpt"]), "sites plan capture")
+            nat.check(nat.lib().maomi_sites_plan_capture(
+                h, min(cus, len(self.sizes)), 512, nat.ORDERS["lpt"], nat.SCHEDULES["queue"]),
+                "sites plan capture")
             self.tuned = {"grid": min(cus, len(self.sizes)), "threads": 512, "order": "lpt",
-                          "ms": -1.0}
+                          "schedule": "queue", "ms": -1.0}
         nat.check(nat.lib().maomi_sites_plan_replay(h, nat.stream_handle(self.device)),
                   "sites plan replay")
         return self.masks, self.energies

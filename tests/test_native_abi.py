@@ -51,7 +51,8 @@ def test_sites_descriptor_layout_matches_header():
     assert ctypes.sizeof(_native.SitesAudit) == 8 * 8
     assert _native.SitesDesc.screen.offset == 8 * (18 + 12 + 15)
     assert _native.SitesDesc.queue.offset == 8 * (19 + 12 + 15)
-    assert ctypes.sizeof(_native.SitesChoice) == 32
+    assert ctypes.sizeof(_native.SitesChoice) == 40
+    assert _native.SitesChoice.ms.offset == 32
 
 
 def test_sites_launch_and_plan_reject_bad_arguments_before_any_gpu_work():
@@ -83,7 +84,12 @@ def test_sites_launch_and_plan_reject_bad_arguments_before_any_gpu_work():
     assert lib.maomi_sites_plan_create(d, ctypes.byref(h)) == -2
     assert b"FP32-phase" in lib.maomi_last_error()
     assert lib.maomi_sites_plan_replay(None, None) == -1
-    assert lib.maomi_sites_plan_capture(None, 4, 512, 1) == -1
+    assert lib.maomi_sites_plan_capture(None, 4, 512, 1, 0) == -1
+    assert b"schedule 0 or 1" in lib.maomi_last_error()
+    d.P = (1 << 30) + 1  # site indices and the work counter would come near int32 overflow
+    d.phase16 = d.slice_scale = None
+    assert lib.maomi_persistent_sites(d, None) == -1
+    assert b"bad sizes" in lib.maomi_last_error()
     assert lib.maomi_sites_plan_destroy(None) == 0
 
 

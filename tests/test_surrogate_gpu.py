@@ -219,11 +219,13 @@ def test_trained_surrogate_beats_mean_psf_predictor(gpu_lib):
     assert err_net < 0.85 * err_mean
 
 
-def test_surrogate_stage_audit_and_fallback(gpu_lib):
-    """Both outcomes of the in-launch audit.  Passed (tol huge, 4 blocks for 81 sites): the
-    launch stops taking unaudited sites once the 9 audited ones are done; every propagated site
-    keeps the propagator's PSF bit for bit and the network predicts the rest.  Failed (tol 0):
-    every site is the propagator's PSF, bit for bit, from the same single launch."""
+@pytest.mark.parametrize("schedule", ["static", "queue"])
+def test_surrogate_stage_audit_and_fallback(gpu_lib, schedule):
+    """Both outcomes of the in-launch audit, under either schedule of the launch.  Passed (tol
+    huge, 4 blocks for 81 sites): the launch stops taking unaudited sites once the 9 audited
+    ones are done; every propagated site keeps the propagator's PSF bit for bit and the network
+    predicts the rest.  Failed (tol 0): every site is the propagator's PSF, bit for bit, from
+    the same single launch."""
     from maomi_amd.plain import PlainStage
     from maomi_amd.surrogate import SurrogateNet, SurrogateStage, site_features, site_psfs
     from maomi_amd.workloads import PlainConfig, index_volume
@@ -234,8 +236,8 @@ def test_surrogate_stage_audit_and_fallback(gpu_lib):
     truth = site_psfs(st)
     pred = net.predict(site_features(vol, cfg, st.offsets, cfg.focus)) * 2.0
     st.grid = 4
-    print("[surrogate stage] 81 sites, 9 audited, 4 blocks", flush=True)
-    sur = SurrogateStage(net, 2.0, st, audit_frac=0.1, tol=1e9, seed=5)
+    print(f"[surrogate stage] 81 sites, 9 audited, 4 blocks, {schedule}", flush=True)
+    sur = SurrogateStage(net, 2.0, st, audit_frac=0.1, tol=1e9, seed=5, schedule=schedule)
     out = sur.run()
     rep = sur.report
     assert rep["mode"] == "surrogate" and rep["audited"] == 9 and rep["audit_fail"] == 0
@@ -247,7 +249,7 @@ def test_surrogate_stage_audit_and_fallback(gpu_lib):
     assert int(done.sum()) >= rep["propagated_sites"]
     left = ~done
     torch.testing.assert_close(out[:, left], pred[:, left], rtol=1e-6, atol=1e-7)
-    sur = SurrogateStage(net, 2.0, st, audit_frac=0.1, tol=0.0, seed=5)
+    sur = SurrogateStage(net, 2.0, st, audit_frac=0.1, tol=0.0, seed=5, schedule=schedule)
     out = sur.run()
     assert sur.report["mode"] == "fallback_all" and sur.report["audit_fail"] == 9
     assert sur.report["propagated_sites"] == 81 and sur.report["predicted_sites"] == 0

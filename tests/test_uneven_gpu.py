@@ -145,9 +145,10 @@ def test_config3_real_draw_sampled_sites_match_oracle(gpu_lib):
 
 @pytest.mark.gpu
 def test_plan_replay_equals_eager_launches(gpu_lib):
-    """The C-ABI plan's captured launch (dynamic scheduling, plan-owned orderings and
-    workspace) gives bit for bit the masks of eager maomi_persistent_sites launches with static
-    striding, for each candidate the tuner timed, over replays with eager work between them."""
+    """The C-ABI plan's captured launch (plan-owned orderings, workspace and work counter)
+    gives bit for bit the masks of eager maomi_persistent_sites launches with static striding,
+    for each candidate the tuner timed (both schedules), over replays with eager work between
+    them."""
     import ctypes
     import torch
     from maomi_amd import _native as nat
@@ -163,7 +164,8 @@ def test_plan_replay_equals_eager_launches(gpu_lib):
     print("[plan] 13 mixed sites", flush=True)
     tuned = st.autotune(grids=(3, 5), threads=(512, 256), orders=("lpt", "class_interleaved"),
                         reps=1)
-    assert len(st.tune_log) == 8 and all(c["ms"] > 0 for c in st.tune_log)
+    assert len(st.tune_log) == 16 and all(c["ms"] > 0 for c in st.tune_log)
+    assert {c["schedule"] for c in st.tune_log} == {"queue", "static"}
     assert tuned["ms"] == min(c["ms"] for c in st.tune_log)
     got = []
     for _ in range(3):
@@ -186,3 +188,4 @@ def test_plan_replay_equals_eager_launches(gpu_lib):
     choice = nat.SitesChoice()
     nat.check(gpu_lib.maomi_sites_plan_choice(st._plan, ctypes.byref(choice)), "choice")
     assert (choice.grid, choice.block_threads) == (tuned["grid"], tuned["threads"])
+    assert choice.schedule == nat.SCHEDULES[tuned["schedule"]]
