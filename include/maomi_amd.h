@@ -357,6 +357,21 @@ int maomi_v4_apply_split_f64_noalloc(const double* points, const double* coef,
 int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* workspace, int* info,
                        maomi_stream_t stream);
 int64_t maomi_cholesky_f64_workspace(void);
+/* Finish of the v4 solve (replaces the rest of np.linalg.solve, maomi/numerics.py:129): w
+ * (n x C, row-major) with G0 w = v for the Green matrix G0 of n > 3 points, from G holding the
+ * projected system A = Q^T G0 Q (Q = I - V T V^T: V n x 3 row-major, T 3 x 3 upper triangular,
+ * the compact WY form of the reflectors of the QR of [1 x y]) with maomi_cholesky_f64's factor
+ * of A[3:, 3:] in its lower triangle (row-major, leading dimension ldg >= n).  Solves
+ * A[3:, 3:] [Z z] = [A[:3, 3:]^T y2] for y = Q^T v by blocked forward and backward substitution,
+ * the 3 x 3 Schur complement by pivoted elimination, and returns w = Q [a; z - Z a].  FP64.
+ * v is n x C (1 <= C <= MAOMI_V4_FINISH_MAX_C); workspace: maomi_v4_finish_f64_workspace(n, C)
+ * bytes of device memory.  G, V, T and v are not modified. */
+#define MAOMI_V4_FINISH_MAX_C 13
+int maomi_v4_finish_f64(const double* G, int64_t ldg, int64_t n, const double* V,
+                        const double* T, const double* v, int64_t C, double* w,
+                        double* workspace, maomi_stream_t stream);
+/* Workspace bytes of maomi_v4_finish_f64, or -1 for sizes it rejects. */
+int64_t maomi_v4_finish_f64_workspace(int64_t n, int64_t C);
 
 /* ---------------------------------------------------------------------------
  * 2-D PSF contracts (maomi/psf.py), FP64.  maomi_amd.psf is the host side.

@@ -184,6 +184,9 @@ _SIGS = {
     "maomi_v4_apply_rows": (c_int64, [c_int64, c_int64]),
     "maomi_cholesky_f64": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     "maomi_cholesky_f64_workspace": (c_int64, []),
+    "maomi_v4_finish_f64": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                    c_int64, c_void_p, c_void_p, c_void_p]),
+    "maomi_v4_finish_f64_workspace": (c_int64, [c_int64, c_int64]),
     "maomi_conv_bf16_fwd": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64,
                                     c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64,
                                     c_float, c_void_p, c_int64, c_void_p]),

@@ -505,6 +505,339 @@ MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* work
   return MAOMI_OK;
 }
 
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// FP64 finish of the v4 solve G w = v from the factored projected system (maomi_v4_finish_f64).
+// With Q = I - V T V^T (compact WY form of the QR reflectors of P = [1 x y]) and A = Q^T G Q,
+// held in G: A11 = A[:3, :3], A12 = A[:3, 3:], and L (A22 = L L^T) in the lower triangle of
+// A[3:, 3:].  Then, with y = Q^T v split into y1 (3 rows) and y2:
+//   [Z z] = A22^-1 [A12^T y2]                 (v4_rhs_kernel, the two triangular sweeps)
+//   a = (A11 - A12 Z)^-1 (y1 - A12 z)         (3 x 3 Schur complement, pivoted elimination)
+//   x = [a; z - Z a],  w = Q x                (v4_schur_kernel)
+// The triangular sweeps go in panels of kTs rows, one launch per panel: every block subtracts
+// the last solved panel's contribution from 64 trailing rows, and block 0 (which holds the next
+// panel's rows) then solves that panel's diagonal block, a wave per group of right-hand-side
+// columns with the rows on the lanes.  R = 3 + C right-hand sides, row-major in the workspace.
+constexpr int kTs = 64;
+constexpr int kTsThreads = 256;  // 4 waves: wave g holds columns g, g + 4, g + 8, g + 12
+constexpr int kTsCols = 4;       // columns per thread
+constexpr int kMaxRhs = kTsThreads / 64 * kTsCols;  // 16
+constexpr int kFinThreads = 1024;
+
+struct TrsShared {
+  double tile[kTs][kTs + 1];  // L tile of the update (stride 65 doubles against bank conflicts)
+  double diag[kTs][kTs + 1];  // the next diagonal block (block 0)
+  double pan[kTs][kMaxRhs];   // solved rows of the last panel
+};
+
+// Lane l's value of a wave-uniform lane index.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+
+// Sweep step of L Y = B (forward) or L^T X = Y (backward) on B (m x R, row-major).
+// Forward, panel s = step: blocks update rows r0 = (s + 1 + k) kTs ...; backward, panel p =
+// step: rows r0 = (p - 1 - k) kTs ...  With diag_only, block 0 solves the diagonal block of
+// panel `step` itself (the first panel of the sweep) and nothing is updated.
+// Block 0's diagonal solve keeps its lane's row (forward) or column (backward) of the block in
+// registers and the reciprocal of its diagonal entry: one step of the substitution is a lane
+// broadcast, a multiply and a fused multiply-add.  Rows past m are padded with the identity.
+template <bool FWD>
+__global__ __launch_bounds__(kTsThreads) void trs_step_kernel(const double* __restrict__ L,
+                                                              int64_t ldl, int64_t m,
+                                                              int64_t step, int diag_only,
+                                                              double* __restrict__ B, int R) {
+  __shared__ TrsShared sm;
+  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+  int64_t r0;
+  if (diag_only)
+    r0 = step * kTs;
+  else
+    r0 = FWD ? (step + 1 + (int64_t)blockIdx.x) * kTs : (step - 1 - (int64_t)blockIdx.x) * kTs;
+  const int rows = (int)(m - r0 < kTs ? m - r0 : kTs);  // >= 1 by the grid's construction
+  const int kc = (R - g + 3) / 4;                       // columns of this wave (g + 4 k < R)
+  // Tiles are staged with every load of a thread in flight at once (unrolled into registers,
+  // then stored to LDS): a loop that waits for each load before the next one is issued spends
+  // 16 memory latencies per tile.
+  constexpr int kPer = kTs * kTs / kTsThreads;  // tile elements per thread
+  double dv[kPer];
+  if (blockIdx.x == 0) {  // requested first: its loads overlap the update's
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int e = tid + it * kTsThreads, i = e / kTs, j = e % kTs;
+      dv[it] = (i < rows && j <= i) ? L[(r0 + i) * ldl + r0 + j] : (i == j) ? 1.0 : 0.0;
+    }
+  }
+  double b[kTsCols];
+#pragma unroll
+  for (int k = 0; k < kTsCols; ++k) {
+    const int c = g + 4 * k;
+    b[k] = (lane < rows && c < R) ? B[(r0 + lane) * R + c] : 0.0;
+  }
+  if (!diag_only) {
+    const int64_t p0 = step * kTs;                               // rows of the solved panel
+    const int bp = (int)(m - p0 < kTs ? m - p0 : kTs);
+    double tv[kPer];
+    constexpr int kPan = kTs * kMaxRhs / kTsThreads;
+    double pv[kPan];
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int e = tid + it * kTsThreads, i = e / kTs, j = e % kTs;
+      if (FWD)  // tile[i][j] = L[r0 + i][p0 + j]: this block's rows, the panel's columns
+        tv[it] = i < rows ? L[(r0 + i) * ldl + p0 + j] : 0.0;
+      else      // tile[j][i] = L[p0 + j][r0 + i]: the panel's rows, this block's columns
+        tv[it] = i < bp ? L[(p0 + i) * ldl + r0 + j] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < kPan; ++it) {
+      const int e = tid + it * kTsThreads, j = e / kMaxRhs, c = e % kMaxRhs;
+      pv[it] = (j < bp && c < R) ? B[(p0 + j) * R + c] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int e = tid + it * kTsThreads;
+      sm.tile[e / kTs][e % kTs] = tv[it];
+    }
+#pragma unroll
+    for (int it = 0; it < kPan; ++it) {
+      const int e = tid + it * kTsThreads;
+      sm.pan[e / kMaxRhs][e % kMaxRhs] = pv[it];
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int j = 0; j < kTs; ++j) {
+      const double l = FWD ? sm.tile[lane][j] : sm.tile[j][lane];
+#pragma unroll
+      for (int k = 0; k < kTsCols; ++k)
+        if (k < kc) b[k] -= l * sm.pan[j][g + 4 * k];
+    }
+  }
+  if (blockIdx.x == 0) {
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int e = tid + it * kTsThreads;
+      sm.diag[e / kTs][e % kTs] = dv[it];
+    }
+    __syncthreads();  // the diagonal block is staged
+    double d[kTs];    // forward: d[j] = D[lane][j]; backward: d[j] = D[j][lane]
+#pragma unroll
+    for (int j = 0; j < kTs; ++j) d[j] = FWD ? sm.diag[lane][j] : sm.diag[j][lane];
+    const double inv = 1.0 / sm.diag[lane][lane];
+#pragma unroll
+    for (int k = 0; k < kTsCols; ++k) {
+      if (k >= kc) continue;  // uniform in the wave
+      double x = b[k];
+      if (FWD) {
+#pragma unroll
+        for (int j = 0; j < kTs; ++j) {  // y_j = b_j / D_jj, then b_i -= D_ij y_j for i > j
+          const double yj = readlane_f64(x, j) * readlane_f64(inv, j);
+          const double t = fma(-d[j], yj, x);
+          x = lane > j ? t : x;
+          x = lane == j ? yj : x;
+        }
+      } else {
+#pragma unroll
+        for (int j = kTs - 1; j >= 0; --j) {  // x_j = b_j / D_jj, then b_i -= D_ji x_j, i < j
+          const double xj = readlane_f64(x, j) * readlane_f64(inv, j);
+          const double t = fma(-d[j], xj, x);
+          x = lane < j ? t : x;
+          x = lane == j ? xj : x;
+        }
+      }
+      b[k] = x;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kTsCols; ++k) {
+    const int c = g + 4 * k;
+    if (lane < rows && c < R) B[(r0 + lane) * R + c] = b[k];
+  }
+}
+
+// Block-wide sums of n <= 64 per-thread values in place (every thread receives every sum).
+__device__ void block_sums(double* v, int n, double* red /* [kFinThreads / 64][64] */,
+                           double* out /* [64] */) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = 0; i < n; ++i) {
+    double x = v[i];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[wave * 64 + i] = x;
+  }
+  __syncthreads();
+  if (tid < n) {
+    double t = 0.0;
+    for (int w = 0; w < kFinThreads / 64; ++w) t += red[w * 64 + tid];
+    out[tid] = t;
+  }
+  __syncthreads();
+  for (int i = 0; i < n; ++i) v[i] = out[i];
+  __syncthreads();
+}
+
+// y = Q^T v = v - V (T^T (V^T v)); y1 = y[:3] -> y1 out; B[i] = [A12^T[i], y[3 + i]].
+__global__ __launch_bounds__(kFinThreads) void v4_rhs_kernel(
+    const double* __restrict__ G, int64_t ldg, int64_t n, const double* __restrict__ V,
+    const double* __restrict__ T, const double* __restrict__ v, int C, double* __restrict__ B,
+    double* __restrict__ y1) {
+  __shared__ double red[kFinThreads];
+  __shared__ double out[64];
+  const int tid = threadIdx.x, R = 3 + C;
+  double u[3 * kMaxRhs];
+  for (int e = 0; e < 3 * C; ++e) u[e] = 0.0;
+  for (int64_t i = tid; i < n; i += kFinThreads)
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < C; ++c) u[k * C + c] += V[i * 3 + k] * v[i * C + c];
+  block_sums(u, 3 * C, red, out);
+  double t[3 * kMaxRhs];  // T^T u
+  for (int k = 0; k < 3; ++k)
+    for (int c = 0; c < C; ++c)
+      t[k * C + c] = T[0 * 3 + k] * u[0 * C + c] + T[1 * 3 + k] * u[1 * C + c] +
+                     T[2 * 3 + k] * u[2 * C + c];
+  for (int64_t i = tid; i < n; i += kFinThreads) {
+    for (int c = 0; c < C; ++c) {
+      const double y = v[i * C + c] - (V[i * 3 + 0] * t[0 * C + c] + V[i * 3 + 1] * t[1 * C + c] +
+                                       V[i * 3 + 2] * t[2 * C + c]);
+      if (i < 3) y1[i * C + c] = y;
+      else B[(i - 3) * R + 3 + c] = y;
+    }
+    if (i >= 3)
+      for (int k = 0; k < 3; ++k) B[(i - 3) * R + k] = G[k * ldg + i];
+  }
+}
+
+// a = (A11 - A12 Z)^-1 (y1 - A12 z); x = [a; z - Z a]; w = x - V (T (V^T x)).
+__global__ __launch_bounds__(kFinThreads) void v4_schur_kernel(
+    const double* __restrict__ G, int64_t ldg, int64_t n, const double* __restrict__ V,
+    const double* __restrict__ T, const double* __restrict__ X, const double* __restrict__ y1,
+    int C, double* __restrict__ w) {
+  __shared__ double red[kFinThreads];
+  __shared__ double out[64];
+  __shared__ double av[3 * kMaxRhs];
+  const int tid = threadIdx.x, R = 3 + C;
+  const int64_t m = n - 3;
+  // S = A12 Z (9 sums) and r = A12 z (3 C sums)
+  double s[9 + 3 * kMaxRhs];
+  const int ns = 9 + 3 * C;
+  for (int e = 0; e < ns; ++e) s[e] = 0.0;
+  for (int64_t i = tid; i < m; i += kFinThreads)
+    for (int k = 0; k < 3; ++k) {
+      const double a12 = G[k * ldg + 3 + i];
+      for (int l = 0; l < R; ++l) s[k * R + l] += a12 * X[i * R + l];
+    }
+  block_sums(s, ns, red, out);  // s[k R + l]: (A12 X)[k][l]
+  if (tid == 0) {
+    // S a = r with partial pivoting; S = A11 - A12 Z, r = y1 - A12 z
+    double S[3][3], r[3][kMaxRhs];
+    for (int k = 0; k < 3; ++k) {
+      for (int l = 0; l < 3; ++l) S[k][l] = G[k * ldg + l] - s[k * R + l];
+      for (int c = 0; c < C; ++c) r[k][c] = y1[k * C + c] - s[k * R + 3 + c];
+    }
+    for (int col = 0; col < 3; ++col) {
+      int piv = col;
+      for (int k = col + 1; k < 3; ++k)
+        if (fabs(S[k][col]) > fabs(S[piv][col])) piv = k;
+      if (piv != col) {
+        for (int l = 0; l < 3; ++l) { const double t = S[col][l]; S[col][l] = S[piv][l]; S[piv][l] = t; }
+        for (int c = 0; c < C; ++c) { const double t = r[col][c]; r[col][c] = r[piv][c]; r[piv][c] = t; }
+      }
+      for (int k = col + 1; k < 3; ++k) {
+        const double f = S[k][col] / S[col][col];
+        for (int l = col; l < 3; ++l) S[k][l] -= f * S[col][l];
+        for (int c = 0; c < C; ++c) r[k][c] -= f * r[col][c];
+      }
+    }
+    for (int k = 2; k >= 0; --k)
+      for (int c = 0; c < C; ++c) {
+        double t = r[k][c];
+        for (int l = k + 1; l < 3; ++l) t -= S[k][l] * av[l * C + c];
+        av[k * C + c] = t / S[k][k];
+      }
+  }
+  __syncthreads();
+  // x into w, and u = V^T x
+  double u[3 * kMaxRhs];
+  for (int e = 0; e < 3 * C; ++e) u[e] = 0.0;
+  for (int64_t i = tid; i < n; i += kFinThreads)
+    for (int c = 0; c < C; ++c) {
+      double x;
+      if (i < 3) {
+        x = av[i * C + c];
+      } else {
+        const double* Xi = X + (i - 3) * R;
+        x = Xi[3 + c] - (Xi[0] * av[0 * C + c] + Xi[1] * av[1 * C + c] + Xi[2] * av[2 * C + c]);
+      }
+      w[i * C + c] = x;
+      for (int k = 0; k < 3; ++k) u[k * C + c] += V[i * 3 + k] * x;
+    }
+  block_sums(u, 3 * C, red, out);
+  double t[3 * kMaxRhs];  // T u
+  for (int k = 0; k < 3; ++k)
+    for (int c = 0; c < C; ++c)
+      t[k * C + c] = T[k * 3 + 0] * u[0 * C + c] + T[k * 3 + 1] * u[1 * C + c] +
+                     T[k * 3 + 2] * u[2 * C + c];
+  for (int64_t i = tid; i < n; i += kFinThreads)  // each thread rereads the x it wrote
+    for (int c = 0; c < C; ++c)
+      w[i * C + c] -= V[i * 3 + 0] * t[0 * C + c] + V[i * 3 + 1] * t[1 * C + c] +
+                      V[i * 3 + 2] * t[2 * C + c];
+}
+
+}  // namespace
+
+MAOMI_API int64_t maomi_v4_finish_f64_workspace(int64_t n, int64_t C) {
+  if (n <= 3 || C < 1 || C > MAOMI_V4_FINISH_MAX_C) return -1;
+  return ((n - 3) * (3 + C) + 3 * C) * (int64_t)sizeof(double);
+}
+
+MAOMI_API int maomi_v4_finish_f64(const double* G, int64_t ldg, int64_t n, const double* V,
+                                  const double* T, const double* v, int64_t C, double* w,
+                                  double* workspace, maomi_stream_t stream) {
+  if (!G || !V || !T || !v || !w || !workspace || n <= 3 || ldg < n || C < 1 ||
+      C > MAOMI_V4_FINISH_MAX_C) {
+    maomi_set_error("maomi_v4_finish_f64: bad arguments (n > 3, ldg >= n, 1 <= C <= %d)",
+                    MAOMI_V4_FINISH_MAX_C);
+    return MAOMI_EINVAL;
+  }
+  static_assert(3 + MAOMI_V4_FINISH_MAX_C <= kMaxRhs, "right-hand sides per sweep");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t m = n - 3;
+  const int R = (int)(3 + C);
+  double* B = workspace;
+  double* y1 = workspace + m * R;
+  const double* L = G + 3 * ldg + 3;
+  hipLaunchKernelGGL(v4_rhs_kernel, dim3(1), dim3(kFinThreads), 0, st, G, ldg, n, V, T, v,
+                     (int)C, B, y1);
+  MAOMI_CHECK_LAUNCH();
+  const int64_t P = (m + kTs - 1) / kTs;
+  // forward sweep: panel 0's diagonal block, then one launch per panel s < P - 1
+  hipLaunchKernelGGL(trs_step_kernel<true>, dim3(1), dim3(kTsThreads), 0, st, L, ldg, m,
+                     (int64_t)0, 1, B, R);
+  MAOMI_CHECK_LAUNCH();
+  for (int64_t s = 0; s + 1 < P; ++s) {
+    const unsigned blocks = (unsigned)((m - (s + 1) * kTs + kTs - 1) / kTs);
+    hipLaunchKernelGGL(trs_step_kernel<true>, dim3(blocks), dim3(kTsThreads), 0, st, L, ldg, m,
+                       s, 0, B, R);
+    MAOMI_CHECK_LAUNCH();
+  }
+  // backward sweep: the last panel's diagonal block, then one launch per panel p >= 1
+  hipLaunchKernelGGL(trs_step_kernel<false>, dim3(1), dim3(kTsThreads), 0, st, L, ldg, m,
+                     P - 1, 1, B, R);
+  MAOMI_CHECK_LAUNCH();
+  for (int64_t p = P - 1; p >= 1; --p) {
+    hipLaunchKernelGGL(trs_step_kernel<false>, dim3((unsigned)p), dim3(kTsThreads), 0, st, L,
+                       ldg, m, p, 0, B, R);
+    MAOMI_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(v4_schur_kernel, dim3(1), dim3(kFinThreads), 0, st, G, ldg, n, V, T, B, y1,
+                     (int)C, w);
+  MAOMI_CHECK_LAUNCH();
+  return MAOMI_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // maomi_v4_apply_split_f64 without a caller-supplied partial-sum buffer, for callers that cannot
 // allocate device memory.  The rows * Q * C partial sums live in `scratch`, a file-scope device

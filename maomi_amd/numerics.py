@@ -365,10 +365,14 @@ def v4_factor(points, G, *, keep_copy=True, reflectors=None):
 
 def v4_finish(f, v):
     """w with G w = v from a v4_factor result; None when the positive-definite route does not
-    apply (the caller then solves by LU).  One host read of the factorisation's pivot report."""
+    apply (the caller then solves by LU).  One host read of the factorisation's pivot report.
+    On the device the whole finish (Q^T v, the triangular sweeps, the 3 x 3 Schur complement and
+    Q x) is maomi_v4_finish_f64."""
     if f is None or f.V is None or int(f.info) != 0:
         return None
     v = v.reshape(f.V.shape[0], -1)
+    if f.G.is_cuda:
+        return _v4_finish_device(f, v)
     V, T = f.V, f.T
     y = v - V @ (T.T @ (V.T @ v))  # Q^T v
     X = torch.cholesky_solve(torch.cat([f.A12.T, y[3:]], 1), f.L)  # A22^-1 [A12^T y2]
@@ -376,6 +380,35 @@ def v4_finish(f, v):
     a = torch.linalg.solve(f.A11 - f.A12 @ Z, y[:3] - f.A12 @ z)  # 3 x 3 Schur complement
     x = torch.cat([a, z - Z @ a], 0)
     return (x - V @ (T @ (V.T @ x))).contiguous()  # Q x
+
+
+def _v4_finish_device(f, v):
+    """maomi_v4_finish_f64 on the factored system held in f.G (value columns in chunks of at
+    most MAOMI_V4_FINISH_MAX_C)."""
+    lib = nat.lib()
+    G, n = f.G, f.G.shape[0]
+    if G.stride(1) != 1 or G.dtype != torch.float64:
+        raise ValueError("the factored v4 system must be float64 with unit column stride")
+    V, T = f.V.contiguous(), f.T.contiguous()
+    v = v.to(torch.float64).contiguous()
+    C = v.shape[1]
+    w = torch.empty_like(v)
+    stream = nat.stream_handle(G.device)
+    for c0 in range(0, C, _V4_FINISH_MAX_C):
+        vc = v[:, c0:c0 + _V4_FINISH_MAX_C].contiguous()
+        k = vc.shape[1]
+        wc = w if k == C else torch.empty_like(vc)
+        work = torch.empty(int(lib.maomi_v4_finish_f64_workspace(n, k)) // 8,
+                           dtype=torch.float64, device=G.device)
+        nat.check(lib.maomi_v4_finish_f64(nat.ptr(G), G.stride(0), n, nat.ptr(V), nat.ptr(T),
+                                          nat.ptr(vc), k, nat.ptr(wc), nat.ptr(work), stream),
+                  "v4 finish")
+        if k != C:
+            w[:, c0:c0 + k] = wc
+    return w
+
+
+_V4_FINISH_MAX_C = 13  # MAOMI_V4_FINISH_MAX_C
 
 
 def v4_solve(points, G, v):

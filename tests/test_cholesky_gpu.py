@@ -113,3 +113,44 @@ def _green_pairs(q, p):
         g = d2 * (np.log(d2) * 0.5 - 1)
     g[d2 == 0] = 0
     return g
+
+
+def _finish_reference(Gf, V, T, v):
+    """w from the device's factored system by LAPACK routines on the host: potrs (scipy's
+    cho_solve) for the triangular sweeps, LU for the 3 x 3 Schur complement."""
+    from scipy.linalg import cho_solve
+    L = np.tril(Gf[3:, 3:])
+    A11, A12 = Gf[:3, :3], Gf[:3, 3:]
+    y = v - V @ (T.T @ (V.T @ v))
+    X = cho_solve((L, True), np.concatenate([A12.T, y[3:]], 1))
+    Z, z = X[:, :3], X[:, 3:]
+    a = np.linalg.solve(A11 - A12 @ Z, y[:3] - A12 @ z)
+    x = np.concatenate([a, z - Z @ a], 0)
+    return x - V @ (T @ (V.T @ x))
+
+
+@pytest.mark.parametrize("n,C", [(4, 1), (5, 2), (67, 3), (68, 1), (131, 13), (700, 15),
+                                 (4225, 3)])
+def test_v4_finish_matches_lapack_on_the_same_factor(gpu_lib, n, C):
+    """maomi_v4_finish_f64 (triangular sweeps in 64-row panels, pivoted 3 x 3 Schur solve,
+    projections) against LAPACK on the host applied to the device's own factorisation: panel
+    edges (m = n - 3 = 1, 2, 64, 65, 128), more value columns than one call takes (C = 15 runs
+    as 13 + 2), and the 4225-point lattice."""
+    import torch
+    from maomi_amd.numerics import v4_factor, v4_finish
+    rng = np.random.default_rng(n + C)
+    p = _points("config2", rng) if n == 4225 else rng.random((n, 2)) * 50
+    G = _green(p)
+    v = rng.standard_normal((n, C))
+    tG = torch.as_tensor(G, device="cuda")
+    f = v4_factor(torch.as_tensor(p, device="cuda"), tG, keep_copy=False)
+    assert f is not None and int(f.info) == 0
+    print(f"[v4 finish] n={n} C={C}", flush=True)
+    w = v4_finish(f, torch.as_tensor(v, device="cuda")).cpu().numpy()
+    ref = _finish_reference(f.G.cpu().numpy(), f.V.cpu().numpy(), f.T.cpu().numpy(), v)
+    err = np.linalg.norm(w - ref) / np.linalg.norm(ref)
+    eta = np.linalg.norm(G @ w - v) / (np.linalg.norm(G) * np.linalg.norm(w))
+    print(f"  against LAPACK on the same factor {err:.2e}, backward error {eta:.2e}", flush=True)
+    assert err < 1e-9
+    assert eta <= 50 * np.finfo(np.float64).eps
+

@@ -115,3 +115,14 @@ def test_library_binds_to_torchs_hip_runtime():
                          cwd=str(HEADER.parents[1])).stdout
     runtimes = eval(out.strip().splitlines()[-1])
     assert len(runtimes) == 1, runtimes
+
+
+def test_v4_finish_rejects_bad_sizes_before_any_gpu_work():
+    lib = _native.lib()
+    assert lib.maomi_v4_finish_f64_workspace(3, 1) == -1
+    assert lib.maomi_v4_finish_f64_workspace(10, 14) == -1
+    assert lib.maomi_v4_finish_f64_workspace(10, 2) == (7 * 5 + 6) * 8
+    one = 8
+    assert lib.maomi_v4_finish_f64(one, 10, 10, one, one, one, 14, one, one, None) == -1
+    assert b"1 <= C <= 13" in lib.maomi_last_error()
+    assert lib.maomi_v4_finish_f64(one, 9, 10, one, one, one, 2, one, one, None) == -1
