@@ -1,16 +1,15 @@
 // Persistent per-site split-step propagation for the plain optical stage.
 //
-// One block owns one site at a time and runs all of its slices: the field
-// lives in a block-private workspace (L2/MALL resident), each slice is one
-// row pass and one column pass over LDS tiles, and every pass reads and
-// writes the field exactly once.  The row pass fuses: inverse FFT along y of
-// the previous slice, the screen exp(i phi) * edge gathered straight from the
-// phase volume, |u|^4 trapezoid accumulation into the site's mask, and the
-// forward FFT along y of the next slice.  The column pass fuses the forward
-// FFT along x, the Fresnel factor Q (separable, 1/N^2 folded in) and the
-// inverse FFT along x.  Reference behaviour: maomi/numerics.py:156-210
-// (fresnel, constant pitch), maomi/backends/cuda/batched.py:157-168 (_stage),
-// maomi/optics.py:327-340 (trapezoid over saved planes).
+// One block owns one site at a time and runs all of its slices: the field lives in a
+// block-private workspace (L2/MALL resident) between passes over LDS tiles of lines.  The
+// Fresnel step is separable and its two axis factors commute, so one pass along an axis
+// finishes the previous slice's step along that axis, applies the plane's screen exp(i phi)
+// times the edge window (read straight from the phase volume) and its |u|^4 trapezoid term,
+// and starts the next slice's step along that axis (forward FFT, 1-D Fresnel factor with 1/N
+// folded in, inverse FFT); the passes alternate between rows and columns.  A site of S slices
+// takes S + 1 passes, each reading and writing the field once.  Reference behaviour:
+// maomi/numerics.py:156-210 (fresnel, constant pitch), maomi/backends/cuda/batched.py:157-168
+// (_stage), maomi/optics.py:327-340 (trapezoid over saved planes).
 #include "fft_reg.hpp"
 #include "maomi_amd.h"
 #include "screen_trig.hpp"
@@ -43,14 +42,17 @@ template <> struct ClassIdx<256> { static constexpr int v = 2; };
 template <> struct ClassIdx<512> { static constexpr int v = 3; };
 
 // Block shapes: NT threads transform L = NT / max(N1, N2) lines per LDS tile.
-// Exchange-buffer slots for field sides up to NMAX.
+// Exchange-buffer slots for field sides up to NMAX: two buffers per side up to 256 (the
+// passes alternate between them, see line_pass), one for 512.
 template <int NT, int NMAX>
 constexpr int lds_slots() {
-  int m = fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2);
-  if (NMAX >= 256) m = fftk::cmax(m, fftk::Tile<256, NT>::LDS2);
+  int m = 2 * fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2);
+  if (NMAX >= 256) m = fftk::cmax(m, 2 * fftk::Tile<256, NT>::LDS2);
   if (NMAX >= 512) m = fftk::cmax(m, fftk::Tile<512, NT>::LDS2);
   return m;
 }
+// Field sides whose tiles get two exchange buffers.
+template <int N> constexpr bool kDoubleBuffer = N <= 256;
 constexpr int kNmax = 512;
 
 struct SitesArgs {
@@ -181,228 +183,208 @@ struct SiteCtx {
   float2* fld;
 };
 
-// Kinds of row pass: the first slice starts from the entry field and applies its screen,
-// middle slices transform back, apply the screen and transform forward, the last slice only
-// transforms back and accumulates; a site with no screens is first and last at once.  Each
-// kind is its own instantiation, so the per-pixel loops carry no slice-dependent branches.
-enum RowKind { kFirst = 0, kMiddle = 1, kLast = 2, kOnly = 3 };
+// Passes of a site.  The Fresnel step is separable, P = Px Py, with Px (along x) and Py (along
+// y) commuting, so the planes u_{p+1} = P T_p u_p (T_p: screen times edge window of plane p)
+// can be produced by one pass per plane, alternating the axis:
+//   pass 0 (x): T_0 and mask(0), then Px;
+//   pass p (y for odd p, x for even p): the other half step of slice p-1 (the field is then
+//     u_p), T_p and mask(p), then this axis' half step of slice p;
+//   pass S: the remaining half step of slice S-1, mask(S).
+// Pass p + 1 starts with the half step that pass p did not do, and the two halves of a slice
+// commute, so pass p + 1 holds u_{p+1}.  Every pass moves the field through memory once: S
+// slices cost S + 1 field round trips and 4 S line transforms.
+enum PassKind { kHead = 0, kMid = 1, kTail = 2, kOnly = 3 };
+enum Axis { kAxisX = 0, kAxisY = 1 };  // the coordinate that varies along a line
 
-// Row pass of slice s: [IFFT_y] -> screen, |u|^4 trapezoid -> [FFT_y].
+// A line pass over tiles of L lines.  A tile runs: (A) stage 1 of the forward transform,
+// global -> xa; (B) stage 2, the 1-D Fresnel factor and stage 1 of the inverse, xa -> xb;
+// then for a middle pass (C) stage 2 of the inverse, the pointwise work of plane p and stage 1
+// of the forward transform, xb -> xa, and (B) again; finally stage 2 of the inverse,
+// xb -> global.  Stage-1 and final jobs hold the same elements: offsets n1 + N1 m, m < N2,
+// along line l1.  With two exchange buffers (field sides up to 256) a stage writes the buffer
+// the previous stage did not read, so a tile needs one barrier per stage boundary; with one
+// buffer each write also waits for the reads of that buffer.
 // Barriers inside a pass only order LDS; each thread's mask pixels and the phase volume need
 // no cross-thread ordering.
-template <int N, int NT, int MODE, int KIND, class SM>
-__device__ __forceinline__ void row_pass(const SitesArgs& a, const SiteCtx& c, int s, int tid,
-                                         SM& sm, double& esum) {
+template <int N, int NT, int MODE, int AXIS, int KIND, class SM>
+__device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, int p, int tid,
+                                          SM& sm, double& esum) {
   constexpr bool half = MODE == kScreens16, PSF = MODE == kScreens32Psf;
   constexpr bool cplx = MODE == kScreensC64;
-  constexpr bool FIRST = KIND == kFirst || KIND == kOnly, LAST = KIND == kLast || KIND == kOnly;
+  constexpr bool FIRST = KIND == kHead || KIND == kOnly, LAST = KIND == kTail || KIND == kOnly;
+  constexpr bool ROWS = AXIS == kAxisY;  // lines are rows x, elements along y
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
   constexpr int J2 = (L * N2 + NT - 1) / NT;  // stage-2 jobs per thread
   constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
-  static_assert(!hoist || J2 == 1, "hoisted loads serve one stage-2 job per thread");
-  const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
-  const float sscale = half && !LAST ? a.slice_scale[s] : 1.0f;
-  const int64_t pslice = (int64_t)s * a.H;
+  constexpr bool DB = kDoubleBuffer<N>;
+  float2* const xa = sm.x;
+  float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
+  // element m of a stage-1 job lies at (x0 + DX m, y0 + DY m); GS is its index stride in the
+  // field, the mask and the entry field, PS in a slice of the phase volume
+  constexpr int DX = ROWS ? 0 : N1, DY = ROWS ? N1 : 0;
+  constexpr int64_t GS = (int64_t)DX * N + DY;
+  const int64_t PS = (int64_t)DX * a.W + DY;
   const bool p1 = tid < L * N1;
-  const int l1 = tid / N1, n1 = tid % N1;
-#pragma unroll 1
-  for (int t = 0; t < N / L; ++t) {
-    const int row0 = t * L;
-    // this tile's screen phases and mask pixels (first stage-2 job), requested before stage 1
-    // so that their latency overlaps the stage-1 loads, transform and exchange
-    float phv[cplx ? 1 : N1], mkv[N1];
-    float2 scv[cplx ? N1 : 1];
-    if (hoist && tid < L * N2) {
-      const int x = row0 + tid / N2, k2 = tid % N2;
-      const float* mrow = c.mask + (int64_t)x * N + k2;
-      const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) {
-        if (!LAST) {
-          if (cplx)
-            scv[k1] = a.screen[prow + N2 * k1];
-          else
-            phv[k1] = half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                           : a.phase[prow + N2 * k1];
-        }
-        if (!FIRST) mkv[k1] = mrow[N2 * k1];
-      }
-    }
-    if (!FIRST) {
-      if (p1) {  // stage 1 of the inverse transform along y
-        float2 v1[N2];
-        const float2* src = c.fld + (int64_t)(row0 + l1) * N + n1;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v1[n2] = src[N1 * n2];
-        fftk::fft<N2, true>(v1);
-        float2* dst = sm.x + l1 * LS + n1 * (N2 + 1);
-#pragma unroll
-        for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v1[j] : cmulc(v1[j], sm.tw[n1 * j]);
-      }
-      lds_barrier();
-    }
-    // Stage-2 jobs, one at a time: job jj covers lines [jj NT / N2, (jj + 1) NT / N2) of the
-    // tile, so its results can be stored (after a barrier for the reads of its lines) while
-    // the next job reads the other lines.
-#pragma unroll
-    for (int jj = 0; jj < J2; ++jj) {
-      const int jb = tid + jj * NT;
-      const bool p2 = jb < L * N2;
-      const int l = jb / N2, k2 = jb % N2;
-      const int x = row0 + l;
-      float2 v[N1];
-      if (p2) {
-        float* mrow = c.mask + (int64_t)x * N + k2;
-        const int64_t prow = (pslice + c.ox + x) * a.W + c.oy + k2;
-        if (FIRST) {
-          const float2* src = c.base + (int64_t)x * N + k2;
-#pragma unroll
-          for (int k1 = 0; k1 < N1; ++k1) v[k1] = src[N2 * k1];
-        } else {
-          const float2* src = sm.x + l * LS + k2;
-#pragma unroll
-          for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-          fftk::fft<N1, true>(v);
-        }
-        const float ex = sm.eg[x];
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-          const int y = k2 + N2 * k1;
-          if (!LAST) {
-            float sn, cs;
-            if (cplx) {
-              const float2 e = hoist ? scv[k1] : a.screen[prow + N2 * k1];
-              cs = e.x;
-              sn = e.y;
-            } else {
-              const float phi = hoist ? phv[k1]
-                                : half ? __half2float(a.phase16[prow + N2 * k1]) * sscale
-                                       : a.phase[prow + N2 * k1];
-              screen_sincos(phi, &sn, &cs);
-            }
-            const float sg = ex * sm.eg[y];
-            v[k1] = cmul(v[k1], make_float2(cs * sg, sn * sg));
-          }
-          const float pw = v[k1].x * v[k1].x + v[k1].y * v[k1].y;
-          if (PSF && s >= a.psf_z0 && s < a.psf_z0 + a.psf_nz) {
-            const int cx = x - a.psf_c0, cy = y - a.psf_c0;
-            if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
-              a.psf_out[(((int64_t)(s - a.psf_z0) * a.P + c.site) * a.psf_w + cx) * a.psf_w +
-                        cy] = pw * pw;
-          }
-          const float m = wgt * (pw * pw);
-          const float acc = FIRST ? m : (hoist ? mkv[k1] : mrow[N2 * k1]) + m;
-          mrow[N2 * k1] = acc;
-          if (LAST) esum += (double)acc;
-        }
-        if (!LAST) {  // stage 1 of the forward transform along y (factors swapped)
-          fftk::fft<N1, false>(v);
-#pragma unroll
-          for (int j = 1; j < N1; ++j) v[j] = cmul(v[j], sm.tw[k2 * j]);
-        }
-      }
-      lds_barrier();
-      if (!LAST && p2) {
-        float2* dst = sm.x + l * LS + k2 * (N1 + 1);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) dst[j] = v[j];
-      }
-    }
-    if (!LAST) {
-      lds_barrier();
-      if (p1) {  // stage 2 of the forward transform along y
-        float2 w[N2];
-        const float2* src = sm.x + l1 * LS + n1;
-#pragma unroll
-        for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
-        fftk::fft<N2, false>(w);
-        float2* dst = c.fld + (int64_t)(row0 + l1) * N + n1;
-#pragma unroll
-        for (int k1 = 0; k1 < N2; ++k1) dst[N1 * k1] = w[k1];
-      }
-      lds_barrier();
-    }
-  }
-  __syncthreads();  // the row pass's stores are read by other threads in the column pass
-}
+  const int l1 = ROWS ? tid / N1 : tid % L, n1 = ROWS ? tid % N1 : tid / L;
+  const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
+  const float sscale = half && !LAST ? a.slice_scale[p] : 1.0f;
 
-// Column pass: FFT_x -> * Q (separable, 1/N^2 folded in) -> IFFT_x.
-template <int N, int NT, class SM>
-__device__ __forceinline__ void column_pass(const SiteCtx& c, int tid, SM& sm) {
-  using T = fftk::Tile<N, NT>;
-  constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
-  constexpr int J2 = (L * N2 + NT - 1) / NT;  // stage-2 jobs per thread
-  const bool c1 = tid < L * N1;
-  const int lc = tid % L, nc = tid / L;
-  constexpr bool prefetch = kPrefetchCols && N <= 256;  // N = 512: 32-point lines, no room
-  float2 cpre[N2];  // next tile's stage-1 input, requested after this tile's stage 2
-#pragma unroll 1
-  for (int t = 0; t < N / L; ++t) {
-    const int col0 = t * L;
-    if (c1) {
-      float2 v[N2];
-      if (prefetch && t > 0) {
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v[n2] = cpre[n2];
-      } else {
-        const float2* src = c.fld + (int64_t)nc * N + col0 + lc;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v[n2] = src[(int64_t)N1 * N * n2];
-      }
-      fftk::fft<N2, false>(v);
-      float2* dst = sm.x + lc * LS + nc * (N2 + 1);
-#pragma unroll
-      for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[nc * j]);
-    }
-    lds_barrier();
-    // Stage-2 jobs, one at a time.  With one job per thread the line index runs fastest; with
-    // two, job jj covers lines [jj NT / N2, (jj + 1) NT / N2), so its results can be stored
-    // (after a barrier for the reads of its lines) while the next job reads the other lines.
+  // stage B: xin -> xout (stage 2 of the forward transform, Fresnel factor, stage 1 of the
+  // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two, so
+  // with one buffer its stores wait only for the reads of its own lines
+  auto stage_b = [&](const float2* xin, float2* xout) {
 #pragma unroll
     for (int jj = 0; jj < J2; ++jj) {
       const int jb = tid + jj * NT;
       const bool p2 = jb < L * N2;
-      const int l = J2 == 1 ? jb % L : jb / N2, k2 = J2 == 1 ? jb / L : jb % N2;
+      const int l = (ROWS || J2 > 1) ? jb / N2 : jb % L;
+      const int k2 = (ROWS || J2 > 1) ? jb % N2 : jb / L;
       float2 v[N1];
       if (p2) {
-        const float2* src = sm.x + l * LS + k2;
+        const float2* src = xin + l * LS + k2;
 #pragma unroll
         for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
         fftk::fft<N1, false>(v);
-        const float2 qc = sm.q[col0 + l];
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], cmul(sm.q[k2 + N2 * k1], qc));
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
         fftk::fft<N1, true>(v);
 #pragma unroll
         for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
       }
-      if (prefetch && jj == J2 - 1 && c1 && t + 1 < N / L) {
-        const float2* src = c.fld + (int64_t)nc * N + col0 + L + lc;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) cpre[n2] = src[(int64_t)N1 * N * n2];
-      }
-      lds_barrier();
+      if (!DB) lds_barrier();  // every read of the single buffer is done
       if (p2) {
-        float2* dst = sm.x + l * LS + k2 * (N1 + 1);
+        float2* dst = xout + l * LS + k2 * (N1 + 1);
 #pragma unroll
         for (int j = 0; j < N1; ++j) dst[j] = v[j];
       }
     }
-    lds_barrier();
-    if (c1) {
-      float2 w[N2];
-      const float2* src = sm.x + lc * LS + nc;
+  };
+
+#pragma unroll 1
+  for (int t = 0; t < N / L; ++t) {
+    const int line = t * L + l1;
+    const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
+    const int64_t g0 = (int64_t)x0 * N + y0;
+    const int64_t ph0 = ((int64_t)p * a.H + c.ox + x0) * a.W + c.oy + y0;
+    // plane p's screen phases and mask pixels, requested at the start of the tile so that their
+    // latency overlaps the transforms before the pointwise stage
+    float phv[cplx ? 1 : N2], mkv[N2];
+    float2 scv[cplx ? N2 : 1];
+    if (hoist && p1) {
 #pragma unroll
-      for (int n = 0; n < N2; ++n) w[n] = src[n * (N1 + 1)];
-      fftk::fft<N2, true>(w);
-      float2* dst = c.fld + (int64_t)nc * N + col0 + lc;
+      for (int m = 0; m < N2; ++m) {
+        if (!LAST) {
+          if (cplx)
+            scv[m] = a.screen[ph0 + PS * m];
+          else
+            phv[m] = half ? __half2float(a.phase16[ph0 + PS * m]) * sscale : a.phase[ph0 + PS * m];
+        }
+        if (!FIRST) mkv[m] = c.mask[g0 + GS * m];
+      }
+    }
+    // pointwise work of plane p on a stage-1 job's elements: screen times edge window (planes
+    // before the last), |u|^4 into the mask with the trapezoid weight, focal-plane output
+    auto pointwise = [&](float2 (&v)[N2]) {
+      const float efix = sm.eg[ROWS ? x0 : y0];
 #pragma unroll
-      for (int k1 = 0; k1 < N2; ++k1) dst[(int64_t)N1 * N * k1] = w[k1];
+      for (int m = 0; m < N2; ++m) {
+        const int x = x0 + DX * m, y = y0 + DY * m;
+        if (!LAST) {
+          float sn, cs;
+          if (cplx) {
+            const float2 e = hoist ? scv[m] : a.screen[ph0 + PS * m];
+            cs = e.x;
+            sn = e.y;
+          } else {
+            const float phi = hoist ? phv[m]
+                              : half ? __half2float(a.phase16[ph0 + PS * m]) * sscale
+                                     : a.phase[ph0 + PS * m];
+            screen_sincos(phi, &sn, &cs);
+          }
+          const float sg = efix * sm.eg[ROWS ? y : x];
+          v[m] = cmul(v[m], make_float2(cs * sg, sn * sg));
+        }
+        const float pw = v[m].x * v[m].x + v[m].y * v[m].y;
+        if (PSF && p >= a.psf_z0 && p < a.psf_z0 + a.psf_nz) {
+          const int cx = x - a.psf_c0, cy = y - a.psf_c0;
+          if (cx >= 0 && cx < a.psf_w && cy >= 0 && cy < a.psf_w)
+            a.psf_out[(((int64_t)(p - a.psf_z0) * a.P + c.site) * a.psf_w + cx) * a.psf_w + cy] =
+                pw * pw;
+        }
+        const float mm = wgt * (pw * pw);
+        const float acc = FIRST ? mm : (hoist ? mkv[m] : c.mask[g0 + GS * m]) + mm;
+        c.mask[g0 + GS * m] = acc;
+        if (LAST) esum += (double)acc;
+      }
+    };
+
+    if constexpr (KIND == kOnly) {  // no slices: the entry plane is first and last
+      if (p1) {
+        float2 v[N2];
+#pragma unroll
+        for (int m = 0; m < N2; ++m) v[m] = c.base[g0 + GS * m];
+        pointwise(v);
+      }
+      continue;
+    }
+    // (A) the pass's first half step: stage 1 of the forward transform
+    if (p1) {
+      float2 v[N2];
+      const float2* src = (FIRST ? c.base : c.fld) + g0;
+#pragma unroll
+      for (int m = 0; m < N2; ++m) v[m] = src[GS * m];
+      if (FIRST) pointwise(v);
+      fftk::fft<N2, false>(v);
+      float2* dst = xa + l1 * LS + n1 * (N2 + 1);
+#pragma unroll
+      for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[n1 * j]);
     }
     lds_barrier();
+    stage_b(xa, xb);
+    lds_barrier();
+    if constexpr (KIND == kMid) {
+      // (C) stage 2 of the inverse (the field is now u_p), plane p's pointwise work, stage 1 of
+      // this axis' next half step
+      float2 v[N2];
+      if (p1) {
+        const float2* src = xb + l1 * LS + n1;
+#pragma unroll
+        for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
+        fftk::fft<N2, true>(v);
+        pointwise(v);
+        fftk::fft<N2, false>(v);
+#pragma unroll
+        for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], sm.tw[n1 * j]);
+      }
+      if (!DB) lds_barrier();
+      if (p1) {
+        float2* dst = xa + l1 * LS + n1 * (N2 + 1);
+#pragma unroll
+        for (int j = 0; j < N2; ++j) dst[j] = v[j];
+      }
+      lds_barrier();
+      stage_b(xa, xb);
+      lds_barrier();
+    }
+    // final stage: stage 2 of the inverse, then the field to memory or the last plane's mask
+    if (p1) {
+      float2 v[N2];
+      const float2* src = xb + l1 * LS + n1;
+#pragma unroll
+      for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
+      fftk::fft<N2, true>(v);
+      if (LAST) {
+        pointwise(v);
+      } else {
+        float2* dst = c.fld + g0;
+#pragma unroll
+        for (int m = 0; m < N2; ++m) dst[GS * m] = v[m];
+      }
+    }
+    if (!DB) lds_barrier();  // the final reads precede the next tile's stage-1 stores
   }
-  __syncthreads();  // the column pass's stores are read by other threads in the next row pass
+  __syncthreads();  // this pass's field stores are read by other threads in the next pass
 }
 
 template <int N, int NT, int MODE, class SM>
@@ -425,15 +407,19 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
                   a.mask + a.mask_off[site], a.scratch + (int64_t)blockIdx.x * a.scratch_stride};
   double esum = 0.0;
   if (S < 1) {
-    row_pass<N, NT, MODE, kOnly>(a, c, 0, tid, sm, esum);
+    line_pass<N, NT, MODE, kAxisX, kOnly>(a, c, 0, tid, sm, esum);
   } else {
-    row_pass<N, NT, MODE, kFirst>(a, c, 0, tid, sm, esum);
-    column_pass<N, NT>(c, tid, sm);
-    for (int s = 1; s < S; ++s) {
-      row_pass<N, NT, MODE, kMiddle>(a, c, s, tid, sm, esum);
-      column_pass<N, NT>(c, tid, sm);
+    line_pass<N, NT, MODE, kAxisX, kHead>(a, c, 0, tid, sm, esum);
+    for (int p = 1; p < S; ++p) {
+      if (p & 1)
+        line_pass<N, NT, MODE, kAxisY, kMid>(a, c, p, tid, sm, esum);
+      else
+        line_pass<N, NT, MODE, kAxisX, kMid>(a, c, p, tid, sm, esum);
     }
-    row_pass<N, NT, MODE, kLast>(a, c, S, tid, sm, esum);
+    if (S & 1)
+      line_pass<N, NT, MODE, kAxisY, kTail>(a, c, S, tid, sm, esum);
+    else
+      line_pass<N, NT, MODE, kAxisX, kTail>(a, c, S, tid, sm, esum);
   }
   const double tot = block_sum_double(esum, sm.red);
   if (tid == 0) a.energy[site] = tot;
