@@ -23,12 +23,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// The column pass requests the next tile's field lines once the current tile's second stage is
-// done, so that the load latency overlaps the exchange and third stage.  (The same prefetch in
-// the row pass measured slower: its registers are taken by the hoisted screen and mask loads.)
-constexpr bool kPrefetchCols = true;
+// A middle pass requests the next tile's field lines right after the current tile's first
+// stage, so that their latency overlaps the rest of the tile.
+#ifndef MAOMI_PREFETCH
+#define MAOMI_PREFETCH 1
+#endif
+constexpr bool kPrefetch = MAOMI_PREFETCH;
 
-// The row pass requests a tile's screen phases and mask pixels before its first stage.
+// A pass requests a tile's screen phases and mask pixels at the start of the tile.
 constexpr bool kHoist = true;
 
 __device__ __forceinline__ void screen_sincos(float phi, float* s, float* c) {
@@ -52,7 +54,10 @@ constexpr int lds_slots() {
   return m;
 }
 // Field sides whose tiles get two exchange buffers.
-template <int N> constexpr bool kDoubleBuffer = N <= 256;
+#ifndef MAOMI_DOUBLE_BUFFER
+#define MAOMI_DOUBLE_BUFFER 1
+#endif
+template <int N> constexpr bool kDoubleBuffer = MAOMI_DOUBLE_BUFFER && N <= 256;
 constexpr int kNmax = 512;
 
 struct SitesArgs {
@@ -218,6 +223,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   constexpr int J2 = (L * N2 + NT - 1) / NT;  // stage-2 jobs per thread
   constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
   constexpr bool DB = kDoubleBuffer<N>;
+  constexpr bool prefetch = kPrefetch && KIND == kMid && N <= 256;
   float2* const xa = sm.x;
   float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
   // element m of a stage-1 job lies at (x0 + DX m, y0 + DY m); GS is its index stride in the
@@ -229,6 +235,8 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   const int l1 = ROWS ? tid / N1 : tid % L, n1 = ROWS ? tid % N1 : tid / L;
   const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
   const float sscale = half && !LAST ? a.slice_scale[p] : 1.0f;
+  constexpr int64_t TS = ROWS ? (int64_t)L * N : L;  // field index step from tile to tile
+  float2 vpre[prefetch ? N2 : 1];                    // the next tile's stage-1 input
 
   // stage B: xin -> xout (stage 2 of the forward transform, Fresnel factor, stage 1 of the
   // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two, so
@@ -332,13 +340,22 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
     if (p1) {
       float2 v[N2];
       const float2* src = (FIRST ? c.base : c.fld) + g0;
+      if (prefetch && t > 0) {
 #pragma unroll
-      for (int m = 0; m < N2; ++m) v[m] = src[GS * m];
+        for (int m = 0; m < N2; ++m) v[m] = vpre[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < N2; ++m) v[m] = src[GS * m];
+      }
       if (FIRST) pointwise(v);
       fftk::fft<N2, false>(v);
       float2* dst = xa + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
       for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[n1 * j]);
+      if (prefetch && t + 1 < N / L) {
+#pragma unroll
+        for (int m = 0; m < N2; ++m) vpre[m] = src[TS + GS * m];
+      }
     }
     lds_barrier();
     stage_b(xa, xb);
