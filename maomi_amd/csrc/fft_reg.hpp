@@ -249,21 +249,24 @@ template <int N> struct Split;
 template <> struct Split<64> { static constexpr int N1 = 8, N2 = 8; };
 template <> struct Split<128> { static constexpr int N1 = 8, N2 = 16; };
 template <> struct Split<256> { static constexpr int N1 = 16, N2 = 16; };
-template <> struct Split<512> { static constexpr int N1 = 16, N2 = 32; };
+template <> struct Split<512> { static constexpr int N1 = 32, N2 = 16; };
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
-// Tile geometry for a block of NT threads transforming L lines of length N.
-// N = 512 (16 x 32) sizes the tile by the smaller factor, so that its 32-point stages fill the
-// block; the 16-point stages then take two jobs per thread.
+// Tile geometry for a block of NT threads transforming L lines of length N: L is sized by the
+// smaller factor, so that both stages have at least NT jobs; the stage with the larger factor
+// then gives each thread J1 (stage 1: N1 jobs per line, N2-point transforms) or J2 (stage 2:
+// N2 jobs per line, N1-point transforms) jobs.
 template <int N, int NT>
 struct Tile {
   static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
-  static constexpr int JMAX = cmax(N1, N2), JMIN = N1 < N2 ? N1 : N2;
-  static constexpr int L = N >= 512 ? NT / JMIN : NT / JMAX;  // lines per tile
+  static constexpr int L = NT / cmin(N1, N2);              // lines per tile
+  static constexpr int J1 = L * N1 / NT, J2 = L * N2 / NT;  // jobs per thread, 1 or 2
   static constexpr int LS = cmax(N1 * (N2 + 1), N2 * (N1 + 1)) | 1;  // odd line stride
   static constexpr int LDS2 = L * LS;                     // float2 slots of exchange buffer
   static_assert(N % L == 0, "tile must divide the field");
+  static_assert(J1 * NT == L * N1 && J2 * NT == L * N2, "whole jobs per thread");
 };
 
 }  // namespace fftk

@@ -33,6 +33,48 @@ constexpr bool kPrefetch = MAOMI_PREFETCH;
 // A pass requests a tile's screen phases and mask pixels at the start of the tile.
 constexpr bool kHoist = true;
 
+// Raw buffer access for the per-site arrays: a resource over [p, p + bytes) held in scalar
+// registers, a 32-bit byte offset per lane in one VGPR and a wave-uniform byte offset (an SGPR
+// or the instruction's immediate), so that the 16 or 32 strided elements of a line job cost no
+// 64-bit address registers each.  Loads outside the range return zero; stores are dropped.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 raw buffer, 32-bit data format
+// The base and size are wave-uniform by construction (per-site or per-block values); they are
+// read from the first lane so that the resource is known to live in scalar registers (a
+// resource the compiler takes for divergent costs a waterfall loop around every access).
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, uint32_t bytes) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  void* base = reinterpret_cast<void*>(((uint64_t)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)__builtin_amdgcn_readfirstlane(bytes),
+                                           kRawBufferWord3);
+}
+__device__ __forceinline__ float2 ld2(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float ld1(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ float ldh(Rsrc r, uint32_t voff, uint32_t soff) {
+  const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+  return __half2float(__ushort_as_half(b));
+}
+__device__ __forceinline__ void st2(Rsrc r, uint32_t voff, uint32_t soff, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void st1(Rsrc r, uint32_t voff, uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// With two jobs per thread, the scheduler would interleave them across a whole tile (one basic
+// block) and hold both jobs' loads and transforms at once; this keeps each job's code together.
+template <int J>
+__device__ __forceinline__ void job_fence() {
+  if constexpr (J > 1) __builtin_amdgcn_sched_barrier(0);
+}
+
 __device__ __forceinline__ void screen_sincos(float phi, float* s, float* c) {
   trig::sincos_screen(phi, s, c);  // |phi| < 2^17, checked on the host
 }
@@ -219,24 +261,43 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   constexpr bool FIRST = KIND == kHead || KIND == kOnly, LAST = KIND == kTail || KIND == kOnly;
   constexpr bool ROWS = AXIS == kAxisY;  // lines are rows x, elements along y
   using T = fftk::Tile<N, NT>;
-  constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS;
-  constexpr int J2 = (L * N2 + NT - 1) / NT;  // stage-2 jobs per thread
-  constexpr bool hoist = kHoist && N <= 256;  // N = 512 has no registers to spare
+  constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS, J1 = T::J1, J2 = T::J2;
   constexpr bool DB = kDoubleBuffer<N>;
-  constexpr bool prefetch = kPrefetch && KIND == kMid && N <= 256;
+  // a tile's screen phases and mask pixels are requested at its start, and a middle pass
+  // requests the next tile's field lines after its first stage (one stage-1 job per thread;
+  // the focal-plane launches keep those registers for the output addressing)
+  constexpr bool hoist = kHoist && J1 == 1 && N <= 256;
+  constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF;
   float2* const xa = sm.x;
   float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
   // element m of a stage-1 job lies at (x0 + DX m, y0 + DY m); GS is its index stride in the
   // field, the mask and the entry field, PS in a slice of the phase volume
   constexpr int DX = ROWS ? 0 : N1, DY = ROWS ? N1 : 0;
   constexpr int64_t GS = (int64_t)DX * N + DY;
+  constexpr int64_t TS = ROWS ? (int64_t)L * N : L;  // field index step from tile to tile
   const int64_t PS = (int64_t)DX * a.W + DY;
-  const bool p1 = tid < L * N1;
-  const int l1 = ROWS ? tid / N1 : tid % L, n1 = ROWS ? tid % N1 : tid / L;
+  // byte strides between a job's elements: field and entry field, mask, screen slice
+  constexpr uint32_t GS8 = (uint32_t)GS * 8, GS4 = (uint32_t)GS * 4;
+  constexpr int SE = cplx ? 8 : half ? 2 : 4;  // bytes per screen element
+  const uint32_t PSB = (uint32_t)PS * SE;
+  const Rsrc r_fld = make_rsrc(c.fld, (uint32_t)N * N * 8);
+  const Rsrc r_base = make_rsrc(c.base, (uint32_t)N * N * 8);
+  const Rsrc r_mask = make_rsrc(c.mask, (uint32_t)N * N * 4);
+  const int64_t slice = LAST ? 0 : (int64_t)p * a.H * a.W;  // plane p's screen
+  const Rsrc r_scr = make_rsrc(cplx ? (const void*)(a.screen + slice)
+                               : half ? (const void*)(a.phase16 + slice)
+                                      : (const void*)(a.phase + slice),
+                               (uint32_t)(a.H * a.W * SE));
   const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
   const float sscale = half && !LAST ? a.slice_scale[p] : 1.0f;
-  constexpr int64_t TS = ROWS ? (int64_t)L * N : L;  // field index step from tile to tile
-  float2 vpre[prefetch ? N2 : 1];                    // the next tile's stage-1 input
+  // stage-1 job jj of this thread: line l1 of the tile, offset n1 (rows: consecutive lanes take
+  // consecutive offsets; columns: consecutive lines, for coalesced global access)
+  auto job1 = [&](int jj, int& l1, int& n1) {
+    const int jb = tid + jj * NT;
+    l1 = ROWS ? jb / N1 : jb % L;
+    n1 = ROWS ? jb % N1 : jb / L;
+  };
+  float2 vpre[prefetch ? N2 : 1];  // the next tile's stage-1 input
 
   // stage B: xin -> xout (stage 2 of the forward transform, Fresnel factor, stage 1 of the
   // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two, so
@@ -245,55 +306,57 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
 #pragma unroll
     for (int jj = 0; jj < J2; ++jj) {
       const int jb = tid + jj * NT;
-      const bool p2 = jb < L * N2;
       const int l = (ROWS || J2 > 1) ? jb / N2 : jb % L;
       const int k2 = (ROWS || J2 > 1) ? jb % N2 : jb / L;
       float2 v[N1];
-      if (p2) {
-        const float2* src = xin + l * LS + k2;
+      const float2* src = xin + l * LS + k2;
 #pragma unroll
-        for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-        fftk::fft<N1, false>(v);
+      for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+      fftk::fft<N1, false>(v);
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
-        fftk::fft<N1, true>(v);
+      for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
+      fftk::fft<N1, true>(v);
 #pragma unroll
-        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
-      }
+      for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
       if (!DB) lds_barrier();  // every read of the single buffer is done
-      if (p2) {
-        float2* dst = xout + l * LS + k2 * (N1 + 1);
+      float2* dst = xout + l * LS + k2 * (N1 + 1);
 #pragma unroll
-        for (int j = 0; j < N1; ++j) dst[j] = v[j];
-      }
+      for (int j = 0; j < N1; ++j) dst[j] = v[j];
+      job_fence<J2>();
     }
   };
 
 #pragma unroll 1
   for (int t = 0; t < N / L; ++t) {
-    const int line = t * L + l1;
-    const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
-    const int64_t g0 = (int64_t)x0 * N + y0;
-    const int64_t ph0 = ((int64_t)p * a.H + c.ox + x0) * a.W + c.oy + y0;
-    // plane p's screen phases and mask pixels, requested at the start of the tile so that their
-    // latency overlaps the transforms before the pointwise stage
+    // plane p's screen phases and mask pixels for the (single) stage-1 job, requested at the
+    // start of the tile so that their latency overlaps the transforms before the pointwise
     float phv[cplx ? 1 : N2], mkv[N2];
     float2 scv[cplx ? N2 : 1];
-    if (hoist && p1) {
+    if (hoist) {
+      int l1, n1;
+      job1(0, l1, n1);
+      const int line = t * L + l1;
+      const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
+      const uint32_t g0 = (uint32_t)(x0 * N + y0);
+      const uint32_t ph0 = (uint32_t)((c.ox + x0) * a.W + c.oy + y0) * SE;
 #pragma unroll
       for (int m = 0; m < N2; ++m) {
         if (!LAST) {
           if (cplx)
-            scv[m] = a.screen[ph0 + PS * m];
+            scv[m] = ld2(r_scr, ph0, PSB * m);
           else
-            phv[m] = half ? __half2float(a.phase16[ph0 + PS * m]) * sscale : a.phase[ph0 + PS * m];
+            phv[m] = half ? ldh(r_scr, ph0, PSB * m) * sscale : ld1(r_scr, ph0, PSB * m);
         }
-        if (!FIRST) mkv[m] = c.mask[g0 + GS * m];
+        if (!FIRST) mkv[m] = ld1(r_mask, g0 * 4, GS4 * m);
       }
     }
-    // pointwise work of plane p on a stage-1 job's elements: screen times edge window (planes
-    // before the last), |u|^4 into the mask with the trapezoid weight, focal-plane output
-    auto pointwise = [&](float2 (&v)[N2]) {
+    // pointwise work of plane p on stage-1 job (l1, n1): screen times edge window (planes before
+    // the last), |u|^4 into the mask with the trapezoid weight, focal-plane output
+    auto pointwise = [&](int l1, int n1, float2 (&v)[N2]) {
+      const int line = t * L + l1;
+      const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
+      const uint32_t g0 = (uint32_t)(x0 * N + y0);
+      const uint32_t ph0 = (uint32_t)((c.ox + x0) * a.W + c.oy + y0) * SE;
       const float efix = sm.eg[ROWS ? x0 : y0];
 #pragma unroll
       for (int m = 0; m < N2; ++m) {
@@ -301,13 +364,13 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         if (!LAST) {
           float sn, cs;
           if (cplx) {
-            const float2 e = hoist ? scv[m] : a.screen[ph0 + PS * m];
+            const float2 e = hoist ? scv[m] : ld2(r_scr, ph0, PSB * m);
             cs = e.x;
             sn = e.y;
           } else {
             const float phi = hoist ? phv[m]
-                              : half ? __half2float(a.phase16[ph0 + PS * m]) * sscale
-                                     : a.phase[ph0 + PS * m];
+                              : half ? ldh(r_scr, ph0, PSB * m) * sscale
+                                     : ld1(r_scr, ph0, PSB * m);
             screen_sincos(phi, &sn, &cs);
           }
           const float sg = efix * sm.eg[ROWS ? y : x];
@@ -321,41 +384,55 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
                 pw * pw;
         }
         const float mm = wgt * (pw * pw);
-        const float acc = FIRST ? mm : (hoist ? mkv[m] : c.mask[g0 + GS * m]) + mm;
-        c.mask[g0 + GS * m] = acc;
+        const float acc = FIRST ? mm : (hoist ? mkv[m] : ld1(r_mask, g0 * 4, GS4 * m)) + mm;
+        st1(r_mask, g0 * 4, GS4 * m, acc);
         if (LAST) esum += (double)acc;
       }
     };
+    auto field_byte = [&](int l1, int n1) {  // byte offset of a stage-1 job's element 0
+      const int line = t * L + l1;
+      return (uint32_t)(ROWS ? line * N + n1 : n1 * N + line) * 8u;
+    };
 
     if constexpr (KIND == kOnly) {  // no slices: the entry plane is first and last
-      if (p1) {
-        float2 v[N2];
 #pragma unroll
-        for (int m = 0; m < N2; ++m) v[m] = c.base[g0 + GS * m];
-        pointwise(v);
+      for (int jj = 0; jj < J1; ++jj) {
+        int l1, n1;
+        job1(jj, l1, n1);
+        float2 v[N2];
+        const uint32_t f0 = field_byte(l1, n1);
+#pragma unroll
+        for (int m = 0; m < N2; ++m) v[m] = ld2(r_base, f0, GS8 * m);
+        pointwise(l1, n1, v);
+        job_fence<J1>();
       }
       continue;
     }
     // (A) the pass's first half step: stage 1 of the forward transform
-    if (p1) {
+#pragma unroll
+    for (int jj = 0; jj < J1; ++jj) {
+      int l1, n1;
+      job1(jj, l1, n1);
       float2 v[N2];
-      const float2* src = (FIRST ? c.base : c.fld) + g0;
+      const Rsrc r_in = FIRST ? r_base : r_fld;
+      const uint32_t f0 = field_byte(l1, n1);
       if (prefetch && t > 0) {
 #pragma unroll
         for (int m = 0; m < N2; ++m) v[m] = vpre[m];
       } else {
 #pragma unroll
-        for (int m = 0; m < N2; ++m) v[m] = src[GS * m];
+        for (int m = 0; m < N2; ++m) v[m] = ld2(r_in, f0, GS8 * m);
       }
-      if (FIRST) pointwise(v);
+      if (FIRST) pointwise(l1, n1, v);
       fftk::fft<N2, false>(v);
       float2* dst = xa + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
       for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[n1 * j]);
       if (prefetch && t + 1 < N / L) {
 #pragma unroll
-        for (int m = 0; m < N2; ++m) vpre[m] = src[TS + GS * m];
+        for (int m = 0; m < N2; ++m) vpre[m] = ld2(r_fld, f0 + (uint32_t)TS * 8, GS8 * m);
       }
+      job_fence<J1>();
     }
     lds_barrier();
     stage_b(xa, xb);
@@ -363,41 +440,52 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
     if constexpr (KIND == kMid) {
       // (C) stage 2 of the inverse (the field is now u_p), plane p's pointwise work, stage 1 of
       // this axis' next half step
-      float2 v[N2];
-      if (p1) {
+      float2 v[J1][N2];
+#pragma unroll
+      for (int jj = 0; jj < J1; ++jj) {
+        int l1, n1;
+        job1(jj, l1, n1);
         const float2* src = xb + l1 * LS + n1;
 #pragma unroll
-        for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
-        fftk::fft<N2, true>(v);
-        pointwise(v);
-        fftk::fft<N2, false>(v);
+        for (int n = 0; n < N2; ++n) v[jj][n] = src[n * (N1 + 1)];
+        fftk::fft<N2, true>(v[jj]);
+        pointwise(l1, n1, v[jj]);
+        fftk::fft<N2, false>(v[jj]);
 #pragma unroll
-        for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], sm.tw[n1 * j]);
+        for (int j = 1; j < N2; ++j) v[jj][j] = cmul(v[jj][j], sm.tw[n1 * j]);
+        job_fence<J1>();
       }
-      if (!DB) lds_barrier();
-      if (p1) {
+      if (!DB) lds_barrier();  // with one buffer, every job's reads precede the stores
+#pragma unroll
+      for (int jj = 0; jj < J1; ++jj) {
+        int l1, n1;
+        job1(jj, l1, n1);
         float2* dst = xa + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
-        for (int j = 0; j < N2; ++j) dst[j] = v[j];
+        for (int j = 0; j < N2; ++j) dst[j] = v[jj][j];
       }
       lds_barrier();
       stage_b(xa, xb);
       lds_barrier();
     }
     // final stage: stage 2 of the inverse, then the field to memory or the last plane's mask
-    if (p1) {
+#pragma unroll
+    for (int jj = 0; jj < J1; ++jj) {
+      int l1, n1;
+      job1(jj, l1, n1);
       float2 v[N2];
       const float2* src = xb + l1 * LS + n1;
 #pragma unroll
       for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
       fftk::fft<N2, true>(v);
       if (LAST) {
-        pointwise(v);
+        pointwise(l1, n1, v);
       } else {
-        float2* dst = c.fld + g0;
+        const uint32_t f0 = field_byte(l1, n1);
 #pragma unroll
-        for (int m = 0; m < N2; ++m) dst[GS * m] = v[m];
+        for (int m = 0; m < N2; ++m) st2(r_fld, f0, GS8 * m, v[m]);
       }
+      job_fence<J1>();
     }
     if (!DB) lds_barrier();  // the final reads precede the next tile's stage-1 stores
   }
