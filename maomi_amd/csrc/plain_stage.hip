@@ -30,8 +30,13 @@ __device__ __forceinline__ void lds_barrier() {
 #endif
 constexpr bool kPrefetch = MAOMI_PREFETCH;
 
-// A pass requests a tile's screen phases and mask pixels at the start of the tile.
+// A pass requests a tile's screen phases and mask pixels at the start of the tile; a middle
+// pass requests those of the next tile right after the current tile's pointwise stage.
 constexpr bool kHoist = true;
+#ifndef MAOMI_HOIST_AHEAD
+#define MAOMI_HOIST_AHEAD 1
+#endif
+constexpr bool kHoistAhead = MAOMI_HOIST_AHEAD;
 
 // Raw buffer access for the per-site arrays: a resource over [p, p + bytes) held in scalar
 // registers, a 32-bit byte offset per lane in one VGPR and a wave-uniform byte offset (an SGPR
@@ -267,6 +272,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   // requests the next tile's field lines after its first stage (one stage-1 job per thread;
   // the focal-plane launches keep those registers for the output addressing)
   constexpr bool hoist = kHoist && J1 == 1 && N <= 256;
+  constexpr bool ahead = kHoistAhead && hoist && KIND == kMid && !PSF;
   constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF;
   float2* const xa = sm.x;
   float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
@@ -326,29 +332,42 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
     }
   };
 
+  // plane p's screen phases and mask pixels of the (single) stage-1 job of tile tt
+  float phn[ahead && !cplx ? N2 : 1], mkn[ahead ? N2 : 1];
+  float2 scn[ahead && cplx ? N2 : 1];
+  auto load_pointwise = [&](int tt, float* ph, float2* sc, float* mk) {
+    int l1, n1;
+    job1(0, l1, n1);
+    const int line = tt * L + l1;
+    const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
+    const uint32_t g0 = (uint32_t)(x0 * N + y0);
+    const uint32_t ph0 = (uint32_t)((c.ox + x0) * a.W + c.oy + y0) * SE;
+#pragma unroll
+    for (int m = 0; m < N2; ++m) {
+      if (!LAST) {
+        if (cplx)
+          sc[m] = ld2(r_scr, ph0, PSB * m);
+        else
+          ph[m] = half ? ldh(r_scr, ph0, PSB * m) * sscale : ld1(r_scr, ph0, PSB * m);
+      }
+      if (!FIRST) mk[m] = ld1(r_mask, g0 * 4, GS4 * m);
+    }
+  };
+
 #pragma unroll 1
   for (int t = 0; t < N / L; ++t) {
-    // plane p's screen phases and mask pixels for the (single) stage-1 job, requested at the
-    // start of the tile so that their latency overlaps the transforms before the pointwise
+    // requested at the start of the tile (or during the previous tile) so that their latency
+    // overlaps the transforms before the pointwise stage
     float phv[cplx ? 1 : N2], mkv[N2];
     float2 scv[cplx ? N2 : 1];
-    if (hoist) {
-      int l1, n1;
-      job1(0, l1, n1);
-      const int line = t * L + l1;
-      const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
-      const uint32_t g0 = (uint32_t)(x0 * N + y0);
-      const uint32_t ph0 = (uint32_t)((c.ox + x0) * a.W + c.oy + y0) * SE;
+    if (ahead && t > 0) {
 #pragma unroll
       for (int m = 0; m < N2; ++m) {
-        if (!LAST) {
-          if (cplx)
-            scv[m] = ld2(r_scr, ph0, PSB * m);
-          else
-            phv[m] = half ? ldh(r_scr, ph0, PSB * m) * sscale : ld1(r_scr, ph0, PSB * m);
-        }
-        if (!FIRST) mkv[m] = ld1(r_mask, g0 * 4, GS4 * m);
+        if (cplx) scv[m] = scn[m]; else phv[m] = phn[m];
+        mkv[m] = mkn[m];
       }
+    } else if (hoist) {
+      load_pointwise(t, phv, scv, mkv);
     }
     // pointwise work of plane p on stage-1 job (l1, n1): screen times edge window (planes before
     // the last), |u|^4 into the mask with the trapezoid weight, focal-plane output
@@ -450,6 +469,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         for (int n = 0; n < N2; ++n) v[jj][n] = src[n * (N1 + 1)];
         fftk::fft<N2, true>(v[jj]);
         pointwise(l1, n1, v[jj]);
+        if (ahead && t + 1 < N / L) load_pointwise(t + 1, phn, scn, mkn);
         fftk::fft<N2, false>(v[jj]);
 #pragma unroll
         for (int j = 1; j < N2; ++j) v[jj][j] = cmul(v[jj][j], sm.tw[n1 * j]);
