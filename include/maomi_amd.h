@@ -110,8 +110,9 @@ int64_t maomi_sites_default_grid(void);
 /* Plan object of the sites launch.  Replaces the captured batches of OpticalPlan.prepare_graph
  * / evaluate_graph (maomi/backends/cuda/batched.py:103-146): one HIP graph holds the whole
  * run's launch (every site, mixed field sides and depths), and
- * the autotuner picks the grid (sites in flight), the block shape (256 or 512 threads: 16- or
- * 32-line LDS tiles of the line transforms), the site ordering and the schedule (static
+ * the autotuner picks the grid (sites in flight), the block shape (256 or 512 threads; a tile of
+ * the line transforms holds block_threads / min(N1, N2) lines of a field side N = N1 N2, e.g.
+ * 32 or 16 lines of 256 = 16 x 16), the site ordering and the schedule (static
  * striding of the order list over the blocks, or the work counter) by timing captured
  * candidates.  The plan owns the orderings, the per-block workspace, the work counter and the
  * hipGraphExec; the descriptor's other buffers stay the caller's and are read at each replay

@@ -505,6 +505,21 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   __syncthreads();  // this pass's field stores are read by other threads in the next pass
 }
 
+// Pass of plane p.  In a focal-plane launch only the passes of the output planes carry the
+// focal-window stores (and their registers); the other passes are the FP32-screen passes.
+template <int N, int NT, int MODE, int AXIS, int KIND, class SM>
+__device__ __forceinline__ void plane_pass(const SitesArgs& a, const SiteCtx& c, int p, int tid,
+                                           SM& sm, double& esum) {
+  if constexpr (MODE == kScreens32Psf) {
+    if (p >= a.psf_z0 && p < a.psf_z0 + a.psf_nz)
+      line_pass<N, NT, kScreens32Psf, AXIS, KIND>(a, c, p, tid, sm, esum);
+    else
+      line_pass<N, NT, kScreens32, AXIS, KIND>(a, c, p, tid, sm, esum);
+  } else {
+    line_pass<N, NT, MODE, AXIS, KIND>(a, c, p, tid, sm, esum);
+  }
+}
+
 template <int N, int NT, int MODE, class SM>
 __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
   constexpr int C = ClassIdx<N>::v;
@@ -525,19 +540,19 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
                   a.mask + a.mask_off[site], a.scratch + (int64_t)blockIdx.x * a.scratch_stride};
   double esum = 0.0;
   if (S < 1) {
-    line_pass<N, NT, MODE, kAxisX, kOnly>(a, c, 0, tid, sm, esum);
+    plane_pass<N, NT, MODE, kAxisX, kOnly>(a, c, 0, tid, sm, esum);
   } else {
-    line_pass<N, NT, MODE, kAxisX, kHead>(a, c, 0, tid, sm, esum);
+    plane_pass<N, NT, MODE, kAxisX, kHead>(a, c, 0, tid, sm, esum);
     for (int p = 1; p < S; ++p) {
       if (p & 1)
-        line_pass<N, NT, MODE, kAxisY, kMid>(a, c, p, tid, sm, esum);
+        plane_pass<N, NT, MODE, kAxisY, kMid>(a, c, p, tid, sm, esum);
       else
-        line_pass<N, NT, MODE, kAxisX, kMid>(a, c, p, tid, sm, esum);
+        plane_pass<N, NT, MODE, kAxisX, kMid>(a, c, p, tid, sm, esum);
     }
     if (S & 1)
-      line_pass<N, NT, MODE, kAxisY, kTail>(a, c, S, tid, sm, esum);
+      plane_pass<N, NT, MODE, kAxisY, kTail>(a, c, S, tid, sm, esum);
     else
-      line_pass<N, NT, MODE, kAxisX, kTail>(a, c, S, tid, sm, esum);
+      plane_pass<N, NT, MODE, kAxisX, kTail>(a, c, S, tid, sm, esum);
   }
   const double tot = block_sum_double(esum, sm.red);
   if (tid == 0) a.energy[site] = tot;

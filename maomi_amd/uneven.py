@@ -6,7 +6,8 @@ launch of the mixed-size persistent kernel (maomi_persistent_sites with
 uniform_n = 0); blocks take sites from a work counter in the order of a list
 sorted by cost (S N^2 log N, largest first) and the launch is captured once in
 a HIP graph and replayed.  An autotuner picks the grid size, the tile shape
-(256- or 512-thread blocks: 16- or 32-line LDS tiles), the ordering policy and
+(256- or 512-thread blocks, whose LDS tiles hold block_threads / min(N1, N2) lines of
+a field side N = N1 N2), the ordering policy and
 the schedule (work counter or static striding) by timing the captured
 candidates on the device.
 """
