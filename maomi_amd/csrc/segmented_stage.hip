@@ -384,13 +384,97 @@ __device__ __forceinline__ float powk(float p, int order) {
   return order == 1 ? p : order == 2 ? p * p : p * p * p;
 }
 
+// Tile lines of the field into A (line-major, A[l N + j] = element j of line l) and back.
+// Rows: line l is row r0 + l.  Columns: line l is column r0 + l, read and written with the line
+// index fastest so that consecutive threads touch consecutive addresses.  Loads go in batches
+// of kBatch per thread, each batch loaded into registers before any of it is stored.
+__device__ __forceinline__ void load_tile(float2* A, const float2* fld, bool rows, int r0, int nl,
+                                          int N) {
+  if (rows) {
+    const float2* f = fld + (int64_t)r0 * N;
+    for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+      float2 ld[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if (b0 + k * NT < nl * N) ld[k] = f[b0 + k * NT];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if (b0 + k * NT < nl * N) A[b0 + k * NT] = ld[k];
+    }
+  } else {
+    const int di = NT / nl, dc = NT - di * nl;
+    int i = threadIdx.x / nl, l = threadIdx.x - (threadIdx.x / nl) * nl;
+    for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+      float2 ld[kBatch];
+      int dst[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        if (b0 + k * NT < nl * N) {
+          ld[k] = fld[(int64_t)i * N + r0 + l];
+          dst[k] = l * N + i;
+        }
+        i += di;
+        l += dc;
+        if (l >= nl) {
+          l -= nl;
+          ++i;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if (b0 + k * NT < nl * N) A[dst[k]] = ld[k];
+    }
+  }
+}
+
+__device__ __forceinline__ void store_tile(const float2* A, float2* fld, bool rows, int r0, int nl,
+                                           int N) {
+  if (rows) {
+    for (int it = threadIdx.x; it < nl * N; it += NT) fld[(int64_t)r0 * N + it] = A[it];
+  } else {
+    const int di = NT / nl, dc = NT - di * nl;
+    int i = threadIdx.x / nl, l = threadIdx.x - (threadIdx.x / nl) * nl;
+    for (int it = threadIdx.x; it < nl * N; it += NT, i += di, l += dc) {
+      if (l >= nl) {
+        l -= nl;
+        ++i;
+      }
+      fld[(int64_t)i * N + r0 + l] = A[l * N + i];
+    }
+  }
+}
+
+// One half step of nl lines along their axis: forward transform, the 1-D Fresnel factor
+// q[k] (1/N folded in) at frequency k, inverse transform.
+template <int CLS>
+__device__ __forceinline__ void half_step(float2* A, float2* B, int nl, int N, int N1, int N2,
+                                          const float2* T, const float2* q) {
+  line_transform<CLS>(A, B, nl, N, N1, N2, T, false);
+  const int dl = NT / N, dk = NT - dl * N;
+  for (int it = threadIdx.x, k = threadIdx.x - (threadIdx.x / N) * N; it < nl * N;
+       it += NT, k += dk) {
+    if (k >= N) k -= N;
+    A[it] = cmul(A[it], q[k]);
+  }
+  lds_barrier();
+  line_transform<CLS>(A, B, nl, N, N1, N2, T, true);
+}
+
 // One segment of one site.  src: first-plane source (complex), with index shift
 // (sx, sy) and side src_n; dst (may be null): receives the last plane (n x n).
+//
+// The Fresnel step is separable, Q(kx, ky) = q(kx) q(ky), so a step is Px Py with Px (along x)
+// and Py (along y) commuting, and the planes u_{s+1} = Px Py T_s u_s (T_s: screen times edge
+// window) are produced by one pass per plane, alternating rows and columns: pass s finishes
+// the half step of slice s - 1 along its axis (the field is then u_s), does plane s's pointwise
+// work (screen, tails, focal staging, the segment's last plane) and starts slice s's half step
+// along the same axis.  Pass 0 starts from the source, pass S only finishes.  Each pass moves
+// the field through the workspace once: S + 1 round trips for S steps.
 template <int CLS>
 __device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox, int64_t oy, const float2* src,
                             int src_n, int sx, int sy, float2* fld, float2* dst, float* tail,
                             float* stage, float* tmp, int site, float2* A, float2* B, float2* T,
-                            float2* Qt, float* E, double* red) {
+                            float2* Qp, float2* Qn, float* E, double* red) {
   const Seg& g = a.seg[k];
   const int N = g.n, N1 = g.n1, N2 = g.n2f, S = g.steps;
   // lines per tile for this side: as many as the line buffers (sized for the largest side)
@@ -407,24 +491,27 @@ __device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox,
     for (int i = threadIdx.x; i < a.n2 * a.n2; i += NT) tail[i] = 0.f;
     __syncthreads();  // other threads accumulate into these pixels
   }
-  // element it = l * N + y of a tile, stepped by NT without divisions
+  // element it = l * N + j of a tile, stepped by NT without divisions
   const int tl0 = threadIdx.x / N, ty0 = threadIdx.x - tl0 * N;
   const int dl = NT / N, dy = NT - dl * N;
   for (int s = 0; s <= S; ++s) {
     const bool first = s == 0, last = s == S;
+    const bool rows = (s & 1) == 0;  // pass s runs along rows for even s, columns for odd s
     const bool save_tail = a.tails && (top || bot) && s >= first_saved;
     const bool save_foc = foc && s < S;
     const float trap = (s == first_saved || s == S) ? 0.5f : 1.0f;
     const float ax = save_tail ? (top ? a.ax_top[s - a.cut] : a.ax_bot[s])
                                : (save_foc ? a.ax_focus[s] : 1.f);
     double prof = 0.0;
-    if (!last) {
-      for (int i = threadIdx.x; i < N; i += NT) Qt[i] = g.q[(int64_t)g.qidx[s] * N + i];
+    // q of the step that this pass finishes (s - 1) and of the step it starts (s)
+    for (int i = threadIdx.x; i < N; i += NT) {
+      if (!first) Qp[i] = g.q[(int64_t)g.qidx[s - 1] * N + i];
+      if (!last) Qn[i] = g.q[(int64_t)g.qidx[s] * N + i];
     }
-    // ---------------- row pass
+    __syncthreads();
     for (int r0 = 0; r0 < N; r0 += L) {
       const int nl = min(L, N - r0);
-      if (first) {
+      if (first) {  // the segment's source, cropped or zero-padded (rows: s = 0 is a row pass)
         for (int it = threadIdx.x; it < nl * N; it += NT) {
           const int l = it / N, y = it % N;
           const int xs = r0 + l + sx, ys = y + sy;
@@ -433,49 +520,66 @@ __device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox,
         }
         lds_barrier();
       } else {
-        const float2* f = fld + (int64_t)r0 * N;
-        for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
-          float2 ld[kBatch];
-#pragma unroll
-          for (int k = 0; k < kBatch; ++k)
-            if (b0 + k * NT < nl * N) ld[k] = f[b0 + k * NT];
-#pragma unroll
-          for (int k = 0; k < kBatch; ++k)
-            if (b0 + k * NT < nl * N) A[b0 + k * NT] = ld[k];
-        }
+        load_tile(A, fld, rows, r0, nl, N);
         lds_barrier();
-        line_transform<CLS>(A, B, nl, N, N1, N2, T, true);
+        half_step<CLS>(A, B, nl, N, N1, N2, T, Qp);
       }
       // the tile's phases, batched the same way, wait in B (free between the transforms) at
-      // the indices of their elements; each thread reads back only its own
+      // the indices of their elements
       float* PH = (float*)B;
       if (!last) {
-        int l = tl0, y = ty0;
-        for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
-          float ph[kBatch];
+        const float* ph_s = g.phase + (int64_t)s * g.H * g.W;
+        if (rows) {  // each thread reads back only its own elements: no barrier needed
+          int l = tl0, y = ty0;
+          for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+            float ph[kBatch];
 #pragma unroll
-          for (int k = 0; k < kBatch; ++k) {
-            if (b0 + k * NT < nl * N)
-              ph[k] = g.phase[((int64_t)s * g.H + ox + r0 + l) * g.W + oy + y];
-            l += dl;
-            y += dy;
-            if (y >= N) {
-              y -= N;
-              ++l;
+            for (int k = 0; k < kBatch; ++k) {
+              if (b0 + k * NT < nl * N) ph[k] = ph_s[(ox + r0 + l) * g.W + oy + y];
+              l += dl;
+              y += dy;
+              if (y >= N) {
+                y -= N;
+                ++l;
+              }
             }
-          }
 #pragma unroll
-          for (int k = 0; k < kBatch; ++k)
-            if (b0 + k * NT < nl * N) PH[b0 + k * NT] = ph[k];
+            for (int k = 0; k < kBatch; ++k)
+              if (b0 + k * NT < nl * N) PH[b0 + k * NT] = ph[k];
+          }
+        } else {  // columns: read with the line index fastest, stored at the element's index
+          const int di = NT / nl, dc = NT - di * nl;
+          int i = threadIdx.x / nl, l = threadIdx.x - (threadIdx.x / nl) * nl;
+          for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
+            float ph[kBatch];
+            int at[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+              if (b0 + k * NT < nl * N) {
+                ph[k] = ph_s[(ox + i) * g.W + oy + r0 + l];
+                at[k] = l * N + i;
+              }
+              i += di;
+              l += dc;
+              if (l >= nl) {
+                l -= nl;
+                ++i;
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k)
+              if (b0 + k * NT < nl * N) PH[at[k]] = ph[k];
+          }
+          lds_barrier();
         }
       }
-      int l = tl0, y = ty0;
-      for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, y += dy) {
-        if (y >= N) {
-          y -= N;
+      int l = tl0, j = ty0;
+      for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, j += dy) {
+        if (j >= N) {
+          j -= N;
           ++l;
         }
-        const int x = r0 + l;
+        const int x = rows ? r0 + l : j, y = rows ? j : r0 + l;
         float2 v = A[it];
         if (!last) {
           const float phi = PH[it];
@@ -502,10 +606,10 @@ __device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox,
       }
       lds_barrier();
       if (!last) {
-        line_transform<CLS>(A, B, nl, N, N1, N2, T, false);
-        for (int it = threadIdx.x; it < nl * N; it += NT) fld[(int64_t)r0 * N + it] = A[it];
-        lds_barrier();  // A is refilled by the next tile
+        half_step<CLS>(A, B, nl, N, N1, N2, T, Qn);
+        store_tile(A, fld, rows, r0, nl, N);
       }
+      lds_barrier();  // A is refilled by the next tile
     }
     if (save_tail) {
       const double p = block_sum_double(prof, red);
@@ -518,63 +622,7 @@ __device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox,
       resample2d(stage, a.fx, a.fy, a.mfx, a.bfx, a.X, a.mfy, a.bfy, a.Y, tmp,
                  a.psf + (int64_t)site * a.X * a.Y * b + s, b, a.focal_scale * ax);
     }
-    if (last) break;
-    __syncthreads();  // the row pass's field stores are read by other threads below
-    // ---------------- column pass
-    for (int c0 = 0; c0 < N; c0 += L) {
-      const int nl = min(L, N - c0);
-      const int ti0 = threadIdx.x / nl, tc0 = threadIdx.x - ti0 * nl;
-      const int di = NT / nl, dc = NT - di * nl;
-      {
-        int i = ti0, l = tc0;
-        for (int b0 = threadIdx.x; b0 < nl * N; b0 += kBatch * NT) {
-          float2 ld[kBatch];
-          int dst[kBatch];
-#pragma unroll
-          for (int k = 0; k < kBatch; ++k) {
-            if (b0 + k * NT < nl * N) {
-              ld[k] = fld[(int64_t)i * N + c0 + l];
-              dst[k] = l * N + i;
-            }
-            i += di;
-            l += dc;
-            if (l >= nl) {
-              l -= nl;
-              ++i;
-            }
-          }
-#pragma unroll
-          for (int k = 0; k < kBatch; ++k)
-            if (b0 + k * NT < nl * N) A[dst[k]] = ld[k];
-        }
-      }
-      lds_barrier();
-      line_transform<CLS>(A, B, nl, N, N1, N2, T, false);
-      {
-        int l = tl0, kx = ty0;
-        for (int it = threadIdx.x; it < nl * N; it += NT, l += dl, kx += dy) {
-          if (kx >= N) {
-            kx -= N;
-            ++l;
-          }
-          A[it] = cmul(A[it], cmul(Qt[kx], Qt[c0 + l]));
-        }
-      }
-      lds_barrier();
-      line_transform<CLS>(A, B, nl, N, N1, N2, T, true);
-      {
-        int i = ti0, l = tc0;
-        for (int it = threadIdx.x; it < nl * N; it += NT, i += di, l += dc) {
-          if (l >= nl) {
-            l -= nl;
-            ++i;
-          }
-          fld[(int64_t)i * N + c0 + l] = A[l * N + i];
-        }
-      }
-      lds_barrier();  // A is refilled by the next tile
-    }
-    __syncthreads();  // the column pass's field stores are read by other threads next
+    __syncthreads();  // this pass's field stores are read by other threads in the next pass
   }
   if ((top || bot) && a.tails) {
     __syncthreads();
@@ -592,8 +640,9 @@ segmented_sites_kernel(SegArgs a) {
   float2* A = seg_lds;
   float2* B = A + a.L * nmax;
   float2* T = B + a.L * a.nb;
-  float2* Qt = T + nmax;
-  float* E = (float*)(Qt + nmax);
+  float2* Qp = T + nmax;
+  float2* Qn = Qp + nmax;
+  float* E = (float*)(Qn + nmax);
   double* red = (double*)(E + nmax + (nmax & 1));
   float2* w = a.work + (int64_t)blockIdx.x * a.work_stride;
   float2* fld = w;
@@ -606,14 +655,14 @@ segmented_sites_kernel(SegArgs a) {
     const int site = a.order_list[i];
     const int64_t ox = a.site_x[site], oy = a.site_y[site];
     // A: base -> planes >= cut (top tail), last plane kept whole in xfer
-    run_segment<CLS>(a, 0, ox, oy, a.base, a.n, 0, 0, fld, xfer, tail, stage, tmp, site, A, B, T, Qt, E,
+    run_segment<CLS>(a, 0, ox, oy, a.base, a.n, 0, 0, fld, xfer, tail, stage, tmp, site, A, B, T, Qp, Qn, E,
                 red);
     // B: crop of A's last plane -> focal planes; last plane kept in xfer (n2 x n2)
     run_segment<CLS>(a, 1, ox, oy, xfer, a.n, a.coff, a.coff, fld, a.tails ? xfer2 : nullptr, tail,
-                stage, tmp, site, A, B, T, Qt, E, red);
+                stage, tmp, site, A, B, T, Qp, Qn, E, red);
     if (a.tails)  // C: B's last plane padded back to n -> bottom tail
       run_segment<CLS>(a, 2, ox, oy, xfer2, a.n2, -a.coff, -a.coff, fld, nullptr, tail, stage, tmp, site,
-                  A, B, T, Qt, E, red);
+                  A, B, T, Qp, Qn, E, red);
   }
 }
 
@@ -730,10 +779,11 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
     const int r2 = fast_split((int)side);
     if (r2 && pad_stride((int)side, r2) > a.nb) a.nb = pad_stride((int)side, r2);
   }
-  // line buffers A (L x n) and B (L x nb) complex plus tables, kept under 60 KiB of LDS
-  a.L = (int)((61440 - 16 * n - 4 * n - 96) / (8 * n + 8 * a.nb));
+  // line buffers A (L x n) and B (L x nb) complex plus tables (twiddles, two Fresnel factors,
+  // edge window), kept under 60 KiB of LDS
+  a.L = (int)((61440 - 24 * n - 4 * n - 96) / (8 * n + 8 * a.nb));
   if (a.L < 1) a.L = 1;
-  const size_t shm = ((size_t)a.L * (n + a.nb) + 2 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
+  const size_t shm = ((size_t)a.L * (n + a.nb) + 3 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
                      (NT / 64) * sizeof(double) + 16;
   int64_t grid = d->grid > 0 ? d->grid : d->P;
   if (grid > d->P) grid = d->P;
