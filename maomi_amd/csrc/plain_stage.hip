@@ -217,7 +217,7 @@ struct Smem {
 // planes of |u|^4 (a.psf_out); complex64 screens exp(i phi) computed once per volume voxel
 // (the same sin/cos as the per-site path, so results are identical; the screen read doubles
 // but the per-pixel sin/cos, shared by every site whose window covers the pixel, is gone).
-enum Mode { kScreens32 = 0, kScreens16 = 1, kScreens32Psf = 2, kScreensC64 = 3 };
+enum Mode { kScreens32 = 0, kScreens16 = 1, kScreens32Psf = 2, kScreensC64 = 3, kScreens32Mixed = 4 };
 
 // Per-site state shared by the passes.
 struct SiteCtx {
@@ -264,9 +264,10 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   // a tile's screen phases and mask pixels are requested at its start, and a middle pass
   // requests the next tile's field lines after its first stage (one stage-1 job per thread;
   // the focal-plane launches keep those registers for the output addressing)
+  constexpr bool lean = MODE == kScreens32Mixed;
   constexpr bool hoist = kHoist && J1 == 1 && N <= 256;
-  constexpr bool ahead = kHoistAhead && hoist && KIND == kMid && !PSF;
-  constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF;
+  constexpr bool ahead = kHoistAhead && hoist && KIND == kMid && !PSF && !lean;
+  constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF && !lean;
   float2* const xa = sm.x;
   float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
   // element m of a stage-1 job lies at (x0 + DX m, y0 + DY m); GS is its index stride in the
@@ -290,11 +291,14 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   const float wgt = (FIRST || LAST) ? 0.5f : 1.0f;
   const float sscale = half && !LAST ? a.slice_scale[p] : 1.0f;
   // stage-1 job jj of this thread: line l1 of the tile, offset n1 (rows: consecutive lanes take
-  // consecutive offsets; columns: consecutive lines, for coalesced global access)
+  // consecutive offsets; columns: consecutive lines, for coalesced global access).  With two
+  // jobs per thread, job jj covers lines [jj L / J1, (jj + 1) L / J1), so that with one buffer
+  // its stores wait only for the reads of its own lines.
+  constexpr int LJ = L / J1;  // lines of one job
   auto job1 = [&](int jj, int& l1, int& n1) {
     const int jb = tid + jj * NT;
-    l1 = ROWS ? jb / N1 : jb % L;
-    n1 = ROWS ? jb % N1 : jb / L;
+    l1 = ROWS ? jb / N1 : jj * LJ + tid % LJ;
+    n1 = ROWS ? jb % N1 : tid / LJ;
   };
   float2 vpre[prefetch ? N2 : 1];  // the next tile's stage-1 input
 
@@ -312,9 +316,12 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
 #pragma unroll
       for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
       fftk::fft<N1, false>(v);
+      if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
+      if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
       fftk::fft<N1, true>(v);
+      if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
       if (!DB) lds_barrier();  // every read of the single buffer is done
@@ -452,30 +459,25 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
     if constexpr (KIND == kMid) {
       // (C) stage 2 of the inverse (the field is now u_p), plane p's pointwise work, stage 1 of
       // this axis' next half step
-      float2 v[J1][N2];
 #pragma unroll
       for (int jj = 0; jj < J1; ++jj) {
         int l1, n1;
         job1(jj, l1, n1);
+        float2 v[N2];
         const float2* src = xb + l1 * LS + n1;
 #pragma unroll
-        for (int n = 0; n < N2; ++n) v[jj][n] = src[n * (N1 + 1)];
-        fftk::fft<N2, true>(v[jj]);
-        pointwise(l1, n1, v[jj]);
+        for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
+        fftk::fft<N2, true>(v);
+        pointwise(l1, n1, v);
         if (ahead && t + 1 < N / L) load_pointwise(t + 1, phn, scn, mkn);
-        fftk::fft<N2, false>(v[jj]);
+        fftk::fft<N2, false>(v);
 #pragma unroll
-        for (int j = 1; j < N2; ++j) v[jj][j] = cmul(v[jj][j], sm.tw[n1 * j]);
-        job_fence<J1>();
-      }
-      if (!DB) lds_barrier();  // with one buffer, every job's reads precede the stores
-#pragma unroll
-      for (int jj = 0; jj < J1; ++jj) {
-        int l1, n1;
-        job1(jj, l1, n1);
+        for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], sm.tw[n1 * j]);
+        if (!DB) lds_barrier();  // with one buffer, the reads of this job's lines are done
         float2* dst = xa + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
-        for (int j = 0; j < N2; ++j) dst[j] = v[jj][j];
+        for (int j = 0; j < N2; ++j) dst[j] = v[j];
+        job_fence<J1>();
       }
       lds_barrier();
       stage_b(xa, xb);
@@ -584,21 +586,27 @@ __global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
   }
 }
 
-// Sites of mixed field sides in one launch (the order list puts large sites first).  The
-// site body is a closure called from both loops (static striding, work counter), so it is not
-// inlined: compiled as a function of its own, the four field sides' values are not held across
-// the site loop.  This form runs the mixed 4225-site launch in 80 ms; inlined into the loops
-// (with the uniform kernels' take_site) the same body runs it in 93-97 ms.
+// One field side's site body of the mixed launch, a function of its own: its registers are
+// allocated for that side alone (one function holding all four sides spilled 480 registers'
+// worth in the 512 side's passes), and none of them is held across the site loop.
+template <int N, int NT, class SM>
+__device__ __noinline__ void mixed_site(const SitesArgs& a, int site, SM& sm) {
+  run_site<N, NT, kScreens32Mixed>(a, site, sm);
+}
+
+// Sites of mixed field sides in one launch (the order list puts large sites first), from both
+// loops (static striding, work counter).  Kept out of line, the site bodies run the mixed
+// 4225-site launch in 80 ms against 93-97 ms inlined into the loops.
 template <int NT>
 __global__ __launch_bounds__(NT) void sites_mixed_kernel(SitesArgs a) {
   __shared__ Smem<NT, kNmax> sm;
   __shared__ int next;
   auto run = [&](int site) {
     switch (a.site_n[site]) {
-      case 64: run_site<64, NT, kScreens32>(a, site, sm); break;
-      case 128: run_site<128, NT, kScreens32>(a, site, sm); break;
-      case 256: run_site<256, NT, kScreens32>(a, site, sm); break;
-      case 512: run_site<512, NT, kScreens32>(a, site, sm); break;
+      case 64: mixed_site<64, NT>(a, site, sm); break;
+      case 128: mixed_site<128, NT>(a, site, sm); break;
+      case 256: mixed_site<256, NT>(a, site, sm); break;
+      case 512: mixed_site<512, NT>(a, site, sm); break;
       default: break;  // rejected on the host
     }
   };
