@@ -266,7 +266,9 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   // the focal-plane launches keep those registers for the output addressing)
   constexpr bool lean = MODE == kScreens32Mixed;
   constexpr bool hoist = kHoist && J1 == 1 && N <= 256;
-  constexpr bool ahead = kHoistAhead && hoist && KIND == kMid && !PSF && !lean;
+  // (FP16 screens: the converted phases held a tile ahead cost more registers than they save
+  // in latency, 88.0 against 85.5 ms on config 4)
+  constexpr bool ahead = kHoistAhead && hoist && KIND == kMid && !PSF && !lean && !half;
   constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF && !lean;
   float2* const xa = sm.x;
   float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
