@@ -123,47 +123,90 @@ __host__ __device__ constexpr int smallest_factor(int n, int f = 2) {
   return f * f > n ? n : (n % f == 0 ? f : smallest_factor(n, f + 1));
 }
 
+// D-point DFT Y[k] = sum_m y[m] W^(m k) with W^j = w[j SW] (the transform's sign already in
+// w).  The terms m and D - m share W^(m k) and its conjugate, so with S_m = y_m + y_(D-m) and
+// D_m = y_m - y_(D-m) (m < D/2),
+//   Y[k]     = y_0 + sum_m S_m c_mk + i sum_m D_m s_mk  (+ y_(D/2) (-1)^k for even D),
+//   Y[D - k] = the same with s_mk negated,
+// so each pair of outputs costs four real h-term sums (h = (D - 1) / 2): 66 operations for 7
+// points against 168 for the direct sums.
+template <int D, int SW, int RW>
+__device__ __forceinline__ void sym_dft(const float2 (&y)[D], const float2 (&w)[RW],
+                                        float2 (&Y)[D]) {
+  if constexpr (D == 1) {
+    Y[0] = y[0];
+  } else if constexpr (D == 2) {
+    Y[0] = cadd(y[0], y[1]);
+    Y[1] = csub(y[0], y[1]);
+  } else {
+    constexpr int H = (D - 1) / 2;
+    constexpr bool EVEN = D % 2 == 0;
+    float2 S[H + 1], Df[H + 1];
+    float2 y0 = y[0], yh = EVEN ? y[D / 2] : make_float2(0.f, 0.f);
+    float2 X0 = y0, XH = y0;
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      S[m] = cadd(y[m], y[D - m]);
+      Df[m] = csub(y[m], y[D - m]);
+      X0 = cadd(X0, S[m]);
+      if constexpr (EVEN) XH = (m & 1) ? csub(XH, S[m]) : cadd(XH, S[m]);
+    }
+    if constexpr (EVEN) {
+      X0 = cadd(X0, yh);
+      XH = ((D / 2) & 1) ? csub(XH, yh) : cadd(XH, yh);
+      Y[D / 2] = XH;
+    }
+    Y[0] = X0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      float ar = y0.x, ai = y0.y, br = 0.f, bi = 0.f;
+      if constexpr (EVEN) {
+        ar = (k & 1) ? ar - yh.x : ar + yh.x;
+        ai = (k & 1) ? ai - yh.y : ai + yh.y;
+      }
+#pragma unroll
+      for (int m = 1; m <= H; ++m) {
+        const float2 t = w[((m * k) % D) * SW];
+        ar = fmaf(S[m].x, t.x, ar);
+        ai = fmaf(S[m].y, t.x, ai);
+        br = fmaf(Df[m].y, t.y, br);
+        bi = fmaf(Df[m].x, t.y, bi);
+      }
+      Y[k] = make_float2(ar - br, ai + bi);
+      Y[D - k] = make_float2(ar + br, ai - bi);
+    }
+  }
+}
+
 // X[k] = sum_m x[m] W_R^(m k) with w[c] = W_R^c.  A composite R = P Q (P its smallest prime
 // factor) runs as P-point DFTs over m = Q n1 + n2, the twiddle W_R^(n2 k1), then Q-point DFTs,
-// X[k1 + P k2] (each level direct; P = 2 as add/subtract); a prime R is one direct DFT.
+// X[k1 + P k2]; every level is a sym_dft.
 template <int R>
 __device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w)[R],
                                           float2 (&X)[R]) {
   constexpr int P = smallest_factor(R), Q = R / P;
   if constexpr (P == R) {
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      float2 acc = x[0];
-#pragma unroll
-      for (int m = 1; m < R; ++m) acc = cmac(acc, x[m], w[(m * k) % R]);
-      X[k] = acc;
-    }
+    sym_dft<R, 1>(x, w, X);
   } else {
     float2 y[R];  // y[n2 P + k1]
 #pragma unroll
     for (int n2 = 0; n2 < Q; ++n2) {
+      float2 t[P], u[P];
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) {
-        float2 acc;
-        if constexpr (P == 2) {
-          acc = k1 == 0 ? cadd(x[n2], x[Q + n2]) : csub(x[n2], x[Q + n2]);
-        } else {
-          acc = x[n2];
+      for (int n1 = 0; n1 < P; ++n1) t[n1] = x[Q * n1 + n2];
+      sym_dft<P, Q>(t, w, u);
 #pragma unroll
-          for (int n1 = 1; n1 < P; ++n1) acc = cmac(acc, x[Q * n1 + n2], w[((n1 * k1) % P) * Q]);
-        }
-        y[n2 * P + k1] = (n2 * k1) % R == 0 ? acc : cmul(acc, w[(n2 * k1) % R]);
-      }
+      for (int k1 = 0; k1 < P; ++k1)
+        y[n2 * P + k1] = (n2 * k1) % R == 0 ? u[k1] : cmul(u[k1], w[(n2 * k1) % R]);
     }
 #pragma unroll
     for (int k1 = 0; k1 < P; ++k1) {
+      float2 t[Q], u[Q];
 #pragma unroll
-      for (int k2 = 0; k2 < Q; ++k2) {
-        float2 acc = y[k1];
+      for (int n2 = 0; n2 < Q; ++n2) t[n2] = y[n2 * P + k1];
+      sym_dft<Q, P>(t, w, u);
 #pragma unroll
-        for (int n2 = 1; n2 < Q; ++n2) acc = cmac(acc, y[n2 * P + k1], w[((n2 * k2) % Q) * P]);
-        X[k1 + P * k2] = acc;
-      }
+      for (int k2 = 0; k2 < Q; ++k2) X[k1 + P * k2] = u[k2];
     }
   }
 }
