@@ -744,6 +744,13 @@ MAOMI_API int maomi_persistent_sites(const maomi_sites_desc* d, maomi_stream_t s
     maomi_set_error("maomi_persistent_sites: bad sizes");
     return MAOMI_EINVAL;
   }
+  // the passes reach a screen slice and a site's arrays through 32-bit byte ranges
+  if (d->H * d->W * (int64_t)sizeof(float2) >= (1LL << 32) ||
+      d->scratch_stride * (int64_t)sizeof(float2) >= (1LL << 32)) {
+    maomi_set_error("maomi_persistent_sites: a screen slice (H x W) or the per-block workspace "
+                    "exceeds 4 GiB");
+    return MAOMI_EINVAL;
+  }
   if (d->uniform_n > 0 && d->scratch_stride < d->uniform_n * d->uniform_n) {
     maomi_set_error("maomi_persistent_sites: scratch_stride %lld is below uniform_n^2 = %lld",
                     (long long)d->scratch_stride, (long long)(d->uniform_n * d->uniform_n));

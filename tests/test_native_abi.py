@@ -86,8 +86,12 @@ def test_sites_launch_and_plan_reject_bad_arguments_before_any_gpu_work():
     assert lib.maomi_sites_plan_replay(None, None) == -1
     assert lib.maomi_sites_plan_capture(None, 4, 512, 1, 0) == -1
     assert b"schedule 0 or 1" in lib.maomi_last_error()
-    d.P = (1 << 30) + 1  # site indices and the work counter would come near int32 overflow
     d.phase16 = d.slice_scale = None
+    d.H, d.W = 40000, 20000  # 8-byte screens of one slice: 6.4e9 bytes, beyond 32-bit ranges
+    assert lib.maomi_persistent_sites(d, None) == -1
+    assert b"exceeds 4 GiB" in lib.maomi_last_error()
+    d.H, d.W = 300, 300
+    d.P = (1 << 30) + 1  # site indices and the work counter would come near int32 overflow
     assert lib.maomi_persistent_sites(d, None) == -1
     assert b"bad sizes" in lib.maomi_last_error()
     assert lib.maomi_sites_plan_destroy(None) == 0
