@@ -254,19 +254,24 @@ template <> struct Split<512> { static constexpr int N1 = 32, N2 = 16; };
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
-// Tile geometry for a block of NT threads transforming L lines of length N: L is sized by the
-// smaller factor, so that both stages have at least NT jobs; the stage with the larger factor
-// then gives each thread J1 (stage 1: N1 jobs per line, N2-point transforms) or J2 (stage 2:
-// N2 jobs per line, N1-point transforms) jobs.
+// Tile geometry for a block of NT threads transforming L lines of length N.  Up to 256 points
+// L is sized by the smaller factor, so that both stages have at least NT jobs; the stage with
+// the larger factor then gives each thread J1 (stage 1: N1 jobs per line, N2-point transforms)
+// or J2 (stage 2: N2 jobs per line, N1-point transforms) jobs.  The 512 side (32 x 16) sizes L
+// by the larger factor instead: stage 1 has NT jobs and stage 2 NT / 2 (A2 < NT: the first
+// A2 / 64 waves transform the 32-point lines, the others go on to the next barrier), and the
+// tile is half as tall, so that two exchange buffers fit in LDS.
 template <int N, int NT>
 struct Tile {
   static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
-  static constexpr int L = NT / cmin(N1, N2);              // lines per tile
-  static constexpr int J1 = L * N1 / NT, J2 = L * N2 / NT;  // jobs per thread, 1 or 2
+  static constexpr int L = NT / (N >= 512 ? cmax(N1, N2) : cmin(N1, N2));  // lines per tile
+  static constexpr int A2 = L * N2;                        // stage-2 jobs of a tile
+  static constexpr int J1 = L * N1 / NT, J2 = cmax(1, A2 / NT);  // jobs per thread, 1 or 2
   static constexpr int LS = cmax(N1 * (N2 + 1), N2 * (N1 + 1)) | 1;  // odd line stride
   static constexpr int LDS2 = L * LS;                     // float2 slots of exchange buffer
   static_assert(N % L == 0, "tile must divide the field");
-  static_assert(J1 * NT == L * N1 && J2 * NT == L * N2, "whole jobs per thread");
+  static_assert(J1 * NT == L * N1, "whole stage-1 jobs per thread");
+  static_assert(J2 * NT == A2 || (A2 < NT && A2 % 64 == 0), "stage 2: whole jobs or waves");
 };
 
 }  // namespace fftk

@@ -86,18 +86,16 @@ template <> struct ClassIdx<128> { static constexpr int v = 1; };
 template <> struct ClassIdx<256> { static constexpr int v = 2; };
 template <> struct ClassIdx<512> { static constexpr int v = 3; };
 
-// Block shapes: NT threads transform L = NT / max(N1, N2) lines per LDS tile.
-// Exchange-buffer slots for field sides up to NMAX: two buffers per side up to 256 (the
-// passes alternate between them, see line_pass), one for 512.
+// Block shapes: NT threads transform the L lines of an LDS tile (fftk::Tile).
+// Exchange-buffer slots for field sides up to NMAX: two buffers per side (the passes alternate
+// between them, see line_pass).
 template <int NT, int NMAX>
 constexpr int lds_slots() {
-  int m = 2 * fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2);
-  if (NMAX >= 256) m = fftk::cmax(m, 2 * fftk::Tile<256, NT>::LDS2);
+  int m = fftk::cmax(fftk::Tile<64, NT>::LDS2, fftk::Tile<128, NT>::LDS2);
+  if (NMAX >= 256) m = fftk::cmax(m, fftk::Tile<256, NT>::LDS2);
   if (NMAX >= 512) m = fftk::cmax(m, fftk::Tile<512, NT>::LDS2);
-  return m;
+  return 2 * m;
 }
-// Field sides whose tiles get two exchange buffers.
-template <int N> constexpr bool kDoubleBuffer = N <= 256;
 constexpr int kNmax = 512;
 
 struct SitesArgs {
@@ -203,14 +201,21 @@ __device__ void audit_site(const SitesArgs& a, int site, int slot, int tid, int 
   }
 }
 
-template <int NT, int NMAX>
-struct Smem {
-  float2 x[lds_slots<NT, NMAX>()];  // line exchange buffer
-  float2 tw[kNmax];           // W_N^j
-  float2 q[kNmax];            // 1-D Fresnel factor (FFT order, times 1/N)
-  float eg[kNmax];            // 1-D edge window
+template <int SLOTS, int TAB, int NT>
+struct SmemT {
+  float2 x[SLOTS];  // line exchange buffer(s)
+  float2 tw[TAB];   // W_N^j
+  float2 q[TAB];    // 1-D Fresnel factor (FFT order, times 1/N)
+  float eg[TAB];    // 1-D edge window
   double red[NT / 64];
 };
+// Mixed launches: room for every field side up to NMAX.
+template <int NT, int NMAX>
+using Smem = SmemT<lds_slots<NT, NMAX>(), kNmax, NT>;
+// Uniform launches: exchange buffers and tables of the one field side N, so that two
+// 256-thread blocks of the 256 side fit one CU (2 x 75 KiB of the 160 KiB).
+template <int N, int NT>
+using SmemU = SmemT<2 * fftk::Tile<N, NT>::LDS2, N, NT>;
 
 // Launch modes, compile-time so that each kernel carries only its own address arithmetic:
 // FP32 screens; FP16 screens with a per-slice scale; FP32 screens plus the focal-window
@@ -246,9 +251,8 @@ enum Axis { kAxisX = 0, kAxisY = 1 };  // the coordinate that varies along a lin
 // then for a middle pass (C) stage 2 of the inverse, the pointwise work of plane p and stage 1
 // of the forward transform, xb -> xa, and (B) again; finally stage 2 of the inverse,
 // xb -> global.  Stage-1 and final jobs hold the same elements: offsets n1 + N1 m, m < N2,
-// along line l1.  With two exchange buffers (field sides up to 256) a stage writes the buffer
-// the previous stage did not read, so a tile needs one barrier per stage boundary; with one
-// buffer each write also waits for the reads of that buffer.
+// along line l1.  A stage writes the exchange buffer that the previous stage did not read, so
+// a tile needs one barrier per stage boundary.
 // Barriers inside a pass only order LDS; each thread's mask pixels and the phase volume need
 // no cross-thread ordering.
 template <int N, int NT, int MODE, int AXIS, int KIND, class SM>
@@ -260,18 +264,21 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   constexpr bool ROWS = AXIS == kAxisY;  // lines are rows x, elements along y
   using T = fftk::Tile<N, NT>;
   constexpr int N1 = T::N1, N2 = T::N2, L = T::L, LS = T::LS, J1 = T::J1, J2 = T::J2;
-  constexpr bool DB = kDoubleBuffer<N>;
   // a tile's screen phases and mask pixels are requested at its start, and a middle pass
   // requests the next tile's field lines after its first stage (one stage-1 job per thread;
   // the focal-plane launches keep those registers for the output addressing)
   constexpr bool lean = MODE == kScreens32Mixed;
-  constexpr bool hoist = kHoist && J1 == 1 && N <= 256;
+  constexpr bool hoist = kHoist && J1 == 1;
   // (FP16 screens: the converted phases held a tile ahead cost more registers than they save
   // in latency, 88.0 against 85.5 ms on config 4)
-  constexpr bool ahead = kHoistAhead && hoist && KIND == kMid && !PSF && !lean && !half;
+  // (the 512 side: neither the next tile's lines nor its phases and mask pixels are held
+  // across the 32-point stage, whose lines take twice the registers; with the lines held, the
+  // uniform 512 launch of 576 sites ran 109.6 against 98.9 ms)
+  constexpr bool ahead =
+      kHoistAhead && hoist && KIND == kMid && !PSF && !lean && !half && N <= 256;
   constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF && !lean;
   float2* const xa = sm.x;
-  float2* const xb = DB ? sm.x + T::LDS2 : sm.x;
+  float2* const xb = sm.x + T::LDS2;
   // element m of a stage-1 job lies at (x0 + DX m, y0 + DY m); GS is its index stride in the
   // field, the mask and the entry field, PS in a slice of the phase volume
   constexpr int DX = ROWS ? 0 : N1, DY = ROWS ? N1 : 0;
@@ -305,9 +312,10 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   float2 vpre[prefetch ? N2 : 1];  // the next tile's stage-1 input
 
   // stage B: xin -> xout (stage 2 of the forward transform, Fresnel factor, stage 1 of the
-  // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two, so
-  // with one buffer its stores wait only for the reads of its own lines
+  // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two.  With
+  // fewer jobs than threads (the 512 side) the first A2 / 64 waves take them.
   auto stage_b = [&](const float2* xin, float2* xout) {
+    if (T::A2 < NT && tid >= T::A2) return;
 #pragma unroll
     for (int jj = 0; jj < J2; ++jj) {
       const int jb = tid + jj * NT;
@@ -326,7 +334,6 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
       if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
-      if (!DB) lds_barrier();  // every read of the single buffer is done
       float2* dst = xout + l * LS + k2 * (N1 + 1);
 #pragma unroll
       for (int j = 0; j < N1; ++j) dst[j] = v[j];
@@ -475,7 +482,6 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         fftk::fft<N2, false>(v);
 #pragma unroll
         for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], sm.tw[n1 * j]);
-        if (!DB) lds_barrier();  // with one buffer, the reads of this job's lines are done
         float2* dst = xa + l1 * LS + n1 * (N2 + 1);
 #pragma unroll
         for (int j = 0; j < N2; ++j) dst[j] = v[j];
@@ -504,7 +510,6 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
       }
       job_fence<J1>();
     }
-    if (!DB) lds_barrier();  // the final reads precede the next tile's stage-1 stores
   }
   __syncthreads();  // this pass's field stores are read by other threads in the next pass
 }
@@ -576,9 +581,11 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
 // static-striding loop.  The schedule is a compile-time choice because the loop's form changes
 // the code generated for the inlined site body: with take_site's barriers in the loop, the
 // uniform 4225-site launch runs 112 ms against 103 ms for the plain loop, static either way.
+// 256-thread blocks are held to two waves per SIMD (256 VGPRs), so that two blocks share a CU
+// as one 512-thread block would, each with barriers of its own.
 template <int N, int NT, int MODE, bool DYN>
-__global__ __launch_bounds__(NT) void sites_kernel(SitesArgs a) {
-  __shared__ Smem<NT, N> sm;
+__global__ __launch_bounds__(NT, 512 / NT) void sites_kernel(SitesArgs a) {
+  __shared__ SmemU<N, NT> sm;
   if constexpr (DYN) {
     __shared__ int next;
     for (int64_t i = take_site(a, -1, &next); i < a.P; i = take_site(a, i, &next))
