@@ -267,7 +267,23 @@ struct Tile {
   static constexpr int L = NT / (N >= 512 ? cmax(N1, N2) : cmin(N1, N2));  // lines per tile
   static constexpr int A2 = L * N2;                        // stage-2 jobs of a tile
   static constexpr int J1 = L * N1 / NT, J2 = cmax(1, A2 / NT);  // jobs per thread, 1 or 2
-  static constexpr int LS = cmax(N1 * (N2 + 1), N2 * (N1 + 1)) | 1;  // odd line stride
+  // Exchange layouts of a line: stage 1 writes y[n1][k2] at S1(n1) + k2, stage 2 reads it and
+  // writes element j of job k2 at k2 KS2 + C2(j).  The 512 side's 32-point lines are
+  // transformed by lane pairs, each lane holding one half (n1 or j below or above N1 / 2); its
+  // layouts keep the two halves in two planes HP float2 apart, HP = 8 mod 16, so that the two
+  // lanes of a pair reach different LDS banks.
+  static constexpr bool SPLIT = N >= 512;
+  static constexpr int H1 = N1 / 2;
+  static constexpr int HP = SPLIT ? H1 * (N2 + 1) + 8 : 0;
+  static constexpr int KS2 = SPLIT ? H1 + 1 : N1 + 1;
+  __host__ __device__ static constexpr int S1(int n1) {
+    return SPLIT ? (n1 >= H1 ? HP : 0) + (n1 % H1) * (N2 + 1) : n1 * (N2 + 1);
+  }
+  __host__ __device__ static constexpr int C2(int j) {
+    return SPLIT ? (j >= H1 ? HP : 0) + j % H1 : j;
+  }
+  static constexpr int LS =  // odd line stride
+      (SPLIT ? HP + cmax(H1 * (N2 + 1), N2 * KS2) : cmax(N1 * (N2 + 1), N2 * (N1 + 1))) | 1;
   static constexpr int LDS2 = L * LS;                     // float2 slots of exchange buffer
   static_assert(N % L == 0, "tile must divide the field");
   static_assert(J1 * NT == L * N1, "whole stage-1 jobs per thread");

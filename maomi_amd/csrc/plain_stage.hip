@@ -69,6 +69,11 @@ __device__ __forceinline__ void st1(Rsrc r, uint32_t voff, uint32_t soff, float 
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 
+// The value of the other lane of an even/odd lane pair (DPP quad permutation 1, 0, 3, 2).
+__device__ __forceinline__ float pair_swap(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
 // With two jobs per thread, the scheduler would interleave them across a whole tile (one basic
 // block) and hold both jobs' loads and transforms at once; this keeps each job's code together.
 template <int J>
@@ -201,12 +206,20 @@ __device__ void audit_site(const SitesArgs& a, int site, int slot, int tid, int 
   }
 }
 
+// Tables of a block.  With room for the 512 side (TAB = 512) they also hold what its lane
+// pairs read: lane 1 of a pair reads its own copies of the twiddles and Fresnel factors, 8
+// slots further along the LDS banks than lane 0's, so that the two lanes of a pair do not
+// meet on a bank.
 template <int SLOTS, int TAB, int NT>
 struct SmemT {
-  float2 x[SLOTS];  // line exchange buffer(s)
-  float2 tw[TAB];   // W_N^j
-  float2 q[TAB];    // 1-D Fresnel factor (FFT order, times 1/N)
-  float eg[TAB];    // 1-D edge window
+  static constexpr bool kPair = TAB >= 512;
+  // the 512 side: W512^(k2 (n + 16 h)) at tw[kTwR + h kH + 16 k2 + n] (k2, n < 16), and
+  // q[k2 + 16 (2 k + h)] at q[h kH + k2 + 16 k]
+  static constexpr int kTwR = TAB + 8, kH = TAB / 2 + 8;
+  float2 x[SLOTS];                          // line exchange buffer(s)
+  float2 tw[kPair ? kTwR + kH + TAB / 2 : TAB];  // W_N^j
+  float2 q[kPair ? TAB + 8 : TAB];          // 1-D Fresnel factor (FFT order, times 1/N)
+  float eg[TAB];                            // 1-D edge window
   double red[NT / 64];
 };
 // Mixed launches: room for every field side up to NMAX.
@@ -312,32 +325,74 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   float2 vpre[prefetch ? N2 : 1];  // the next tile's stage-1 input
 
   // stage B: xin -> xout (stage 2 of the forward transform, Fresnel factor, stage 1 of the
-  // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two.  With
-  // fewer jobs than threads (the 512 side) the first A2 / 64 waves take them.
+  // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two.
   auto stage_b = [&](const float2* xin, float2* xout) {
-    if (T::A2 < NT && tid >= T::A2) return;
+    if constexpr (N1 == 32) {
+      // 32-point lines (the 512 side): a lane pair per line, lane h holding the half
+      // y[m + 16 h], m < 16.  Forward, decimation in frequency: the pair exchange gives
+      // y[m] + y[m + 16] on lane 0 and y[m + 16] - y[m] on lane 1, which multiplies by
+      // -W32^m; 16-point transforms then give X[2k + h].  After the Fresnel factor, the
+      // inverse (decimation in time): 16-point transforms, lane 1 multiplies by -conj W32^n,
+      // and the pair exchange gives x[n] = E + O' on lane 0 and x[n + 16] = E - O' on lane 1.
+      static_assert(2 * T::A2 == NT, "one lane pair per 32-point line");
+      const int h = tid & 1, jb = tid >> 1;
+      const int l = ROWS ? jb / N2 : jb % L;
+      const int k2 = ROWS ? jb % N2 : jb / L;
+      const float sf = h ? -1.0f : 1.0f;  // forward exchange: own + sf * partner
+      // the pair twiddles, 1 on lane 0 and -W32^m (forward) or -conj W32^n (inverse) on lane 1,
+      // as hf (-c, +-s) + (1 - hf, 0) with hf = h: exact on both lanes, no table reads
+      const float hf = (float)h, omh = 1.0f - hf;
+      auto pair_tw = [&](int m, bool inv) {
+        const float c = fftk::kCos32[m], sn = fftk::kSin32[m];
+        return make_float2(fmaf(hf, -c, omh), hf * (inv ? -sn : sn));
+      };
+      const float2* qh = sm.q + h * SM::kH + k2;  // q[k2 + 16 (2 k + h)] at qh[16 k]
+      const float2* twh = sm.tw + SM::kTwR + h * SM::kH + 16 * k2;  // W^(k2 (n + 16 h)) at twh[n]
+      // (loads, table reads, exchanges and products in separate loops: with each exchange
+      // right after its element's load, the loads were issued one at a time)
+      float2 v[16];
+      const float2* src = xin + l * LS + k2 + h * T::HP;  // y[m + 16 h][k2], m < 16
 #pragma unroll
-    for (int jj = 0; jj < J2; ++jj) {
-      const int jb = tid + jj * NT;
-      const int l = (ROWS || J2 > 1) ? jb / N2 : jb % L;
-      const int k2 = (ROWS || J2 > 1) ? jb % N2 : jb / L;
-      float2 v[N1];
-      const float2* src = xin + l * LS + k2;
+      for (int m = 0; m < 16; ++m) v[m] = src[m * (N2 + 1)];
 #pragma unroll
-      for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
-      fftk::fft<N1, false>(v);
-      if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
+      for (int m = 0; m < 16; ++m)
+        v[m] = make_float2(fmaf(sf, pair_swap(v[m].x), v[m].x), fmaf(sf, pair_swap(v[m].y), v[m].y));
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
-      if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
-      fftk::fft<N1, true>(v);
-      if constexpr (N1 > 16) __builtin_amdgcn_sched_barrier(0);
+      for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], pair_tw(m, false));
+      fftk::fft<16, false>(v);
 #pragma unroll
-      for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
-      float2* dst = xout + l * LS + k2 * (N1 + 1);
+      for (int k = 0; k < 16; ++k) v[k] = cmul(v[k], qh[16 * k]);
+      fftk::fft<16, true>(v);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) dst[j] = v[j];
-      job_fence<J2>();
+      for (int n = 0; n < 16; ++n) v[n] = cmul(v[n], pair_tw(n, true));
+#pragma unroll
+      for (int n = 0; n < 16; ++n)
+        v[n] = make_float2(fmaf(-sf, pair_swap(v[n].x), v[n].x),
+                           fmaf(-sf, pair_swap(v[n].y), v[n].y));
+      float2* dst = xout + l * LS + k2 * T::KS2 + h * T::HP;  // element n + 16 h
+#pragma unroll
+      for (int n = 0; n < 16; ++n) dst[n] = cmulc(v[n], twh[n]);
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < J2; ++jj) {
+        const int jb = tid + jj * NT;
+        const int l = (ROWS || J2 > 1) ? jb / N2 : jb % L;
+        const int k2 = (ROWS || J2 > 1) ? jb % N2 : jb / L;
+        float2 v[N1];
+        const float2* src = xin + l * LS + k2;
+#pragma unroll
+        for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
+        fftk::fft<N1, false>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
+        fftk::fft<N1, true>(v);
+#pragma unroll
+        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+        float2* dst = xout + l * LS + k2 * (N1 + 1);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) dst[j] = v[j];
+        job_fence<J2>();
+      }
     }
   };
 
@@ -453,7 +508,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
       }
       if (FIRST) pointwise(l1, n1, v);
       fftk::fft<N2, false>(v);
-      float2* dst = xa + l1 * LS + n1 * (N2 + 1);
+      float2* dst = xa + l1 * LS + T::S1(n1);
 #pragma unroll
       for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[n1 * j]);
       if (prefetch && t + 1 < N / L) {
@@ -473,16 +528,16 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         int l1, n1;
         job1(jj, l1, n1);
         float2 v[N2];
-        const float2* src = xb + l1 * LS + n1;
+        const float2* src = xb + l1 * LS + T::C2(n1);
 #pragma unroll
-        for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
+        for (int n = 0; n < N2; ++n) v[n] = src[n * T::KS2];
         fftk::fft<N2, true>(v);
         pointwise(l1, n1, v);
         if (ahead && t + 1 < N / L) load_pointwise(t + 1, phn, scn, mkn);
         fftk::fft<N2, false>(v);
 #pragma unroll
         for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], sm.tw[n1 * j]);
-        float2* dst = xa + l1 * LS + n1 * (N2 + 1);
+        float2* dst = xa + l1 * LS + T::S1(n1);
 #pragma unroll
         for (int j = 0; j < N2; ++j) dst[j] = v[j];
         job_fence<J1>();
@@ -497,9 +552,9 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
       int l1, n1;
       job1(jj, l1, n1);
       float2 v[N2];
-      const float2* src = xb + l1 * LS + n1;
+      const float2* src = xb + l1 * LS + T::C2(n1);
 #pragma unroll
-      for (int n = 0; n < N2; ++n) v[n] = src[n * (N1 + 1)];
+      for (int n = 0; n < N2; ++n) v[n] = src[n * T::KS2];
       fftk::fft<N2, true>(v);
       if (LAST) {
         pointwise(l1, n1, v);
@@ -538,9 +593,18 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   for (int i = tid; i < N; i += NT) {
-    sm.tw[i] = a.twid[a.ctw[C] + i];
-    sm.q[i] = a.q1d[a.cq[C] + i];
+    const float2 t = a.twid[a.ctw[C] + i];
+    const float2 q = a.q1d[a.cq[C] + i];
+    sm.tw[i] = t;
     sm.eg[i] = a.edge[a.cedge[C] + i];
+    if constexpr (N == 512) {  // the lane pairs' tables
+      const int k2 = i & 15, k1 = i >> 4;
+      sm.q[(k1 & 1) * SM::kH + k2 + 16 * (k1 >> 1)] = q;
+      const int h = i >> 8, r = (i >> 4) & 15;  // row r = k2 of the pair twiddles, column k2
+      sm.tw[SM::kTwR + h * SM::kH + i % 256] = a.twid[a.ctw[C] + r * (k2 + 16 * h)];
+    } else {
+      sm.q[i] = q;
+    }
   }
   __syncthreads();
 
