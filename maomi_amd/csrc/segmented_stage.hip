@@ -288,6 +288,70 @@ __device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl
   }
 }
 
+// Fused middle of a half step (R1 <= 16): per (line, k2) job, in place in the scratch layout,
+// the forward R1-point DFT over n1, the 1-D Fresnel factor q[k2 + R2 k1], the inverse
+// R1-point DFT (the forward one with its outputs read in reverse order, IDFT[n] = DFT[-n]) and
+// the inverse transform's twiddle conj W_N^(n1 k2).  A job reads and writes only its own
+// column k2 of a line, so no barrier separates its reads from its writes.  With the inverse
+// transform's factors swapped (n = n1 + R1 n2, k = k2 + R2 k1), what remains of it is an
+// R2-point transform per (line, n1) (fused_last).
+template <int R1>
+__device__ __noinline__ void fused_mid(int b_off, int t_off, int q_off, int nl, int N) {
+  const int R2 = N / R1, NB = pad_stride(N, R2);
+  const float2* T = seg_lds + t_off;
+  const float2* Q = seg_lds + q_off;
+  float2 w[R1];  // W_R1^c = W_N^(c R2), forward
+#pragma unroll
+  for (int c = 0; c < R1; ++c) w[c] = T[c * R2];
+  for (int job = threadIdx.x; job < nl * R2; job += NT) {
+    const int l = job / R2, k2 = job - l * R2;
+    float2* b = seg_lds + b_off + l * NB + k2;
+    float2 x[R1], X[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) x[m] = b[m * (R2 + 1)];
+    small_dft<R1>(x, w, X);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) x[k1] = cmul(X[k1], Q[k2 + R2 * k1]);
+    small_dft<R1>(x, w, X);
+    // n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction
+    b[0] = X[0];
+#pragma unroll
+    for (int n1 = 1; n1 < R1; ++n1) b[n1 * (R2 + 1)] = cmul(X[R1 - n1], twid(T, n1 * k2, true));
+  }
+}
+
+// Last stage of the fused half step: per (line, n1) job, the inverse R2-point transform over
+// k2 of scratch row n1, giving x[n1 + R1 n2] in natural order in A.
+template <int R2>
+__device__ __noinline__ void fused_last(int a_off, int b_off, int t_off, int nl, int N) {
+  constexpr bool POW2 = (R2 & (R2 - 1)) == 0;
+  const int R1 = N / R2, NB = pad_stride(N, R2);
+  const float2* T = seg_lds + t_off;
+  float2 wr[POW2 ? 1 : R2];  // W_R2^-c = conj W_N^(c R1)
+  if constexpr (!POW2) {
+#pragma unroll
+    for (int c = 0; c < R2; ++c) wr[c] = twid(T, c * R1, true);
+  }
+  for (int job = threadIdx.x; job < nl * R1; job += NT) {
+    const int l = job / R1, n1 = job - l * R1;
+    const float2* b = seg_lds + b_off + l * NB + n1 * (R2 + 1);
+    float2 v[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) v[m] = b[m];
+    if constexpr (POW2) {
+      fftk::fft<R2, true>(v);
+    } else {
+      float2 X[R2];
+      small_dft<R2>(v, wr, X);
+#pragma unroll
+      for (int k = 0; k < R2; ++k) v[k] = X[k];
+    }
+    float2* a = seg_lds + a_off + l * N + n1;
+#pragma unroll
+    for (int m = 0; m < R2; ++m) a[R1 * m] = v[m];
+  }
+}
+
 // Stage 2 for R1 > 16: one output X[k2 + R2 k1] per work item, inputs and twiddles W_R1^(m k1)
 // read from LDS with the exponent advanced incrementally.
 template <bool INV>
@@ -380,6 +444,67 @@ __device__ void fast_line_transform(int a_off, int b_off, int t_off, int nl, int
     }
   }
   lds_barrier();
+}
+
+// One half step of nl lines through the fused stages: forward stage 1 (A -> B), fused_mid
+// (B in place) and fused_last (B -> A), two barriers fewer and two LDS passes fewer than a
+// forward transform, a separate Fresnel-factor pass and an inverse transform.  For splits of
+// classes 0 and 1 (R1 <= 16); returns false for the others.
+template <int CLS>
+__device__ bool fused_half_step(float2* A, float2* B, int nl, int N, const float2* T,
+                                const float2* q) {
+  const int r2 = fast_r2(N);
+  if (!r2) return false;
+  const int r1 = N / r2, c = split_class(N, r2);
+  if (c > CLS || c > 1) return false;
+  const int a_off = (int)(A - seg_lds), b_off = (int)(B - seg_lds), t_off = (int)(T - seg_lds),
+            q_off = (int)(q - seg_lds);
+  if (CLS == 0 || c == 0) {
+    switch (r2) {
+      case 2: fast_stage1<2, false>(a_off, b_off, t_off, nl, N); break;
+      case 4: fast_stage1<4, false>(a_off, b_off, t_off, nl, N); break;
+      case 8: fast_stage1<8, false>(a_off, b_off, t_off, nl, N); break;
+      default: fast_stage1<16, false>(a_off, b_off, t_off, nl, N); break;
+    }
+  } else if constexpr (CLS >= 1) {
+    switch (r2) {
+#define MAOMI_R2(R) case R: fast_stage1<R, false>(a_off, b_off, t_off, nl, N); break;
+      MAOMI_R2(2) MAOMI_R2(3) MAOMI_R2(4) MAOMI_R2(5) MAOMI_R2(6) MAOMI_R2(7) MAOMI_R2(8)
+      MAOMI_R2(9) MAOMI_R2(10) MAOMI_R2(11) MAOMI_R2(12) MAOMI_R2(13) MAOMI_R2(14)
+      MAOMI_R2(15) MAOMI_R2(16)
+#undef MAOMI_R2
+      default: break;
+    }
+  }
+  lds_barrier();
+  switch (r1) {
+#define MAOMI_R1(R) case R: fused_mid<R>(b_off, t_off, q_off, nl, N); break;
+    MAOMI_R1(1) MAOMI_R1(2) MAOMI_R1(3) MAOMI_R1(4) MAOMI_R1(5) MAOMI_R1(6) MAOMI_R1(7)
+    MAOMI_R1(8) MAOMI_R1(9) MAOMI_R1(10) MAOMI_R1(11) MAOMI_R1(12) MAOMI_R1(13) MAOMI_R1(14)
+    MAOMI_R1(15) MAOMI_R1(16)
+#undef MAOMI_R1
+    default: break;
+  }
+  lds_barrier();
+  if (CLS == 0 || c == 0) {
+    switch (r2) {
+      case 2: fused_last<2>(a_off, b_off, t_off, nl, N); break;
+      case 4: fused_last<4>(a_off, b_off, t_off, nl, N); break;
+      case 8: fused_last<8>(a_off, b_off, t_off, nl, N); break;
+      default: fused_last<16>(a_off, b_off, t_off, nl, N); break;
+    }
+  } else if constexpr (CLS >= 1) {
+    switch (r2) {
+#define MAOMI_R2(R) case R: fused_last<R>(a_off, b_off, t_off, nl, N); break;
+      MAOMI_R2(2) MAOMI_R2(3) MAOMI_R2(4) MAOMI_R2(5) MAOMI_R2(6) MAOMI_R2(7) MAOMI_R2(8)
+      MAOMI_R2(9) MAOMI_R2(10) MAOMI_R2(11) MAOMI_R2(12) MAOMI_R2(13) MAOMI_R2(14)
+      MAOMI_R2(15) MAOMI_R2(16)
+#undef MAOMI_R2
+      default: break;
+    }
+  }
+  lds_barrier();
+  return true;
 }
 
 // Transform nl lines of A through B with the fast split when N has one, else line_dft.
@@ -492,6 +617,7 @@ __device__ __forceinline__ void store_tile(const float2* A, float2* fld, bool ro
 template <int CLS>
 __device__ __forceinline__ void half_step(float2* A, float2* B, int nl, int N, int N1, int N2,
                                           const float2* T, const float2* q) {
+  if (fused_half_step<CLS>(A, B, nl, N, T, q)) return;
   line_transform<CLS>(A, B, nl, N, N1, N2, T, false);
   const int dl = NT / N, dk = NT - dl * N;
   for (int it = threadIdx.x, k = threadIdx.x - (threadIdx.x / N) * N; it < nl * N;
