@@ -123,15 +123,22 @@ __host__ __device__ constexpr int smallest_factor(int n, int f = 2) {
   return f * f > n ? n : (n % f == 0 ? f : smallest_factor(n, f + 1));
 }
 
-// D-point DFT Y[k] = sum_m y[m] W^(m k) with W^j = w[j SW] (the transform's sign already in
-// w).  The terms m and D - m share W^(m k) and its conjugate, so with S_m = y_m + y_(D-m) and
+// Twiddle tables of the small DFTs hold W_R^c for c <= R / 2 only: W_R^(R - c) is conj W_R^c,
+// which halves the registers a table takes (its sign folds into the products).
+template <int R>
+__device__ __forceinline__ float2 wtab(const float2 (&w)[R / 2 + 1], int i) {
+  return i <= R / 2 ? w[i] : make_float2(w[R - i].x, -w[R - i].y);
+}
+
+// D-point DFT Y[k] = sum_m y[m] W^(m k) with W^j = W_(D SW)^(j SW) from the half table w (the
+// transform's sign already in w).  The terms m and D - m share W^(m k) and its conjugate, so with S_m = y_m + y_(D-m) and
 // D_m = y_m - y_(D-m) (m < D/2),
 //   Y[k]     = y_0 + sum_m S_m c_mk + i sum_m D_m s_mk  (+ y_(D/2) (-1)^k for even D),
 //   Y[D - k] = the same with s_mk negated,
 // so each pair of outputs costs four real h-term sums (h = (D - 1) / 2): 66 operations for 7
 // points against 168 for the direct sums.
-template <int D, int SW, int RW>
-__device__ __forceinline__ void sym_dft(const float2 (&y)[D], const float2 (&w)[RW],
+template <int D, int SW>
+__device__ __forceinline__ void sym_dft(const float2 (&y)[D], const float2 (&w)[D * SW / 2 + 1],
                                         float2 (&Y)[D]) {
   if constexpr (D == 1) {
     Y[0] = y[0];
@@ -166,7 +173,7 @@ __device__ __forceinline__ void sym_dft(const float2 (&y)[D], const float2 (&w)[
       }
 #pragma unroll
       for (int m = 1; m <= H; ++m) {
-        const float2 t = w[((m * k) % D) * SW];
+        const float2 t = wtab<D * SW>(w, ((m * k) % D) * SW);
         ar = fmaf(S[m].x, t.x, ar);
         ai = fmaf(S[m].y, t.x, ai);
         br = fmaf(Df[m].y, t.y, br);
@@ -182,7 +189,7 @@ __device__ __forceinline__ void sym_dft(const float2 (&y)[D], const float2 (&w)[
 // factor) runs as P-point DFTs over m = Q n1 + n2, the twiddle W_R^(n2 k1), then Q-point DFTs,
 // X[k1 + P k2]; every level is a sym_dft.
 template <int R>
-__device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w)[R],
+__device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w)[R / 2 + 1],
                                           float2 (&X)[R]) {
   constexpr int P = smallest_factor(R), Q = R / P;
   if constexpr (P == R) {
@@ -197,7 +204,7 @@ __device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w
       sym_dft<P, Q>(t, w, u);
 #pragma unroll
       for (int k1 = 0; k1 < P; ++k1)
-        y[n2 * P + k1] = (n2 * k1) % R == 0 ? u[k1] : cmul(u[k1], w[(n2 * k1) % R]);
+        y[n2 * P + k1] = (n2 * k1) % R == 0 ? u[k1] : cmul(u[k1], wtab<R>(w, (n2 * k1) % R));
     }
 #pragma unroll
     for (int k1 = 0; k1 < P; ++k1) {
@@ -239,10 +246,10 @@ __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl
   constexpr bool POW2 = (R2 & (R2 - 1)) == 0;
   const int R1 = N / R2, NB = pad_stride(N, R2);
   const float2* T = seg_lds + t_off;
-  float2 wr[POW2 ? 1 : R2];  // W_R2^c = W_N^(c R1) for the factorised DFT
+  float2 wr[POW2 ? 1 : R2 / 2 + 1];  // W_R2^c = W_N^(c R1), c <= R2 / 2, for the factorised DFT
   if constexpr (!POW2) {
 #pragma unroll
-    for (int c = 0; c < R2; ++c) wr[c] = twid(T, c * R1, INV);
+    for (int c = 0; c <= R2 / 2; ++c) wr[c] = twid(T, c * R1, INV);
   }
   for (int job = threadIdx.x; job < nl * R1; job += NT) {
     const int l = job / R1, n1 = job - l * R1;
@@ -271,9 +278,9 @@ __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl
 template <int R1, bool INV>
 __device__ __noinline__ void fast_stage2(int a_off, int b_off, int t_off, int nl, int N) {
   const int R2 = N / R1, NB = pad_stride(N, R2);
-  float2 w[R1];  // W_R1^c = W_N^(c R2)
+  float2 w[R1 / 2 + 1];  // W_R1^c = W_N^(c R2), c <= R1 / 2
 #pragma unroll
-  for (int c = 0; c < R1; ++c) w[c] = twid(seg_lds + t_off, c * R2, INV);
+  for (int c = 0; c <= R1 / 2; ++c) w[c] = twid(seg_lds + t_off, c * R2, INV);
   for (int job = threadIdx.x; job < nl * R2; job += NT) {
     const int l = job / R2, k2 = job - l * R2;
     const float2* b = seg_lds + b_off + l * NB + k2;
@@ -300,9 +307,9 @@ __device__ __noinline__ void fused_mid(int b_off, int t_off, int q_off, int nl, 
   const int R2 = N / R1, NB = pad_stride(N, R2);
   const float2* T = seg_lds + t_off;
   const float2* Q = seg_lds + q_off;
-  float2 w[R1];  // W_R1^c = W_N^(c R2), forward
+  float2 w[R1 / 2 + 1];  // W_R1^c = W_N^(c R2), c <= R1 / 2, forward
 #pragma unroll
-  for (int c = 0; c < R1; ++c) w[c] = T[c * R2];
+  for (int c = 0; c <= R1 / 2; ++c) w[c] = T[c * R2];
   for (int job = threadIdx.x; job < nl * R2; job += NT) {
     const int l = job / R2, k2 = job - l * R2;
     float2* b = seg_lds + b_off + l * NB + k2;
@@ -310,9 +317,14 @@ __device__ __noinline__ void fused_mid(int b_off, int t_off, int q_off, int nl, 
 #pragma unroll
     for (int m = 0; m < R1; ++m) x[m] = b[m * (R2 + 1)];
     small_dft<R1>(x, w, X);
+    // (scheduling fences: the factor and twiddle reads are not hoisted above the transforms,
+    // where their registers pushed the larger sizes into spills)
+    if constexpr (R1 > 12) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) x[k1] = cmul(X[k1], Q[k2 + R2 * k1]);
+    if constexpr (R1 > 12) __builtin_amdgcn_sched_barrier(0);
     small_dft<R1>(x, w, X);
+    if constexpr (R1 > 12) __builtin_amdgcn_sched_barrier(0);
     // n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction
     b[0] = X[0];
 #pragma unroll
@@ -327,10 +339,10 @@ __device__ __noinline__ void fused_last(int a_off, int b_off, int t_off, int nl,
   constexpr bool POW2 = (R2 & (R2 - 1)) == 0;
   const int R1 = N / R2, NB = pad_stride(N, R2);
   const float2* T = seg_lds + t_off;
-  float2 wr[POW2 ? 1 : R2];  // W_R2^-c = conj W_N^(c R1)
+  float2 wr[POW2 ? 1 : R2 / 2 + 1];  // W_R2^-c = conj W_N^(c R1), c <= R2 / 2
   if constexpr (!POW2) {
 #pragma unroll
-    for (int c = 0; c < R2; ++c) wr[c] = twid(T, c * R1, true);
+    for (int c = 0; c <= R2 / 2; ++c) wr[c] = twid(T, c * R1, true);
   }
   for (int job = threadIdx.x; job < nl * R1; job += NT) {
     const int l = job / R1, n1 = job - l * R1;
