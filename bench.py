@@ -31,7 +31,7 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC_CONFIG2_FP32 = (2.0 * 1.0785e8 + 1.0 * 1.9576e8) * 1024
 # The same for segmented_sites_kernel on the reference demo geometry (4225 sites), from
 # profiles/segmented_sites_kernel_reference_geometry.md.
-TRAFFIC_REF_DEMO = (2.0 * 1.8437e7 + 1.0 * 3.4719e7) * 1024
+TRAFFIC_REF_DEMO = (2.0 * 1.945e7 + 1.0 * 3.474e7) * 1024
 METRIC = "sites/s on 4225-site optical stage; surrogate vs propagator at ≤1% rel L2"
 
 
