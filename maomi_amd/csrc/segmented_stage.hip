@@ -223,7 +223,9 @@ __device__ __forceinline__ void small_dft(const float2 (&x)[R], const float2 (&w
 // factorised small_dft) followed by the twiddle W_N^(n1 k2); stage 2 an R1-point DFT per
 // (line, k2) job in registers (R1 <= 32) or per output from LDS (wide_stage2).
 // X[k2 + R2 k1] lands in natural order.  Lines live at LDS offsets a_off (in and out, line
-// stride N) and b_off (scratch); the twiddle table W_N^j at t_off.  In the scratch buffer,
+// stride N) and b_off (scratch); the twiddle table W_N^j at t_off, and just below it, ending
+// at t_off, the row twiddles W_N^(n1 k2) at n1 (R2 + 1) + k2 (the segment loads both).  In
+// the scratch buffer,
 // element (n1, k2) of a line sits at n1 (R2 + 1) + k2 and lines are pad_stride(N) apart: the
 // padding keeps the stage-1 stores of neighbouring jobs (R2 apart otherwise) off the same
 // LDS banks.
@@ -257,9 +259,11 @@ __device__ __noinline__ void fast_stage1(int a_off, int b_off, int t_off, int nl
     float2 v[R2], w[R2];
 #pragma unroll
     for (int m = 0; m < R2; ++m) v[m] = a[R1 * m];
-    // W_N^(n1 k2): n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction; read before any store
+    // W_N^(n1 k2) from row n1 of the row twiddles (consecutive jobs on different banks); read
+    // before any store
+    const float2* tr = T - NB + n1 * (R2 + 1);
 #pragma unroll
-    for (int k2 = 1; k2 < R2; ++k2) w[k2] = twid(T, n1 * k2, INV);
+    for (int k2 = 1; k2 < R2; ++k2) w[k2] = twid(tr, k2, INV);
     if constexpr (POW2) {
       fftk::fft<R2, INV>(v);
     } else {
@@ -310,6 +314,7 @@ __device__ __noinline__ void fused_mid(int b_off, int t_off, int q_off, int nl, 
   float2 w[R1 / 2 + 1];  // W_R1^c = W_N^(c R2), c <= R1 / 2, forward
 #pragma unroll
   for (int c = 0; c <= R1 / 2; ++c) w[c] = T[c * R2];
+  const float2* TR = T - NB;  // row twiddles W_N^(n1 k2) at TR[n1 (R2 + 1) + k2]
   for (int job = threadIdx.x; job < nl * R2; job += NT) {
     const int l = job / R2, k2 = job - l * R2;
     float2* b = seg_lds + b_off + l * NB + k2;
@@ -325,10 +330,10 @@ __device__ __noinline__ void fused_mid(int b_off, int t_off, int q_off, int nl, 
     if constexpr (R1 > 12) __builtin_amdgcn_sched_barrier(0);
     small_dft<R1>(x, w, X);
     if constexpr (R1 > 12) __builtin_amdgcn_sched_barrier(0);
-    // n1 k2 <= (R1 - 1)(R2 - 1) < N, so no reduction
     b[0] = X[0];
 #pragma unroll
-    for (int n1 = 1; n1 < R1; ++n1) b[n1 * (R2 + 1)] = cmul(X[R1 - n1], twid(T, n1 * k2, true));
+    for (int n1 = 1; n1 < R1; ++n1)
+      b[n1 * (R2 + 1)] = cmul(X[R1 - n1], twid(TR + n1 * (R2 + 1), k2, true));
   }
 }
 
@@ -663,6 +668,13 @@ __device__ __forceinline__ void run_segment(const SegArgs& a, int k, int64_t ox,
   const int r2s = fast_r2(N), NBs = r2s ? pad_stride(N, r2s) : N;
   const int L = max(1, min(a.L * a.n / N, a.L * a.nb / NBs));
   for (int i = threadIdx.x; i < N; i += NT) { T[i] = g.tw[i]; E[i] = g.edge[i]; }
+  if (r2s) {  // row twiddles W_N^(n1 k2) at TR[n1 (R2 + 1) + k2], TR = T - pad_stride(N, R2)
+    float2* TR = T - NBs;
+    for (int i = threadIdx.x; i < N; i += NT) {
+      const int n1 = i / r2s, k2 = i - n1 * r2s;
+      TR[n1 * (r2s + 1) + k2] = g.tw[n1 * k2];
+    }
+  }
   __syncthreads();
   const bool top = k == 0, bot = k == 2, foc = k == 1;
   const int first_saved = top ? a.cut : 0;
@@ -820,7 +832,7 @@ segmented_sites_kernel(SegArgs a) {
   const int nmax = a.n;
   float2* A = seg_lds;
   float2* B = A + a.L * nmax;
-  float2* T = B + a.L * a.nb;
+  float2* T = B + a.L * a.nb + a.nb;  // after the row-twiddle region (see run_segment)
   float2* Qp = T + nmax;
   float2* Qn = Qp + nmax;
   float* E = (float*)(Qn + nmax);
@@ -960,12 +972,12 @@ MAOMI_API int maomi_segmented_sites(const maomi_segmented_desc* d, maomi_stream_
     const int r2 = fast_split((int)side);
     if (r2 && pad_stride((int)side, r2) > a.nb) a.nb = pad_stride((int)side, r2);
   }
-  // line buffers A (L x n) and B (L x nb) complex plus tables (twiddles, two Fresnel factors,
-  // edge window), kept under 60 KiB of LDS
-  a.L = (int)((61440 - 24 * n - 4 * n - 96) / (8 * n + 8 * a.nb));
+  // line buffers A (L x n) and B (L x nb) complex plus tables (row twiddles, twiddles, two
+  // Fresnel factors, edge window), kept under 60 KiB of LDS
+  a.L = (int)((61440 - 24 * n - 8 * a.nb - 4 * n - 96) / (8 * n + 8 * a.nb));
   if (a.L < 1) a.L = 1;
-  const size_t shm = ((size_t)a.L * (n + a.nb) + 3 * n) * sizeof(float2) + (n + 2) * sizeof(float) +
-                     (NT / 64) * sizeof(double) + 16;
+  const size_t shm = ((size_t)a.L * (n + a.nb) + a.nb + 3 * n) * sizeof(float2) +
+                     (n + 2) * sizeof(float) + (NT / 64) * sizeof(double) + 16;
   int64_t grid = d->grid > 0 ? d->grid : d->P;
   if (grid > d->P) grid = d->P;
   // the kernel of the largest line-split class among the two field sides
