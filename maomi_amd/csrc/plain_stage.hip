@@ -213,12 +213,16 @@ __device__ void audit_site(const SitesArgs& a, int site, int slot, int tid, int 
 template <int SLOTS, int TAB, int NT>
 struct SmemT {
   static constexpr bool kPair = TAB >= 512;
+  // the uniform 256 side: W256^(a b) at twr[17 a + b] (a, b < 16), the inter-stage twiddles of
+  // a job read at immediate offsets from its row, neighbouring jobs' rows on different banks
+  static constexpr bool kRowTw = TAB == 256;
   // the 512 side: W512^(k2 (n + 16 h)) at tw[kTwR + h kH + 16 k2 + n] (k2, n < 16), and
   // q[k2 + 16 (2 k + h)] at q[h kH + k2 + 16 k]
   static constexpr int kTwR = TAB + 8, kH = TAB / 2 + 8;
   float2 x[SLOTS];                          // line exchange buffer(s)
   float2 tw[kPair ? kTwR + kH + TAB / 2 : TAB];  // W_N^j
   float2 q[kPair ? TAB + 8 : TAB];          // 1-D Fresnel factor (FFT order, times 1/N)
+  float2 twr[kRowTw ? 16 * 17 : 1];
   float eg[TAB];                            // 1-D edge window
   double red[NT / 64];
 };
@@ -292,6 +296,11 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   constexpr bool prefetch = kPrefetch && KIND == kMid && J1 == 1 && N <= 256 && !PSF && !lean;
   float2* const xa = sm.x;
   float2* const xb = sm.x + T::LDS2;
+  // the inter-stage twiddle W_N^(a b) of job a, element b
+  auto tw_at = [&](int a, int b) {
+    if constexpr (SM::kRowTw && N == 256) return sm.twr[17 * a + b];
+    else return sm.tw[a * b];
+  };
   // element m of a stage-1 job lies at (x0 + DX m, y0 + DY m); GS is its index stride in the
   // field, the mask and the entry field, PS in a slice of the phase volume
   constexpr int DX = ROWS ? 0 : N1, DY = ROWS ? N1 : 0;
@@ -387,7 +396,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
         fftk::fft<N1, true>(v);
 #pragma unroll
-        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], sm.tw[k2 * j]);
+        for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], tw_at(k2, j));
         float2* dst = xout + l * LS + k2 * (N1 + 1);
 #pragma unroll
         for (int j = 0; j < N1; ++j) dst[j] = v[j];
@@ -510,7 +519,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
       fftk::fft<N2, false>(v);
       float2* dst = xa + l1 * LS + T::S1(n1);
 #pragma unroll
-      for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], sm.tw[n1 * j]);
+      for (int j = 0; j < N2; ++j) dst[j] = j == 0 ? v[j] : cmul(v[j], tw_at(n1, j));
       if (prefetch && t + 1 < N / L) {
 #pragma unroll
         for (int m = 0; m < N2; ++m) vpre[m] = ld2(r_fld, f0 + (uint32_t)TS * 8, GS8 * m);
@@ -536,7 +545,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         if (ahead && t + 1 < N / L) load_pointwise(t + 1, phn, scn, mkn);
         fftk::fft<N2, false>(v);
 #pragma unroll
-        for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], sm.tw[n1 * j]);
+        for (int j = 1; j < N2; ++j) v[j] = cmul(v[j], tw_at(n1, j));
         float2* dst = xa + l1 * LS + T::S1(n1);
 #pragma unroll
         for (int j = 0; j < N2; ++j) dst[j] = v[j];
@@ -597,6 +606,10 @@ __device__ __forceinline__ void run_site(const SitesArgs& a, int site, SM& sm) {
     const float2 q = a.q1d[a.cq[C] + i];
     sm.tw[i] = t;
     sm.eg[i] = a.edge[a.cedge[C] + i];
+    if constexpr (SM::kRowTw && N == 256) {
+      const int r = i >> 4, c = i & 15;
+      sm.twr[17 * r + c] = a.twid[a.ctw[C] + r * c];
+    }
     if constexpr (N == 512) {  // the lane pairs' tables
       const int k2 = i & 15, k1 = i >> 4;
       sm.q[(k1 & 1) * SM::kH + k2 + 16 * (k1 >> 1)] = q;
