@@ -335,6 +335,15 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
 
   // stage B: xin -> xout (stage 2 of the forward transform, Fresnel factor, stage 1 of the
   // inverse); job jj covers lines [jj NT / N2, (jj + 1) NT / N2) when a thread has two.
+  // The uniform 256 side holds its stage-B job's Fresnel factors in registers for the whole
+  // pass: a thread's k2 is the same in every tile.
+  constexpr bool qreg = SM::kRowTw && N == 256 && J2 == 1 && !PSF;
+  float2 qk[qreg ? N1 : 1];
+  if constexpr (qreg) {
+    const int k2 = ROWS ? tid % N2 : tid / L;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) qk[k1] = sm.q[k2 + N2 * k1];
+  }
   auto stage_b = [&](const float2* xin, float2* xout) {
     if constexpr (N1 == 32) {
       // 32-point lines (the 512 side): a lane pair per line, lane h holding the half
@@ -393,7 +402,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         for (int n = 0; n < N1; ++n) v[n] = src[n * (N2 + 1)];
         fftk::fft<N1, false>(v);
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], sm.q[k2 + N2 * k1]);
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = cmul(v[k1], qreg ? qk[k1] : sm.q[k2 + N2 * k1]);
         fftk::fft<N1, true>(v);
 #pragma unroll
         for (int j = 1; j < N1; ++j) v[j] = cmulc(v[j], tw_at(k2, j));
