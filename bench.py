@@ -28,7 +28,7 @@ PEAK_HBM_GBS = 8000.0
 # whole rounds and the last round), from two rocprofv3 --pmc runs of this script: FETCH_SIZE x
 # 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy, --copy-probe); the runs and their commands
 # are in profiles/sites_kernel_256_config2_pmc.md.
-TRAFFIC_CONFIG2_FP32 = (2.0 * 1.0785e8 + 1.0 * 1.9576e8) * 1024
+TRAFFIC_CONFIG2_FP32 = (2.0 * 1.108e8 + 1.0 * 1.958e8) * 1024
 # The same for segmented_sites_kernel on the reference demo geometry (4225 sites), from
 # profiles/segmented_sites_kernel_reference_geometry.md.
 TRAFFIC_REF_DEMO = (2.0 * 1.945e7 + 1.0 * 3.474e7) * 1024
