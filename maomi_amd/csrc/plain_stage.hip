@@ -453,12 +453,26 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
     }
     // pointwise work of plane p on stage-1 job (l1, n1): screen times edge window (planes before
     // the last), |u|^4 into the mask with the trapezoid weight, focal-plane output
+    // FP16 screens: a job whose phases (held in registers) are all below pi/2 in magnitude, on
+    // every lane of the wave, skips the sin/cos range reduction (bit-identical there, 8 VALU
+    // operations per pixel fewer); a physical screen of a few tenths of a radian takes this
+    // path almost always.  (The FP32 kernel has no registers left for the second copy of the
+    // pointwise stage: it spilled 22 and ran 0.8% slower.)
     auto pointwise = [&](int l1, int n1, float2 (&v)[N2]) {
       const int line = t * L + l1;
       const int x0 = ROWS ? line : n1, y0 = ROWS ? n1 : line;
       const uint32_t g0 = (uint32_t)(x0 * N + y0);
       const uint32_t ph0 = (uint32_t)((c.ox + x0) * a.W + c.oy + y0) * SE;
       const float efix = sm.eg[ROWS ? x0 : y0];
+      constexpr bool split = !LAST && !cplx && hoist && half;
+      bool small = false;
+      if constexpr (split) {
+        bool in = true;
+#pragma unroll
+        for (int m = 0; m < N2; ++m) in = in && fabsf(phv[m]) <= trig::kNoReduction;
+        small = __all(in);
+      }
+      auto body = [&](auto reduce) {
 #pragma unroll
       for (int m = 0; m < N2; ++m) {
         const int x = x0 + DX * m, y = y0 + DY * m;
@@ -472,7 +486,10 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
             const float phi = hoist ? phv[m]
                               : half ? ldh(r_scr, ph0, PSB * m) * sscale
                                      : ld1(r_scr, ph0, PSB * m);
-            screen_sincos(phi, &sn, &cs);
+            if constexpr (decltype(reduce)::value)
+              screen_sincos(phi, &sn, &cs);
+            else
+              trig::sincos_reduced(phi, &sn, &cs);
           }
           const float sg = efix * sm.eg[ROWS ? y : x];
           v[m] = cmul(v[m], make_float2(cs * sg, sn * sg));
@@ -489,6 +506,11 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
         st1(r_mask, g0 * 4, GS4 * m, acc);
         if (LAST) esum += (double)acc;
       }
+      };
+      if (split && small)
+        body(std::false_type{});
+      else
+        body(std::true_type{});
     };
     auto field_byte = [&](int l1, int n1) {  // byte offset of a stage-1 job's element 0
       const int line = t * L + l1;

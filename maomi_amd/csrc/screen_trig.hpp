@@ -16,21 +16,31 @@ namespace trig {
 
 constexpr float kScreenPhaseLimit = 131072.0f;
 
+// Below this magnitude the nearest multiple of pi is 0 (x / pi rounds below 0.5), the reduction
+// leaves x unchanged and the sign is +: sincos_reduced(x) is then bit for bit sincos_screen(x).
+constexpr float kNoReduction = 1.5707f;
+
+// The polynomials of sincos_screen on a reduced argument r in [-pi/2, pi/2].
+__host__ __device__ __forceinline__ void sincos_reduced(float r, float* s, float* c) {
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.6000243e-06f, -1.9806602e-04f);
+  ps = fmaf(r2, ps, 8.333017e-03f);
+  ps = fmaf(r2, ps, -1.6666657e-01f);
+  *s = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, -2.619355e-07f, 2.4769291e-05f);
+  pc = fmaf(r2, pc, -1.3888569e-03f);
+  pc = fmaf(r2, pc, 4.1666657e-02f);
+  pc = fmaf(r2, pc, -0.5f);
+  *c = fmaf(r2, pc, 1.0f);
+}
+
 __host__ __device__ __forceinline__ void sincos_screen(float x, float* s, float* c) {
   const float k = rintf(x * 3.183098733e-01f);  // nearest multiple of pi
   float r = fmaf(-k, 3.141592741e+00f, x);
   r = fmaf(-k, -8.742277657e-08f, r);
   r = fmaf(-k, -3.552713679e-15f, r);
-  const float r2 = r * r;
-  float ps = fmaf(r2, 2.6000243e-06f, -1.9806602e-04f);
-  ps = fmaf(r2, ps, 8.333017e-03f);
-  ps = fmaf(r2, ps, -1.6666657e-01f);
-  const float sr = fmaf(r * r2, ps, r);
-  float pc = fmaf(r2, -2.619355e-07f, 2.4769291e-05f);
-  pc = fmaf(r2, pc, -1.3888569e-03f);
-  pc = fmaf(r2, pc, 4.1666657e-02f);
-  pc = fmaf(r2, pc, -0.5f);
-  const float cr = fmaf(r2, pc, 1.0f);
+  float sr, cr;
+  sincos_reduced(r, &sr, &cr);
   // sin(r + k pi) = (-1)^k sin r, cos(r + k pi) = (-1)^k cos r
   const unsigned sign = static_cast<unsigned>(static_cast<int>(k)) << 31;
   *s = __uint_as_float(__float_as_uint(sr) ^ sign);
