@@ -196,3 +196,29 @@ def test_config2_fp32_against_fp64_oracle(gpu_lib):
     print(f"config2 FP32 vs FP64 oracle: {err:.3e}; from the FP64 beam: {err64:.3e}", flush=True)
     assert err < 4e-5
     assert err64 < 1.5e-5
+
+
+def test_fp16_screens_match_oracle_on_the_same_screens(gpu_lib):
+    """The FP16-screen kernels against the complex64 oracle given the same dequantized screens
+    (phi = float(phase16) * slice scale, as the kernel reads them), at the FP32 bar.  With index
+    std 0.035 about one phase sample in a thousand lies beyond pi/2, so a wave's 16 x 64 phases
+    of a job are all below it about half the time: part of the jobs take the full sin/cos range
+    reduction and part skip it, and both paths of the kernel are compared."""
+    import torch
+    from maomi_amd.workloads import CONFIG1, site_offsets, zernike_weights, index_volume
+    cfg = replace(CONFIG1, dn_std=0.035)
+    st, vol, offs, zw = _run(cfg, precision="mixed")
+    ph16 = st.phase16.float().cpu().numpy()
+    scale = st.scale.cpu().numpy()
+    phase = (ph16 * scale[:, None, None]).astype(np.float32)
+    big = np.mean(np.abs(phase) > 1.5707)
+    print(f"fp16 screens: max |phi| {np.abs(phase).max():.2f} rad, {big:.1%} of samples "
+          f"beyond pi/2", flush=True)
+    assert 1e-4 < big < 0.02
+    base, _ = O.plain_beam(cfg, zw)
+    ref = O.plain_stage_masks(base, phase, np.asarray(offs, int),
+                              wvl=np.float32(cfg.wavelength_um * 1e-6), nr=np.float32(cfg.n0),
+                              dx=np.float32(cfg.dx_um * 1e-6), dz=cfg.dz_um * 1e-6)
+    err = rel_l2(st.masks.cpu().numpy(), np.stack(ref))
+    print(f"fp16 screens vs oracle on the same screens: {err:.3e}", flush=True)
+    assert err <= 1e-5
