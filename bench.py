@@ -321,7 +321,7 @@ def bench_uneven(args):
     centres = site_offsets(cfg)
     vol_np = index_volume(cfg)
     vol = torch.as_tensor(vol_np, device="cuda:0")
-    st = UnevenStage(cfg, vol, centres, sizes, slices).plan()
+    st = UnevenStage(cfg, vol, centres, sizes, slices).plan(overlap_v4=False)
     tuned = st.autotune()
     print(f"[bench] autotune: {st.tune_log}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -333,7 +333,7 @@ def bench_uneven(args):
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        st.plan()
+        st.plan(overlap_v4=args.overlap_v4)
         torch.cuda.synchronize()
         tp = time.perf_counter()
         ev0.record()

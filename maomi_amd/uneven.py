@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .numerics import V4Ahead
 from .plain import (SiteTables, _CLASSES, _SLOT, base_beams, check_screen_range, make_pupil,
                     v4_masks)
 from .workloads import PlainConfig, beam_parameters, zernike_weights
@@ -101,8 +102,16 @@ class UnevenStage:
         self.timing = {}
         self._plan = None
 
-    def plan(self):
+    def plan(self, overlap_v4=True):
+        """Phase screens, Fresnel tables and base beams.  With overlap_v4, the point-only part of
+        the v4 solve (Green matrix, projection, Cholesky: the site centres are all it needs) is
+        queued first on a side stream, where its latency-bound kernels run beside this stage's
+        small ones; aggregate() picks it up."""
         cfg, d = self.cfg, self.device
+        if overlap_v4:
+            if getattr(self, "_v4", None) is None:
+                self._v4 = V4Ahead(d)
+            self._v4.start(self.centres.astype(np.float64))
         lib = nat.lib()
         st = nat.stream_handle(d)
         Z, H = cfg.slices, cfg.padded
@@ -235,8 +244,9 @@ class UnevenStage:
 
     def aggregate(self):
         cfg = self.cfg
+        prep = self._v4.take() if getattr(self, "_v4", None) is not None else None
         m = v4_masks(self.centres.astype(np.float64), self.energies, (cfg.fov, cfg.fov),
-                     self.device)[..., 0]
+                     self.device, prepared=prep)[..., 0]
         self.mask_raw, self.mask_mean = m, m.mean()
         self.mask = m / self.mask_mean
         return self.mask
