@@ -46,8 +46,10 @@ def field_shapes(config):
 
 
 def assemble(config, out, shape, *, mask_mode="matlab_v4", z_positions=None, cut=None,
-             device=None):
-    """Masks and PSF statistics from native outputs (fixed.py:119-190); v4 on the GPU in FP64."""
+             device=None, v4_ahead=None):
+    """Masks and PSF statistics from native outputs (fixed.py:119-190); v4 on the GPU in FP64.
+    v4_ahead: a numerics.V4Ahead started on these points (times vres), whose point-only part of
+    the solve is then taken instead of computed here."""
     from .numerics import v4_gpu
     if mask_mode == "cubic":
         raise ValueError("mask_mode 'cubic' is a legacy diagnostic outside this package")
@@ -64,7 +66,9 @@ def assemble(config, out, shape, *, mask_mode="matlab_v4", z_positions=None, cut
         masks = np.broadcast_to(values[0], tuple(shape) + values.shape[1:]).copy()
     else:
         dev = torch.device(device or "cuda")
-        masks = v4_gpu(points * config.vres, values, q.reshape(-1, 2), dev).cpu().numpy()
+        prep = v4_ahead.take() if v4_ahead is not None else None
+        masks = v4_gpu(points * config.vres, values, q.reshape(-1, 2), dev,
+                       prepared=prep).cpu().numpy()
         masks = masks.reshape(tuple(shape) + (values.shape[1],))
     weights = np.full(len(points), 1 / len(points))
     mask = masks[..., 0]
@@ -244,6 +248,13 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
     timings["fields_and_collection"] = time.perf_counter() - t
     with torch.cuda.device(device):
         t = time.perf_counter()
+        # the point-only part of the v4 solve (Green matrix, projection, Cholesky) depends on the
+        # lattice alone: queued first on a side stream, it runs beside the plan stage
+        ahead = None
+        if len(points) > 1:
+            from .numerics import V4Ahead
+            ahead = V4Ahead(torch.device(device))
+            ahead.start(np.asarray(points, np.float64) * config.vres)
         u = (pupil(config, custom_phase=custom_phase, phase_units=phase_units,
                    phase_aperture=phase_aperture, device=device)
              if input_pupil is None
@@ -265,7 +276,8 @@ def run_fixed(config, volume, *, vessels=None, raw_fields=None, scatter=True, cu
             timings["propagation_kernel"] = plan.last_launch_seconds()
         t = time.perf_counter()
         result = assemble(config, outputs, expected[:2], mask_mode=mask_mode,
-                          z_positions=plan.z_positions, cut=plan.cut, device=device)
+                          z_positions=plan.z_positions, cut=plan.cut, device=device,
+                          v4_ahead=ahead)
         if hasattr(result["psf_3d"], "result"):
             result["psf_3d"] = result["psf_3d"].result()
         result["colmask"] = col

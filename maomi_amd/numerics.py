@@ -503,9 +503,10 @@ def v4_apply(p, coef, q, out, device):
     return out
 
 
-def v4_gpu(points, values, flat_queries, device, chunk=1 << 20):
-    """Device FP64 v4: returns a (Q, C) float64 tensor on `device`."""
-    p, G, factor = v4_prepare(points, device)
+def v4_gpu(points, values, flat_queries, device, chunk=1 << 20, prepared=None):
+    """Device FP64 v4: returns a (Q, C) float64 tensor on `device`.  prepared: a v4_prepare
+    result for these points (e.g. from V4Ahead.take())."""
+    p, G, factor = prepared if prepared is not None else v4_prepare(points, device)
     v = torch.as_tensor(values, dtype=torch.float64, device=device).reshape(len(p), -1).contiguous()
     q = torch.as_tensor(np.ascontiguousarray(flat_queries, np.float64), device=device)
     P, C = v.shape
