@@ -13,7 +13,8 @@
 // (the roles of c and s exchanged when |s| > |c|) costs 6 instructions instead of 8 (a
 // complex product and two complex additions).  A 16-point transform takes 148 instructions
 // against 168 for the radix-2 form with the same trivial twiddles skipped; 32 points take 388
-// against 456, but the 32-point lines keep the radix-2 form (fft_radix2 below).
+// against 456.  fft<32> runs the radix-2 form (fft_radix2 below); the sites kernels' 32-point
+// lines (the 512 side) are transformed by lane pairs of 16-point transforms instead.
 //
 // Line decomposition, N = N1 * N2, two register stages joined by one LDS exchange:
 //   stage 1, job n1 in [0,N1): y[n1][k2] = W_N^{n1 k2} sum_n2 x[n1 + N1 n2] W_N2^{n2 k2}

@@ -323,8 +323,7 @@ __device__ __forceinline__ void line_pass(const SitesArgs& a, const SiteCtx& c, 
   const float sscale = half && !LAST ? a.slice_scale[p] : 1.0f;
   // stage-1 job jj of this thread: line l1 of the tile, offset n1 (rows: consecutive lanes take
   // consecutive offsets; columns: consecutive lines, for coalesced global access).  With two
-  // jobs per thread, job jj covers lines [jj L / J1, (jj + 1) L / J1), so that with one buffer
-  // its stores wait only for the reads of its own lines.
+  // jobs per thread, job jj covers lines [jj L / J1, (jj + 1) L / J1).
   constexpr int LJ = L / J1;  // lines of one job
   auto job1 = [&](int jj, int& l1, int& n1) {
     const int jb = tid + jj * NT;
