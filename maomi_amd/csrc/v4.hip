@@ -123,9 +123,10 @@ __global__ void green_matrix_kernel(const double* __restrict__ p, double* __rest
 // FP64 Cholesky factorisation A = L L^T of the v4 system's positive-definite block (row-major,
 // lower triangle, in place), right-looking in steps of kNb columns with a one-step lookahead
 // on the diagonal block.  L11 of step k (the factored diagonal block, zero above the diagonal,
-// identity past the matrix) lives in side buffer k % 2.  Per step:
+// identity past the matrix) lives in side buffer k % 2, transposed: buf[c * kNb + i] = L11[i][c].
+// Per step:
 //   chol_panel_kernel: solves the rows of the panel below the diagonal block, x L11^T = a,
-//     with L11 from the side buffer (64 rows per block, a wave per 16 columns);
+//     with L11 from the side buffer (a wave per 64 rows, a row per lane);
 //   chol_update_kernel: block 0 copies L11 into the diagonal block of A; the others subtract
 //     L21 L21^T from the lower 64 x 64 tiles of the trailing matrix, and the block of the first
 //     tile (the next diagonal block) then factors it into the other side buffer, so that the
@@ -133,6 +134,12 @@ __global__ void green_matrix_kernel(const double* __restrict__ p, double* __rest
 // chol_diag_kernel factors the first diagonal block.  Only the lower triangle is read or
 // written.  The first non-positive (or NaN) pivot sets *info to its 1-based column, as
 // LAPACK's potrf does.
+//
+// Both sequential parts run on one wave with no workgroup barrier: a barrier per column costs
+// about a thousand cycles, 64 of them a step.  The columns go in four groups of 16; a group
+// takes its own triangle column by column, then its update of the later columns as one
+// rank-16 update, and the lane's row is then shifted down 16 registers, so that one loop body
+// serves the four groups.
 
 constexpr int kNb = 64;
 constexpr int kCholThreads = 256;
@@ -140,8 +147,7 @@ constexpr int kCholThreads = 256;
 struct CholShared {
   double lt[kNb][kNb];  // lt[c][i] = L11[i][c] (zero above the diagonal)
   double rd[kNb];       // 1 / L11[j][j]
-  double piv[kNb];      // pivots of the LDL^T form, L11[j][j]^2
-  double col[2][kNb];   // column j of the running factorisation, double-buffered
+  double col[16][kNb];  // the current group's columns as broadcast, col[JJ][c] = S[c][16 jg + JJ]
 };
 
 // Workgroup barrier that orders LDS only: global loads stay in flight across it.
@@ -151,164 +157,184 @@ __device__ __forceinline__ void chol_lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Step j = 16 jg + JJ of the factorisation: column j is broadcast through LDS, then the wave
-// holding columns 16 g .. 16 g + 15 of every row updates S[i][c] -= S[i][j] S[c][j] / S[j][j]
-// for its columns c > j: all 16 when g > jg, those past JJ when g == jg (JJ is a template
-// argument, g and jg are uniform).  The column groups jg run in a loop and the 16 steps of a
-// group are unrolled, so a group's code stays in the instruction cache across the groups.
-template <int JJ>
-struct CholGroupSteps {
-  __device__ __forceinline__ static void run(double (&s)[16], int i, int t, int g, int jg,
-                                             CholShared& sm) {
-    const int j = 16 * jg + JJ;
-    double* cj = sm.col[JJ & 1];
-    if (g == jg) cj[i] = s[JJ];
-    chol_lds_barrier();
-    if (t == 0) sm.piv[j] = cj[j];
-    if (g > jg) {
-      const double lij = cj[i] / cj[j];
-      const double* cg = cj + 16 * g;
+// Orders one wave's LDS accesses for the compiler.  A wave's LDS operations are carried out in
+// issue order, so a store read back by other lanes of the same wave needs no barrier.
+__device__ __forceinline__ void chol_wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// x[16 + 16 Q + jj] -= m[LL] B[LL][jj] for LL, jj < 16, B's rows kNb doubles apart in LDS (a
+// broadcast operand: every lane reads the same words).  LL runs in order, and row LL + 1 of B
+// is requested before the products of row LL, so that one wave keeps the LDS latency hidden.
+template <int Q>
+__device__ __forceinline__ void chol_rank16(double (&x)[kNb], const double (&mul)[16],
+                                            const double* __restrict__ B) {
+  double cur[16], nxt[16];
 #pragma unroll
-      for (int cc = 0; cc < 16; ++cc) s[cc] = fma(-lij, cg[cc], s[cc]);
-    } else if (g == jg && JJ < 15) {
-      const double lij = cj[i] / cj[j];
-      const double* cg = cj + 16 * g;
+  for (int jj = 0; jj < 16; ++jj) cur[jj] = B[jj];
 #pragma unroll
-      for (int cc = JJ + 1; cc < 16; ++cc) s[cc] = fma(-lij, cg[cc], s[cc]);
+  for (int LL = 0; LL < 16; ++LL) {
+    if (LL < 15) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) nxt[jj] = B[(LL + 1) * kNb + jj];
     }
-    CholGroupSteps<JJ + 1>::run(s, i, t, g, jg, sm);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) x[16 + 16 * Q + jj] = fma(-mul[LL], cur[jj], x[16 + 16 * Q + jj]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) cur[jj] = nxt[jj];
+  }
+}
+
+// Step JJ of column group jg (column j = 16 jg + JJ) of the one-wave factorisation.  Lane i
+// holds row i, columns 16 jg .. 63 in s[0 .. 63 - 16 jg].  Column j is broadcast through LDS;
+// the lane's multiplier l_ij = S[i][j] / S[j][j] updates the group's later columns at once,
+// S[i][c] -= l_ij S[c][j], and is kept for the columns past the group.  Rows above j hold
+// zeros in column j, so their multipliers are zero.  The pivot S[j][j] is kept by lane j.
+template <int JJ>
+struct CholWaveSteps {
+  __device__ __forceinline__ static void run(double (&s)[kNb], double (&l)[16], double& piv,
+                                             int i, int jg, double (*col)[kNb]) {
+    const double* cj = col[JJ] + 16 * jg;
+    col[JJ][i] = s[JJ];
+    chol_wave_fence();
+    const double d = cj[JJ];
+    piv = i == 16 * jg + JJ ? d : piv;
+    l[JJ] = s[JJ] / d;
+#pragma unroll
+    for (int cc = JJ + 1; cc < 16; ++cc) s[cc] = fma(-l[JJ], cj[cc], s[cc]);
+    CholWaveSteps<JJ + 1>::run(s, l, piv, i, jg, col);
   }
 };
 template <>
-struct CholGroupSteps<16> {
-  __device__ __forceinline__ static void run(double (&)[16], int, int, int, int, CholShared&) {}
+struct CholWaveSteps<16> {
+  __device__ __forceinline__ static void run(double (&)[kNb], double (&)[16], double&, int, int,
+                                             double (*)[kNb]) {}
 };
 
-// Factor the b x b block at D (b <= kNb).  Thread t holds row i = t % 64, columns
-// 16 g .. 16 g + 15 (g = t / 64, uniform in a wave) of the block; rows and columns past b are
-// the identity.  Step j broadcasts column j through LDS; column j itself is not written in
-// step j, so one barrier per step suffices.
+// Factor the b x b block at D (b <= kNb) into sm.lt and sm.rd on wave 0; rows and columns past
+// b are the identity.  Called by the whole workgroup (the other waves wait at the barrier).
 __device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, int64_t lda, int b,
                                                  CholShared& sm, int* info, int64_t k0,
                                                  bool report) {
-  const int t = threadIdx.x, i = t & 63;
-  const int g = __builtin_amdgcn_readfirstlane(t >> 6);
-  double s[16];
+  const int t = threadIdx.x;
+  if (t < 64) {
+    const int i = t;
+    // every lane loads from the block's lower triangle (clamped indices) and then selects: a
+    // load under a condition is issued in a branch of its own that waits for it
+    const int ib = min(i, b - 1);
+    const double* drow = D + (int64_t)ib * lda;
+    double s[kNb];
 #pragma unroll
-  for (int cc = 0; cc < 16; ++cc) {
-    const int c = 16 * g + cc;
-    s[cc] = (i < b && c <= i) ? D[(int64_t)i * lda + c] : (c == i ? 1.0 : 0.0);
-  }
+    for (int c = 0; c < kNb; ++c) {
+      const double v = drow[min(c, ib)];
+      s[c] = (i < b && c <= i) ? v : (c == i ? 1.0 : 0.0);
+    }
+    double piv = 1.0;
 #pragma unroll 1
-  for (int jg = 0; jg < 4; ++jg) CholGroupSteps<0>::run(s, i, t, g, jg, sm);
-  chol_lds_barrier();
-  if (report && t == 0) {
-    int bad = 0;
-    for (int j = b - 1; j >= 0; --j) bad = sm.piv[j] > 0.0 ? bad : j + 1;
-    if (bad) atomicCAS(info, 0, (int)(k0 + bad));
-  }
+    for (int jg = 0; jg < 4; ++jg) {
+      double l[16];
+      chol_wave_fence();  // the previous group's reads of col come before this group's stores
+      CholWaveSteps<0>::run(s, l, piv, i, jg, sm.col);
+      // the group's update of columns 16 (jg + 1 + Q) .. + 15, its 16 steps in column order
+      if (jg < 3) chol_rank16<0>(s, l, &sm.col[0][16 * jg + 16]);
+      if (jg < 2) chol_rank16<1>(s, l, &sm.col[0][16 * jg + 32]);
+      if (jg < 1) chol_rank16<2>(s, l, &sm.col[0][16 * jg + 48]);
 #pragma unroll
-  for (int cc = 0; cc < 16; ++cc) {
-    const int c = 16 * g + cc;
-    const double d = sqrt(sm.piv[c]);
-    sm.lt[c][i] = c < i ? s[cc] / d : (c == i ? d : 0.0);
+      for (int cc = 0; cc < 16; ++cc) sm.lt[16 * jg + cc][i] = s[cc];  // unscaled S[i][c]
+#pragma unroll
+      for (int c = 0; c < kNb - 16; ++c) s[c] = s[c + 16];
+    }
+    const unsigned long long bad = __ballot(i < b && !(piv > 0.0));
+    if (report && bad != 0ull && i == 0) atomicCAS(info, 0, (int)(k0 + __ffsll(bad)));
+    const double d = sqrt(piv);
+    sm.rd[i] = 1.0 / d;
+    chol_wave_fence();
+#pragma unroll 8
+    for (int c = 0; c < kNb; ++c) {
+      const double v = sm.lt[c][i];
+      sm.lt[c][i] = c < i ? v * sm.rd[c] : (c == i ? d : 0.0);
+    }
   }
-  if (t < kNb) sm.rd[t] = 1.0 / sqrt(sm.piv[t]);
   chol_lds_barrier();
 }
 
-// L11 of the factored block (sm.lt) to a side buffer, row-major 64 x 64.
-__device__ __forceinline__ void chol_store_l11(const CholShared& sm, double* __restrict__ l11) {
-  for (int e = threadIdx.x; e < kNb * kNb; e += kCholThreads) {
-    const int i = e >> 6, c = e & 63;
-    l11[e] = sm.lt[c][i];
-  }
+// L11 of the factored block (sm.lt) to a side buffer, transposed as sm.lt.
+__device__ __forceinline__ void chol_store_l11(const CholShared& sm, double* __restrict__ l11t) {
+  const double* src = &sm.lt[0][0];
+  for (int e = threadIdx.x; e < kNb * kNb; e += kCholThreads) l11t[e] = src[e];
 }
 
 __global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double* __restrict__ A,
                                                                  int64_t lda, int64_t m,
                                                                  int64_t k,
-                                                                 double* __restrict__ l11,
+                                                                 double* __restrict__ l11t,
                                                                  int* info) {
   __shared__ __align__(16) CholShared sm;
   chol_factor_diag(A + k * lda + k, lda, (int)min((int64_t)kNb, m - k), sm, info, k, true);
-  chol_store_l11(sm, l11);
+  chol_store_l11(sm, l11t);
 }
 
-// Step l = 16 lg + LL of the row solve x L11^T = a: the wave holding column l finishes x[l]
-// (times 1 / L11[l][l], kept in registers) and hands it over through LDS; the wave holding
-// columns 16 g .. 16 g + 15 then applies x[j] -= x[l] L11[j][l] for its columns j > l.  Groups
-// in a loop, steps of a group unrolled, as in the factorisation.
-template <int LL>
-struct CholPanelGroupSteps {
-  __device__ __forceinline__ static void run(double (&x)[16], const double (&rdv)[16], int lane,
-                                             int g, int lg, const double (*lt)[kNb],
-                                             double (*xs)[64]) {
-    const int l = 16 * lg + LL;
-    if (g == lg) {
-      x[LL] *= rdv[LL];
-      xs[LL & 1][lane] = x[LL];
-    }
-    chol_lds_barrier();
-    if (g > lg) {
-      const double v = xs[LL & 1][lane];
-      const double* row = lt[l] + 16 * g;
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) x[jj] = fma(-v, row[jj], x[jj]);
-    } else if (g == lg && LL < 15) {
-      const double v = xs[LL & 1][lane];
-      const double* row = lt[l] + 16 * g;
-#pragma unroll
-      for (int jj = LL + 1; jj < 16; ++jj) x[jj] = fma(-v, row[jj], x[jj]);
-    }
-    CholPanelGroupSteps<LL + 1>::run(x, rdv, lane, g, lg, lt, xs);
-  }
-};
-template <>
-struct CholPanelGroupSteps<16> {
-  __device__ __forceinline__ static void run(double (&)[16], const double (&)[16], int, int, int,
-                                             const double (*)[kNb], double (*)[64]) {}
-};
-
-// Panel rows k + kNb .. m - 1, 64 per block (launched only when there are some): lane = row,
-// wave g = columns 16 g .. 16 g + 15.
-__global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(double* __restrict__ A,
-                                                                  int64_t lda, int64_t m,
-                                                                  int64_t k,
-                                                                  const double* __restrict__ l11) {
+// Panel rows k + kNb .. m - 1, one wave per 64 rows (launched only when there are some): lane =
+// row, its 64 columns in registers, L11 in LDS.  The rows are independent, so x L11^T = a is
+// solved per lane without any exchange: column l is x[l] /= L11[l][l], then x[j] -= x[l]
+// L11[j][l] for j > l, in column groups of 16 as in the factorisation (the group's triangle,
+// then its update of the later columns; the same operations in the same order per column).
+__global__ __launch_bounds__(64) void chol_panel_kernel(double* __restrict__ A, int64_t lda,
+                                                        int64_t m, int64_t k,
+                                                        const double* __restrict__ l11t) {
   __shared__ __align__(16) double lt[kNb][kNb];  // lt[c][i] = L11[i][c]
   __shared__ double rd[kNb];                     // 1 / L11[j][j]
-  __shared__ double xs[2][64];                   // x[L] of the current step, double-buffered
-  const int t = threadIdx.x, lane = t & 63;
-  const int g = __builtin_amdgcn_readfirstlane(t >> 6);
-  for (int e = t; e < kNb * kNb; e += kCholThreads) {
-    const int i = e >> 6, c = e & 63;
-    lt[c][i] = l11[e];
-  }
-  if (t < kNb) rd[t] = 1.0 / l11[t * kNb + t];
+  const int lane = threadIdx.x;
   const int64_t r = k + kNb + (int64_t)blockIdx.x * 64 + lane;
   const bool live = r < m;
-  double* row = A + (live ? r : k) * lda + k + 16 * g;
-  double x[16];
+  double* row = A + (live ? r : k) * lda + k;
+  // every load in flight at once: the row, then L11 (staged to LDS when it has arrived)
+  double x[kNb];  // a lane past m works on row k (finite values) and stores nothing
 #pragma unroll
-  for (int jj = 0; jj < 16; ++jj) x[jj] = live ? row[jj] : 0.0;
-  __syncthreads();
-  double rdv[16];  // 1 / L11[j][j] for this wave's columns
+  for (int jj = 0; jj < kNb; ++jj) x[jj] = row[jj];
+  {
+    double lv[kNb];
 #pragma unroll
-  for (int jj = 0; jj < 16; ++jj) rdv[jj] = rd[16 * g + jj];
+    for (int c = 0; c < kNb; ++c) lv[c] = l11t[c * kNb + lane];
+    const double dg = l11t[lane * kNb + lane];
+#pragma unroll
+    for (int c = 0; c < kNb; ++c) lt[c][lane] = lv[c];
+    rd[lane] = 1.0 / dg;
+  }
+  chol_wave_fence();
 #pragma unroll 1
-  for (int lg = 0; lg < 4; ++lg) CholPanelGroupSteps<0>::run(x, rdv, lane, g, lg, lt, xs);
-  if (live) {
+  for (int lg = 0; lg < 4; ++lg) {
+    const double* L = &lt[16 * lg][16 * lg];  // L[LL * kNb + jj] = L11[16 lg + jj][16 lg + LL]
 #pragma unroll
-    for (int jj = 0; jj < 16; ++jj) row[jj] = x[jj];
+    for (int LL = 0; LL < 16; ++LL) {
+      x[LL] *= rd[16 * lg + LL];
+#pragma unroll
+      for (int jj = LL + 1; jj < 16; ++jj) x[jj] = fma(-x[LL], L[LL * kNb + jj], x[jj]);
+    }
+    double xl[16];
+#pragma unroll
+    for (int LL = 0; LL < 16; ++LL) xl[LL] = x[LL];
+    if (lg < 3) chol_rank16<0>(x, xl, L + 16);
+    if (lg < 2) chol_rank16<1>(x, xl, L + 32);
+    if (lg < 1) chol_rank16<2>(x, xl, L + 48);
+    if (live) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) row[16 * lg + jj] = x[jj];
+    }
+#pragma unroll
+    for (int jj = 0; jj < kNb - 16; ++jj) x[jj] = x[jj + 16];
   }
 }
 
 __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __restrict__ A,
                                                                    int64_t lda, int64_t m,
                                                                    int64_t k,
-                                                                   const double* __restrict__ l11,
-                                                                   double* __restrict__ l11_next,
+                                                                   const double* __restrict__ l11t,
+                                                                   double* __restrict__ l11t_next,
                                                                    int* info) {
   constexpr int LS = kNb + 1;  // odd row stride: the transposed stores spread over the banks
   // the two transposed panel tiles; the first tile's block reuses the space to factor
@@ -319,9 +345,13 @@ __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __res
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
     const int b = (int)min((int64_t)kNb, m - k);
-    for (int e = t; e < kNb * kNb; e += kCholThreads) {
-      const int i = e >> 6, c = e & 63;
-      if (i < b && c <= i) A[(k + i) * lda + k + c] = l11[e];
+    double v[kNb * kNb / kCholThreads];
+#pragma unroll
+    for (int n = 0; n < kNb * kNb / kCholThreads; ++n) v[n] = l11t[t + n * kCholThreads];
+#pragma unroll
+    for (int n = 0; n < kNb * kNb / kCholThreads; ++n) {
+      const int e = t + n * kCholThreads, c = e >> 6, i = e & 63;
+      if (i < b && c <= i) A[(k + i) * lda + k + c] = v[n];
     }
     return;
   }
@@ -337,8 +367,10 @@ __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __res
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
     const int rr = (t >> 6) + 4 * n, kk = t & 63;
-    va[n] = r0 + rr < m ? A[(r0 + rr) * lda + k + kk] : 0.0;
-    vb[n] = c0 + rr < m ? A[(c0 + rr) * lda + k + kk] : 0.0;
+    // clamped rows, loaded unconditionally (a load under a condition gets a branch of its own
+    // that waits for it): a row past m only reaches outputs past m, which are not stored
+    va[n] = A[min(r0 + rr, m - 1) * lda + k + kk];
+    vb[n] = A[min(c0 + rr, m - 1) * lda + k + kk];
   }
   // the tile being updated, requested before the product
   double cv[4][4];
@@ -348,7 +380,8 @@ __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __res
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int64_t c = c0 + 4 * tx + v;
-      cv[u][v] = (rw < m && c <= rw) ? A[rw * lda + c] : 0.0;
+      const int64_t rwc = min(rw, m - 1);  // in the lower triangle; stored only if c <= rw < m
+      cv[u][v] = A[rwc * lda + min(c, rwc)];
     }
   }
 #pragma unroll
@@ -391,7 +424,7 @@ __global__ __launch_bounds__(kCholThreads) void chol_update_kernel(double* __res
   __syncthreads();
   CholShared& sm = *reinterpret_cast<CholShared*>(smem);
   chol_factor_diag(A + k1 * lda + k1, lda, (int)min((int64_t)kNb, m - k1), sm, info, k1, true);
-  chol_store_l11(sm, l11_next);
+  chol_store_l11(sm, l11t_next);
 }
 
 __global__ void chol_info_reset_kernel(int* info) {
@@ -493,8 +526,8 @@ MAOMI_API int maomi_cholesky_f64(double* A, int64_t lda, int64_t m, double* work
     double* cur = buf[step & 1];
     double* next = buf[(step + 1) & 1];
     if (mp > 0) {
-      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((mp + 63) / 64)), dim3(kCholThreads), 0,
-                         st, A, lda, m, k, cur);
+      hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)((mp + 63) / 64)), dim3(64), 0, st, A,
+                         lda, m, k, cur);
       MAOMI_CHECK_LAUNCH();
     }
     const int64_t T = (mp + kNb - 1) / kNb;
@@ -563,21 +596,26 @@ __global__ __launch_bounds__(kTsThreads) void trs_step_kernel(const double* __re
   const int kc = (R - g + 3) / 4;                       // columns of this wave (g + 4 k < R)
   // Tiles are staged with every load of a thread in flight at once (unrolled into registers,
   // then stored to LDS): a loop that waits for each load before the next one is issued spends
-  // 16 memory latencies per tile.
+  // 16 memory latencies per tile.  So does a load under a condition (it gets a branch of its
+  // own that waits for it): the loads use clamped, in-bounds indices and the values are
+  // selected afterwards.
   constexpr int kPer = kTs * kTs / kTsThreads;  // tile elements per thread
   double dv[kPer];
   if (blockIdx.x == 0) {  // requested first: its loads overlap the update's
 #pragma unroll
     for (int it = 0; it < kPer; ++it) {
       const int e = tid + it * kTsThreads, i = e / kTs, j = e % kTs;
-      dv[it] = (i < rows && j <= i) ? L[(r0 + i) * ldl + r0 + j] : (i == j) ? 1.0 : 0.0;
+      const int ic = min(i, rows - 1);
+      dv[it] = L[(r0 + ic) * ldl + r0 + min(j, ic)];  // padded when staged
     }
   }
   double b[kTsCols];
 #pragma unroll
   for (int k = 0; k < kTsCols; ++k) {
     const int c = g + 4 * k;
-    b[k] = (lane < rows && c < R) ? B[(r0 + lane) * R + c] : 0.0;
+    // clamped (finite) values for lanes past the block and columns past R: those are never
+    // stored, and in the diagonal solve they meet only the zero entries of the padded block
+    b[k] = B[(r0 + min(lane, rows - 1)) * R + min(c, R - 1)];
   }
   if (!diag_only) {
     const int64_t p0 = step * kTs;                               // rows of the solved panel
@@ -588,15 +626,17 @@ __global__ __launch_bounds__(kTsThreads) void trs_step_kernel(const double* __re
 #pragma unroll
     for (int it = 0; it < kPer; ++it) {
       const int e = tid + it * kTsThreads, i = e / kTs, j = e % kTs;
+      // rows past m are clamped: forward, they reach only this block's rows past m; backward,
+      // they meet the zero rows of the staged panel
       if (FWD)  // tile[i][j] = L[r0 + i][p0 + j]: this block's rows, the panel's columns
-        tv[it] = i < rows ? L[(r0 + i) * ldl + p0 + j] : 0.0;
+        tv[it] = L[(r0 + min(i, rows - 1)) * ldl + p0 + j];
       else      // tile[j][i] = L[p0 + j][r0 + i]: the panel's rows, this block's columns
-        tv[it] = i < bp ? L[(p0 + i) * ldl + r0 + j] : 0.0;
+        tv[it] = L[(p0 + min(i, bp - 1)) * ldl + r0 + j];
     }
 #pragma unroll
     for (int it = 0; it < kPan; ++it) {
       const int e = tid + it * kTsThreads, j = e / kMaxRhs, c = e % kMaxRhs;
-      pv[it] = (j < bp && c < R) ? B[(p0 + j) * R + c] : 0.0;
+      pv[it] = B[(p0 + min(j, bp - 1)) * R + min(c, R - 1)];
     }
 #pragma unroll
     for (int it = 0; it < kPer; ++it) {
@@ -605,8 +645,8 @@ __global__ __launch_bounds__(kTsThreads) void trs_step_kernel(const double* __re
     }
 #pragma unroll
     for (int it = 0; it < kPan; ++it) {
-      const int e = tid + it * kTsThreads;
-      sm.pan[e / kMaxRhs][e % kMaxRhs] = pv[it];
+      const int e = tid + it * kTsThreads, j = e / kMaxRhs, c = e % kMaxRhs;
+      sm.pan[j][c] = (j < bp && c < R) ? pv[it] : 0.0;
     }
     __syncthreads();
 #pragma unroll 8
@@ -620,8 +660,8 @@ __global__ __launch_bounds__(kTsThreads) void trs_step_kernel(const double* __re
   if (blockIdx.x == 0) {
 #pragma unroll
     for (int it = 0; it < kPer; ++it) {
-      const int e = tid + it * kTsThreads;
-      sm.diag[e / kTs][e % kTs] = dv[it];
+      const int e = tid + it * kTsThreads, i = e / kTs, j = e % kTs;
+      sm.diag[i][j] = (i < rows && j <= i) ? dv[it] : (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();  // the diagonal block is staged
     double d[kTs];    // forward: d[j] = D[lane][j]; backward: d[j] = D[j][lane]
