@@ -165,6 +165,15 @@ __device__ __forceinline__ void chol_wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
+// Lane l's value of a wave-uniform lane index.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+
 // x[16 + 16 Q + jj] -= m[LL] B[LL][jj] for LL, jj < 16, B's rows kNb doubles apart in LDS (a
 // broadcast operand: every lane reads the same words).  LL runs in order, and row LL + 1 of B
 // is requested before the products of row LL, so that one wave keeps the LDS latency hidden.
@@ -190,22 +199,23 @@ __device__ __forceinline__ void chol_rank16(double (&x)[kNb], const double (&mul
 }
 
 // Step JJ of column group jg (column j = 16 jg + JJ) of the one-wave factorisation.  Lane i
-// holds row i, columns 16 jg .. 63 in s[0 .. 63 - 16 jg].  Column j is broadcast through LDS;
-// the lane's multiplier l_ij = S[i][j] / S[j][j] updates the group's later columns at once,
-// S[i][c] -= l_ij S[c][j], and is kept for the columns past the group.  Rows above j hold
-// zeros in column j, so their multipliers are zero.  The pivot S[j][j] is kept by lane j.
+// holds row i, columns 16 jg .. 63 in s[0 .. 63 - 16 jg].  The pivot S[j][j] and the group's
+// rows of column j, S[16 jg + cc][j], are read from their lanes (v_readlane: no LDS round trip
+// on the chain of dependent steps); the lane's multiplier l_ij = S[i][j] / S[j][j] updates the
+// group's later columns at once, S[i][c] -= l_ij S[c][j], and is kept for the columns past
+// the group, whose rows of column j go through LDS for that update.  Rows above j hold zeros
+// in column j, so their multipliers are zero.  The pivot is kept by lane j.
 template <int JJ>
 struct CholWaveSteps {
   __device__ __forceinline__ static void run(double (&s)[kNb], double (&l)[16], double& piv,
                                              int i, int jg, double (*col)[kNb]) {
-    const double* cj = col[JJ] + 16 * jg;
     col[JJ][i] = s[JJ];
-    chol_wave_fence();
-    const double d = cj[JJ];
+    const double d = readlane_f64(s[JJ], 16 * jg + JJ);
     piv = i == 16 * jg + JJ ? d : piv;
     l[JJ] = s[JJ] / d;
 #pragma unroll
-    for (int cc = JJ + 1; cc < 16; ++cc) s[cc] = fma(-l[JJ], cj[cc], s[cc]);
+    for (int cc = JJ + 1; cc < 16; ++cc)
+      s[cc] = fma(-l[JJ], readlane_f64(s[JJ], 16 * jg + cc), s[cc]);
     CholWaveSteps<JJ + 1>::run(s, l, piv, i, jg, col);
   }
 };
@@ -239,6 +249,7 @@ __device__ __forceinline__ void chol_factor_diag(const double* __restrict__ D, i
       double l[16];
       chol_wave_fence();  // the previous group's reads of col come before this group's stores
       CholWaveSteps<0>::run(s, l, piv, i, jg, sm.col);
+      chol_wave_fence();  // the group's columns are in LDS before the trailing update reads them
       // the group's update of columns 16 (jg + 1 + Q) .. + 15, its 16 steps in column order
       if (jg < 3) chol_rank16<0>(s, l, &sm.col[0][16 * jg + 16]);
       if (jg < 2) chol_rank16<1>(s, l, &sm.col[0][16 * jg + 32]);
@@ -564,14 +575,6 @@ struct TrsShared {
   double pan[kTs][kMaxRhs];   // solved rows of the last panel
 };
 
-// Lane l's value of a wave-uniform lane index.
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
-  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
-                                          (unsigned long long)(unsigned)lo));
-}
 
 // Sweep step of L Y = B (forward) or L^T X = Y (backward) on B (m x R, row-major).
 // Forward, panel s = step: blocks update rows r0 = (s + 1 + k) kTs ...; backward, panel p =
