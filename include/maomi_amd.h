@@ -339,6 +339,19 @@ int maomi_v4_apply_split_f64(const double* points, const double* coef, const dou
                              double* out, int64_t P, int64_t Q, int64_t C, double* partial,
                              int64_t rows, maomi_stream_t stream);
 int64_t maomi_v4_apply_rows(int64_t P, int64_t Q);
+/* maomi_v4_apply_split_f64 for points on a complete nx x ny tensor grid and queries on a
+ * complete mx x my tensor grid, with G from a table instead of a logarithm per pair (replaces
+ * green_apply, maomi/backends/cuda/interpolation.cu:3-19, on such grids).  du: the U <= 65536
+ * distinct axis distances |dx| and |dy| (device); table: U * U doubles of workspace, filled
+ * here with G(du[u], du[v]); ix (mx * nx int32): index into du of |qx_a - px_j|; iy
+ * (my * ny int32, ny <= 128): of |qy_b - py_k|; every index in [0, U).  coef: nx * ny * C
+ * doubles, point (j, k) at row j * ny + k; out: (mx * my) x C, query q = (a, b) at
+ * a * my + b.  The point columns are split over `rows` block rows (clamped to nx) whose
+ * partial sums (rows * mx * my * C doubles) are added in row order. */
+int maomi_v4_apply_grid_f64(const double* du, int64_t U, double* table, const int32_t* ix,
+                            const int32_t* iy, const double* coef, double* out, int64_t nx,
+                            int64_t ny, int64_t mx, int64_t my, int64_t C, double* partial,
+                            int64_t rows, maomi_stream_t stream);
 /* maomi_v4_apply_split_f64 that allocates nothing and takes no partial-sum buffer.  No
  * counterpart in the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops).  The partial sums live in a static device

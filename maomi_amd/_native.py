@@ -182,6 +182,9 @@ _SIGS = {
     "maomi_v4_apply_split_f64_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
                                                  c_int64, c_int64, c_int64, c_int64, c_void_p]),
     "maomi_v4_apply_rows": (c_int64, [c_int64, c_int64]),
+    "maomi_v4_apply_grid_f64": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64,
+                                        c_int64, c_void_p, c_int64, c_void_p]),
     "maomi_cholesky_f64": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
     "maomi_cholesky_f64_workspace": (c_int64, []),
     "maomi_v4_finish_f64": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,

@@ -94,6 +94,61 @@ __global__ __launch_bounds__(256) void green_apply_kernel(const double* __restri
   }
 }
 
+// G on a table, for points on a complete nx x ny tensor grid and queries on a complete
+// mx x my tensor grid: every |dx| and |dy| is one of U distances du, so G(dx, dy) =
+// T[u * U + v] with u, v the indices of |dx| and |dy| (green_table_kernel fills T with the
+// same green_r2).  Query q = (a, b) with a = q / my, b = q % my; ix[a * nx + j] indexes
+// |qx_a - px_j|, iy[b * ny + k] indexes |qy_b - py_k|; coef is in grid order, point (j, k) at
+// j * ny + k.  A pair costs an index read from LDS, one gather of T and C fused multiply-adds
+// instead of a logarithm.  Point columns [j0, j1) go to block row blockIdx.y, as in
+// green_apply_kernel.
+__global__ void green_table_kernel(const double* __restrict__ du, int64_t U,
+                                   double* __restrict__ T) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < U * U;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double a = du[e / U], b = du[e % U];
+    T[e] = green_r2(a * a + b * b);
+  }
+}
+
+constexpr int kGridMaxNy = 128;  // point rows staged in LDS per query (16-bit indices)
+
+// yl: ny x 256 16-bit indices in dynamic LDS, this block's queries' rows of iy
+
+template <int C>
+__global__ __launch_bounds__(256) void green_grid_kernel(
+    const double* __restrict__ T, int64_t U, const int* __restrict__ ix,
+    const int* __restrict__ iy, const double* __restrict__ coef, double* __restrict__ out,
+    int nx, int ny, int64_t mx, int64_t my, int jchunk) {
+  extern __shared__ unsigned short yl[];
+  const int tid = threadIdx.x;
+  const int64_t Q = mx * my, q = blockIdx.x * (int64_t)blockDim.x + tid;
+  const bool live = q < Q;
+  const int64_t qq = live ? q : Q - 1;
+  const int64_t a = qq / my, b = qq % my;
+  for (int k = 0; k < ny; ++k) yl[k * 256 + tid] = (unsigned short)iy[b * ny + k];
+  __syncthreads();
+  const int j0 = blockIdx.y * jchunk, j1 = min(nx, j0 + jchunk);
+  out += (int64_t)blockIdx.y * Q * C;
+  double acc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) acc[c] = 0.0;
+  for (int j = j0; j < j1; ++j) {
+    const double* trow = T + (int64_t)ix[a * nx + j] * U;
+    const double* cj = coef + (int64_t)j * ny * C;
+#pragma unroll 8
+    for (int k = 0; k < ny; ++k) {
+      const double g = trow[yl[k * 256 + tid]];
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[c] = fma(g, cj[k * C + c], acc[c]);
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) out[q * C + c] = acc[c];
+  }
+}
+
 // out[e] = sum over rows r < R of part[r][e], in row order
 __global__ void reduce_chunks_kernel(const double* __restrict__ part, int R, int64_t n,
                                      double* __restrict__ out) {
@@ -508,6 +563,47 @@ MAOMI_API int maomi_v4_apply_split_f64(const double* points, const double* coef,
     const int64_t n = Q * C;
     const unsigned b = (unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535);
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3(b), dim3(256), 0, st, partial, (int)rows, n,
+                       out);
+    MAOMI_CHECK_LAUNCH();
+  }
+  return MAOMI_OK;
+}
+
+MAOMI_API int maomi_v4_apply_grid_f64(const double* du, int64_t U, double* table,
+                                      const int32_t* ix, const int32_t* iy, const double* coef,
+                                      double* out, int64_t nx, int64_t ny, int64_t mx,
+                                      int64_t my, int64_t C, double* partial, int64_t rows,
+                                      maomi_stream_t stream) {
+  if (!du || !table || !ix || !iy || !coef || !out || U < 1 || U > 65536 || nx < 1 || ny < 1 ||
+      ny > kGridMaxNy || nx > INT32_MAX / (ny > 0 ? ny : 1) || mx < 1 || my < 1 || C < 1 ||
+      C > kMaxC || rows < 1 || rows > 65535 || (rows > 1 && !partial) ||
+      (mx * my + 255) / 256 > INT32_MAX) {
+    maomi_set_error("maomi_v4_apply_grid_f64: bad arguments (1 <= C <= %d, ny <= %d, U <= 65536)",
+                    kMaxC, kGridMaxNy);
+    return MAOMI_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t tot = U * U;
+  const unsigned tb = (unsigned)((tot + 255) / 256 < 65535 ? (tot + 255) / 256 : 65535);
+  hipLaunchKernelGGL(green_table_kernel, dim3(tb), dim3(256), 0, st, du, U, table);
+  MAOMI_CHECK_LAUNCH();
+  const int64_t Q = mx * my;
+  const int64_t rr = rows < nx ? rows : nx;  // at least one point column per block row
+  const int jchunk = (int)((nx + rr - 1) / rr);
+  const dim3 grid((unsigned)((Q + 255) / 256), (unsigned)((nx + jchunk - 1) / jchunk));
+  double* dst = grid.y > 1 ? partial : out;
+  const size_t lds = (size_t)ny * 256 * sizeof(unsigned short);
+  switch (C) {
+#define CASE(c) \
+  case c: hipLaunchKernelGGL(green_grid_kernel<c>, grid, dim3(256), lds, st, table, U, ix, iy, coef, dst, (int)nx, (int)ny, mx, my, jchunk); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+  }
+  MAOMI_CHECK_LAUNCH();
+  if (grid.y > 1) {
+    const int64_t n = Q * C;
+    const unsigned b = (unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(b), dim3(256), 0, st, partial, (int)grid.y, n,
                        out);
     MAOMI_CHECK_LAUNCH();
   }

@@ -503,6 +503,69 @@ def v4_apply(p, coef, q, out, device):
     return out
 
 
+# The table of G for grid evaluation holds U^2 doubles for U distinct axis distances.
+_GRID_MAX_U = 2048
+_GRID_MAX_NY = 128
+
+
+def point_grid(points):
+    """(xs, ys, order) when the points are a complete tensor grid of their distinct x and y
+    values, point order[j * len(ys) + k] sitting at (xs[j], ys[k]); else None."""
+    pts = np.asarray(points, np.float64)
+    if pts.ndim != 2 or pts.shape[1] != 2 or len(pts) < 2:
+        return None
+    xs, jx = np.unique(pts[:, 0], return_inverse=True)
+    ys, ky = np.unique(pts[:, 1], return_inverse=True)
+    if len(xs) * len(ys) != len(pts):
+        return None
+    cell = jx.ravel() * len(ys) + ky.ravel()
+    if np.unique(cell).size != len(pts):
+        return None
+    order = np.empty(len(pts), np.int64)
+    order[cell] = np.arange(len(pts))
+    return xs, ys, order
+
+
+def grid_tables(grid, qx, qy):
+    """Distinct axis distances du and the index tables of maomi_v4_apply_grid_f64 for points
+    on `grid` (point_grid) and queries (qx[a], qy[b]); None past the table's bounds."""
+    xs, ys, _ = grid
+    if len(ys) > _GRID_MAX_NY:
+        return None
+    dx = np.abs(np.asarray(qx, np.float64)[:, None] - xs[None, :])
+    dy = np.abs(np.asarray(qy, np.float64)[:, None] - ys[None, :])
+    du, inv = np.unique(np.concatenate([dx.ravel(), dy.ravel()]), return_inverse=True)
+    if len(du) > _GRID_MAX_U:
+        return None
+    inv = inv.ravel().astype(np.int32)
+    return du, inv[:dx.size], inv[dx.size:]
+
+
+def v4_apply_grid(grid, coef, qx, qy, out, device):
+    """out (len(qx) len(qy), C) = sum_i G(|q - p_i|) coef[i] for points on `grid` (point_grid,
+    coef in the points' order) and queries q = (qx[a], qy[b]) at row a len(qy) + b, with G
+    from a table (maomi_v4_apply_grid_f64).  Returns out, or None when the grid is outside the
+    table's bounds (the caller then uses v4_apply)."""
+    tabs = grid_tables(grid, qx, qy)
+    if tabs is None:
+        return None
+    du, ix, iy = tabs
+    xs, ys, order = grid
+    nx, ny, mx, my = len(xs), len(ys), len(qx), len(qy)
+    C, U = coef.shape[1], len(du)
+    lib = nat.lib()
+    cg = coef.index_select(0, _upload(order, device)).contiguous()
+    d_du, d_ix, d_iy = (_upload(a, device) for a in (du, ix, iy))
+    table = torch.empty(U * U, dtype=torch.float64, device=device)
+    rows = min(int(lib.maomi_v4_apply_rows(nx * ny, mx * my)), nx)
+    part = torch.empty(rows * mx * my * C if rows > 1 else 0, dtype=torch.float64, device=device)
+    nat.check(lib.maomi_v4_apply_grid_f64(nat.ptr(d_du), U, nat.ptr(table), nat.ptr(d_ix),
+                                          nat.ptr(d_iy), nat.ptr(cg), nat.ptr(out), nx, ny, mx,
+                                          my, C, nat.ptr(part) if rows > 1 else None, rows,
+                                          nat.stream_handle(device)), "v4 apply on a grid")
+    return out
+
+
 def v4_gpu(points, values, flat_queries, device, chunk=1 << 20, prepared=None):
     """Device FP64 v4: returns a (Q, C) float64 tensor on `device`.  prepared: a v4_prepare
     result for these points (e.g. from V4Ahead.take())."""

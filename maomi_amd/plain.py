@@ -110,7 +110,7 @@ def base_beams(u, bp, dz_focus, edge2d, tw, out, device):
 def v4_masks(points, values, shape, device, prepared=None):
     """MATLAB v4 on the GPU in FP64: solve G c = v, evaluate on pixel centres (numerics.py:93-153).
     prepared: a v4_prepare result for these points."""
-    from .numerics import v4_apply, v4_coefficients
+    from .numerics import point_grid, v4_apply, v4_apply_grid, v4_coefficients
     v = torch.as_tensor(values, dtype=torch.float64, device=device)
     if v.ndim == 1:
         v = v[:, None]
@@ -118,12 +118,16 @@ def v4_masks(points, values, shape, device, prepared=None):
     if P == 1:
         return v[0].expand(shape[0], shape[1], C).clone()
     p, G, factor = prepared if prepared is not None else v4_prepare(points, device)
-    xs = torch.arange(shape[0], dtype=torch.float64, device=device) + 0.5
-    ys = torch.arange(shape[1], dtype=torch.float64, device=device) + 0.5
-    q = torch.stack(torch.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2).contiguous()
     coef = v4_coefficients(p, G, v, factor=factor)
-    out = torch.empty((q.shape[0], C), dtype=torch.float64, device=device)
-    v4_apply(p, coef, q, out, device)
+    out = torch.empty((shape[0] * shape[1], C), dtype=torch.float64, device=device)
+    # pixel centres; on a lattice of points G comes from a table of the few axis distances
+    qx, qy = np.arange(shape[0]) + 0.5, np.arange(shape[1]) + 0.5
+    grid = point_grid(points)
+    if grid is None or v4_apply_grid(grid, coef, qx, qy, out, device) is None:
+        xs = torch.as_tensor(qx, dtype=torch.float64, device=device)
+        ys = torch.as_tensor(qy, dtype=torch.float64, device=device)
+        q = torch.stack(torch.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2).contiguous()
+        v4_apply(p, coef, q, out, device)
     return out.reshape(shape[0], shape[1], C)
 
 

@@ -74,3 +74,51 @@ def test_cholesky_of_empty_matrix(gpu_lib):
     import torch
     from maomi_amd.numerics import cholesky_f64_
     assert cholesky_f64_(torch.empty((0, 0), dtype=torch.float64, device="cuda")) == 0
+
+
+@pytest.mark.parametrize("case", ["config2_lattice", "shuffled_uneven_grid", "narrow_rows"])
+def test_v4_apply_on_a_grid_matches_the_per_pair_kernel_and_direct_sum(gpu_lib, case):
+    """maomi_v4_apply_grid_f64 (G from a table of axis distances) against the per-pair kernel
+    on the same inputs and a direct FP64 sum; the sums run in another point order, so the bar
+    is relative to the sum of the magnitudes of the terms."""
+    import torch
+    from maomi_amd.numerics import point_grid, v4_apply, v4_apply_grid
+    rng = np.random.default_rng(len(case))
+    if case == "config2_lattice":  # the 65 x 65 sites and the 512^2 pixel centres
+        from maomi_amd.workloads import CONFIG2, site_offsets
+        p = site_offsets(CONFIG2).astype(np.float64)
+        qx = qy = np.arange(512) + 0.5
+        C = 1
+    elif case == "shuffled_uneven_grid":  # uneven spacings, points in random order, C = 3
+        xs = np.sort(rng.choice(np.arange(0.0, 300.0, 0.5), 23, replace=False))
+        ys = np.sort(rng.choice(np.arange(0.0, 300.0, 0.5), 17, replace=False))
+        p = np.stack(np.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2)[rng.permutation(23 * 17)]
+        qx, qy = np.arange(40) * 7.5 + 0.25, np.arange(33) * 9.0 + 0.5
+        C = 3
+    else:  # one point column per block row, a query grid narrower than a block
+        xs, ys = np.arange(3) * 10.0, np.arange(90) * 2.0
+        p = np.stack(np.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2)
+        qx, qy = np.arange(5) + 0.5, np.arange(7) * 3.0
+        C = 2
+    grid = point_grid(p)
+    assert grid is not None
+    coef = rng.standard_normal((len(p), C))
+    dev = torch.device("cuda")
+    tc = torch.as_tensor(coef, device=dev)
+    q = np.stack(np.meshgrid(qx, qy, indexing="ij"), -1).reshape(-1, 2)
+    out = torch.empty((len(q), C), dtype=torch.float64, device=dev)
+    print(f"[v4 grid] {case} P={len(p)} Q={len(q)} C={C}", flush=True)
+    assert v4_apply_grid(grid, tc, qx, qy, out, dev) is out
+    ref_dev = torch.empty_like(out)
+    v4_apply(torch.as_tensor(p, device=dev), tc, torch.as_tensor(q, device=dev), ref_dev, dev)
+    got, per_pair = out.cpu().numpy(), ref_dev.cpu().numpy()
+    if len(q) * len(p) <= 2e7:
+        d2 = ((q[:, None, :] - p[None, :, :]) ** 2).sum(-1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            g = d2 * (0.5 * np.log(d2) - 1)
+        g[d2 == 0] = 0
+        bound = np.abs(g) @ np.abs(coef)
+        assert (np.abs(got - g @ coef) <= 1e-13 * bound + 1e-300).all()
+    else:
+        bound = np.abs(per_pair).max() * len(p)
+    assert (np.abs(got - per_pair) <= 1e-13 * bound + 1e-300).all()

@@ -52,3 +52,26 @@ def test_lu_library_avoids_magma_off_the_default_stream():
     from maomi_amd.numerics import lu_library
     assert lu_library(True) == "magma"
     assert lu_library(False) == "cusolver"
+
+
+def test_point_grid_and_distance_tables():
+    from maomi_amd.numerics import grid_tables, point_grid
+    rng = np.random.default_rng(3)
+    xs, ys = np.array([0.0, 2.5, 9.0]), np.array([1.0, 4.0])
+    full = np.stack(np.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2)
+    perm = rng.permutation(len(full))
+    gx, gy, order = point_grid(full[perm])
+    np.testing.assert_array_equal(gx, xs)
+    np.testing.assert_array_equal(gy, ys)
+    # order[j * len(ys) + k] is the index of the point at (xs[j], ys[k])
+    np.testing.assert_array_equal(full[perm][order], full)
+    assert point_grid(full[:-1]) is None                          # incomplete grid
+    assert point_grid(np.concatenate([full[:-1], full[:1]])) is None  # a duplicate
+    qx, qy = np.array([0.5, 3.0]), np.array([0.0, 1.0, 7.5])
+    du, ix, iy = grid_tables((xs, ys, order), qx, qy)
+    np.testing.assert_array_equal(du[ix].reshape(2, 3), np.abs(qx[:, None] - xs[None, :]))
+    np.testing.assert_array_equal(du[iy].reshape(3, 2), np.abs(qy[:, None] - ys[None, :]))
+    assert np.all(np.diff(du) > 0) and ix.dtype == np.int32
+    # more than 128 point rows: the table path declines
+    many = np.stack(np.meshgrid([0.0, 1.0], np.arange(129.0), indexing="ij"), -1).reshape(-1, 2)
+    assert grid_tables(point_grid(many), qx, qy) is None
