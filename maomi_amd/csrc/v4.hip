@@ -113,9 +113,10 @@ __global__ void green_table_kernel(const double* __restrict__ du, int64_t U,
 
 constexpr int kGridMaxNy = 128;  // point rows staged in LDS per query (16-bit indices)
 
-// yl: ny x 256 16-bit indices in dynamic LDS, this block's queries' rows of iy
-
-template <int C>
+// yl: ny x 256 16-bit indices in dynamic LDS, this block's queries' rows of iy.  ROW: the
+// block's 256 queries share the query row a (my a multiple of 256), so each point column's
+// row of T (U doubles, after yl in LDS) is staged once and read from LDS instead of L2.
+template <int C, bool ROW>
 __global__ __launch_bounds__(256) void green_grid_kernel(
     const double* __restrict__ T, int64_t U, const int* __restrict__ ix,
     const int* __restrict__ iy, const double* __restrict__ coef, double* __restrict__ out,
@@ -130,11 +131,18 @@ __global__ __launch_bounds__(256) void green_grid_kernel(
   __syncthreads();
   const int j0 = blockIdx.y * jchunk, j1 = min(nx, j0 + jchunk);
   out += (int64_t)blockIdx.y * Q * C;
+  double* tl = reinterpret_cast<double*>(yl + ny * 256);  // ny * 512 bytes: 8-byte aligned
   double acc[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) acc[c] = 0.0;
   for (int j = j0; j < j1; ++j) {
     const double* trow = T + (int64_t)ix[a * nx + j] * U;
+    if (ROW) {
+      __syncthreads();  // the previous row's reads are done
+      for (int e = tid; e < U; e += 256) tl[e] = trow[e];
+      __syncthreads();
+      trow = tl;
+    }
     const double* cj = coef + (int64_t)j * ny * C;
 #pragma unroll 8
     for (int k = 0; k < ny; ++k) {
@@ -592,10 +600,14 @@ MAOMI_API int maomi_v4_apply_grid_f64(const double* du, int64_t U, double* table
   const int jchunk = (int)((nx + rr - 1) / rr);
   const dim3 grid((unsigned)((Q + 255) / 256), (unsigned)((nx + jchunk - 1) / jchunk));
   double* dst = grid.y > 1 ? partial : out;
-  const size_t lds = (size_t)ny * 256 * sizeof(unsigned short);
-  switch (C) {
+  // the table row in LDS when every block's queries share one query row and it fits
+  const size_t idx_lds = (size_t)ny * 256 * sizeof(unsigned short);
+  const bool row = my % 256 == 0 && idx_lds + (size_t)U * sizeof(double) <= 65536;
+  const size_t lds = idx_lds + (row ? (size_t)U * sizeof(double) : 0);
+  switch (C * 2 + (row ? 1 : 0)) {
 #define CASE(c) \
-  case c: hipLaunchKernelGGL(green_grid_kernel<c>, grid, dim3(256), lds, st, table, U, ix, iy, coef, dst, (int)nx, (int)ny, mx, my, jchunk); break;
+  case 2 * c: hipLaunchKernelGGL((green_grid_kernel<c, false>), grid, dim3(256), lds, st, table, U, ix, iy, coef, dst, (int)nx, (int)ny, mx, my, jchunk); break; \
+  case 2 * c + 1: hipLaunchKernelGGL((green_grid_kernel<c, true>), grid, dim3(256), lds, st, table, U, ix, iy, coef, dst, (int)nx, (int)ny, mx, my, jchunk); break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
   }
